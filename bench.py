@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark of the reionization grid path on one MI355X.
+
+One step = one find_HII_bubbles call (forward r2c of deltax/stars/weighted_sfr, every filter
+radius from R_max = 50 Mpc/h down to the cell scale, the final FP64 globals) on the 512^3,
+400 cMpc/h seeded synthetic workload with the sharp-k filter, inputs resident in HBM.
+Metric: cell-radius updates/s = N^3 x n_radii / (time per call).
+
+Prints ONE JSON line.  Besides the contract fields it carries
+  roofline     : HBM roofline of the dominant kernel class (live hipEvent timing inside the
+                 library, algorithmic bytes per launch from the kept-mode geometry),
+  plain        : the same workload through the one-kernel-per-step PLAIN path,
+  cpu_baseline : the oracle's FP32 restatement timed on this host on a bounded sample,
+  parity       : xH of the fused vs plain path (bit-identical fields are expected).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(REPO, "meraxes-devs_qyxnew_amd")
+sys.path.insert(0, PKG)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_state(torch, n):
+    import mxr
+
+    return mxr.HiiState(torch.ones(n, device="cuda"), torch.full((n,), -1.0, device="cuda"),
+                        torch.zeros(n, device="cuda"), torch.zeros(n, device="cuda"), torch.zeros(n, device="cuda"))
+
+
+def reset_state(st):
+    st.xH.fill_(1.0)
+    st.z_at_ionization.fill_(-1.0)
+    st.J_21_at_ionization.zero_()
+    st.r_bubble.zero_()
+    st.temp_kinetic_all_gas.zero_()
+
+
+def cpu_baseline(N, box, inputs_host, n_radii_sample, n_radii_total):
+    """The oracle's FP32 restatement (own FFT, OpenMP) on a bounded sample: the forward
+    transforms plus the first `n_radii_sample` radii, extrapolated to a whole call."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import mxr
+    import oracle_lib as O
+
+    p = mxr.hii_params(N, box, filter_type=1, zeta=1.0)
+    st = mxr.HiiState.host(N)
+    n, secs, fwd = O.hii_sample(p, inputs_host, st, n_radii_sample)
+    per_radius = (secs - fwd) / max(n, 1)
+    t_call = fwd + n_radii_total * per_radius
+    return {
+        "value": N ** 3 * n_radii_total / t_call,
+        "unit": "cell-radius updates/s",
+        "cores": int(O.lib().oracle_num_threads()),
+        "kind": "port",
+        "sample": (f"{N}^3 FP32 restatement: forward r2c x3 ({fwd:.1f} s) + {n} of {n_radii_total} radii "
+                   f"({per_radius:.2f} s each), extrapolated to one call ({t_call:.0f} s)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--N", type=int, default=512)
+    ap.add_argument("--box", type=float, default=400.0)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-radii", type=int, default=2, help="radii in the CPU-baseline sample (0: skip)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this path runs the whole grid on a single MI355X and has "
+              "no multi-GPU mode (see DESIGN.md for how it would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import mxr
+    import synth
+
+    N, box = args.N, args.box
+    params = mxr.hii_params(N, box, filter_type=1, zeta=1.0)
+    n_radii = len(mxr.radii(params))
+    log(f"bench: N={N} box={box} radii={n_radii} generating inputs")
+    t0 = time.time()
+    inputs = {k: v.contiguous() for k, v in
+              synth.realisation(N, box, args.seed, device="cuda", stars_to_matter=0.45).items()}
+    torch.cuda.synchronize()
+    log(f"bench: inputs ready in {time.time() - t0:.1f} s")
+    st = make_state(torch, N ** 3)
+    ctx = mxr.Context(N)
+
+    torch.cuda.synchronize()
+
+    def call(c):
+        arrays = dict(inputs)
+        arrays.update(st.arrays())
+        return c.find_hii_bubbles(params, **arrays)
+
+    # ---- fused path: the headline
+    for _ in range(args.warmup):
+        call(ctx)
+    torch.cuda.synchronize()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g = call(ctx)
+        dev_ms.append(g.device_ms)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = 1e3 * wall / args.steps
+    fused_xh = g.volume_weighted_global_xH
+    log(f"bench: fused {ms_step:.2f} ms/step (device {sum(dev_ms) / len(dev_ms):.2f} ms), xH={fused_xh:.6f}")
+
+    # ---- per-kernel timing (one profiled call)
+    ctx.profile(True)
+    call(ctx)
+    ks = ctx.kernel_stats()
+    ctx.profile(False)
+    dom_name, dom = max(ks.items(), key=lambda kv: kv[1]["total_ms"])
+    avg_s = dom["total_ms"] / dom["launches"] / 1e3
+    bytes_per_launch = dom["alg_bytes"] / dom["launches"]
+    achieved = bytes_per_launch / avg_s
+    kernels = {k: dict(launches=v["launches"], total_ms=round(v["total_ms"], 3),
+                       GBps=round(v["alg_bytes"] / max(v["total_ms"], 1e-9) / 1e6, 1)) for k, v in ks.items()}
+    log("bench: kernels " + json.dumps(kernels))
+
+    # ---- plain path on the same GPU
+    plain = None
+    if args.plain_steps > 0:
+        ctx.set_path(mxr.MXR_PATH_PLAIN)
+        call(ctx)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            gp = call(ctx)
+        torch.cuda.synchronize()
+        pms = 1e3 * (time.perf_counter() - t0) / args.plain_steps
+        plain = {"value": N ** 3 * n_radii / (pms / 1e3), "ms_per_step": round(pms, 3),
+                 "xH": gp.volume_weighted_global_xH, "fused_speedup": round(pms / ms_step, 2)}
+        log(f"bench: plain {pms:.2f} ms/step xH={gp.volume_weighted_global_xH:.6f}")
+        ctx.set_path(mxr.MXR_PATH_FUSED)
+
+    # ---- CPU baseline (oracle restatement on this host, bounded sample)
+    cpu = None
+    if args.cpu_radii > 0:
+        host = {k: v.cpu().numpy() for k, v in inputs.items()}
+        del inputs
+        torch.cuda.empty_cache()
+        log("bench: CPU baseline sample")
+        cpu = cpu_baseline(N, box, host, args.cpu_radii, n_radii)
+        log("bench: cpu " + json.dumps(cpu))
+
+    value = N ** 3 * n_radii / (ms_step / 1e3)
+    out = {
+        "metric": "cell-radius updates/s at 512^3 (find_HII_bubbles, sharp-k, batched/fused path)",
+        "value": value,
+        "unit": "cell-radius updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FFT, filter) + f64 (criterion, reductions)",
+        "data": "synthetic (seeded lognormal deltax + Poisson/power-law stars, sfr = stars/100 Myr)",
+        "config": {"workload": f"find_HII_bubbles N={N} box={box} cMpc/h, {n_radii} radii R_max=50 -> cell, "
+                               f"ReionFilterType=1, 3 grids, z=8, ReionEfficiency=1",
+                   "grid_dim": N, "radii": n_radii, "cells": N ** 3},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s * 1e3},
+        "kernels": kernels,
+        "plain": plain,
+        "cpu_baseline": cpu,
+        "xH_global": fused_xh,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

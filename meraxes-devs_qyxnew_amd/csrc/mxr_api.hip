@@ -1,0 +1,477 @@
+// mxr_api.hip -- C ABI entry points, context lifetime, error reporting and the host-side
+// physics (units, RtoM, radius schedule, T_RECFAST) of the reionization grid path.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "mxr_internal.hpp"
+
+namespace mxr {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line)
+{
+  set_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+  return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? MXR_ERR_OOM : MXR_ERR_HIP;
+}
+
+// ---- units and cosmology -----------------------------------------------------------------
+// set_units (src/core/init.c:203-225): UnitTime = L/V, G in internal units, Hubble = HUBBLE UT,
+// RhoCrit = 3 Hubble^2 / (8 pi G).
+static double unit_time_s(const mxr_hii_params* p) { return p->UnitLength_in_cm / p->UnitVelocity_in_cm_per_s; }
+
+double rho_crit(const mxr_hii_params* p)
+{
+  const double UT = unit_time_s(p);
+  const double G = GRAVITY / std::pow(p->UnitLength_in_cm, 3) * p->UnitMass_in_g * std::pow(UT, 2);
+  const double Hub = HUBBLE * UT;
+  return 3 * Hub * Hub / (8 * M_PI * G);
+}
+
+// hubble_time (src/core/virial_properties.c:6-20, 79-96): 1 / (Hubble E(z)).
+double hubble_time(const mxr_hii_params* p)
+{
+  const double opz = 1. + p->redshift;
+  const double Ez = std::sqrt(p->OmegaM * opz * opz * opz + p->OmegaK * opz * opz + p->OmegaLambda);
+  return 1.0 / (HUBBLE * unit_time_s(p) * Ez);
+}
+
+// cT_approx (src/core/XRayHeatingFunctions.c:566-569)
+float cT_approx(float z) { return 0.58 - 0.006 * (z - 10.0); }
+
+// Natural cubic spline through (x, y) evaluated at xq -- the interpolant GSL's
+// gsl_interp_cspline builds (S'' = 0 at both ends), written from the textbook tridiagonal
+// system for the knot second derivatives.
+static double natural_spline(const double* x, const double* y, int n, double xq)
+{
+  std::vector<double> M(n, 0.0), cp(n, 0.0), dp(n, 0.0);
+  for (int i = 1; i < n - 1; i++) {
+    const double h0 = x[i] - x[i - 1], h1 = x[i + 1] - x[i];
+    const double r = 6.0 * ((y[i + 1] - y[i]) / h1 - (y[i] - y[i - 1]) / h0);
+    const double den = 2.0 * (h0 + h1) - (i > 1 ? h0 * cp[i - 1] : 0.0);
+    cp[i] = h1 / den;
+    dp[i] = (r - (i > 1 ? h0 * dp[i - 1] : 0.0)) / den;
+  }
+  for (int i = n - 2; i >= 1; i--)
+    M[i] = dp[i] - (i < n - 2 ? cp[i] * M[i + 1] : 0.0);
+  int lo = 0, hi = n - 1;
+  while (hi > lo + 1) {
+    const int mid = (lo + hi) / 2;
+    if (x[mid] > xq)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  const double h = x[lo + 1] - x[lo], t = xq - x[lo];
+  const double b = (y[lo + 1] - y[lo]) / h - h * (2.0 * M[lo] + M[lo + 1]) / 6.0;
+  return y[lo] + t * (b + t * (M[lo] / 2.0 + t * (M[lo + 1] - M[lo]) / (6.0 * h)));
+}
+
+// T_RECFAST (src/core/XRayHeatingFunctions.c:134-192): IGM temperature from the 501-row
+// recfast_LCDM.dat table (columns z, x_e, T_gas?, T_K; z descending), read as floats.
+double T_RECFAST(const char* tables_dir, float z)
+{
+  if (!tables_dir) {
+    set_error("T_RECFAST: no table directory");
+    return -1;
+  }
+  std::string fname = std::string(tables_dir) + "/recfast_LCDM.dat";
+  FILE* F = std::fopen(fname.c_str(), "r");
+  if (!F) {
+    set_error("T_RECFAST: unable to open %s", fname.c_str());
+    return -1;
+  }
+  double zt[RECFAST_NPTS], TK[RECFAST_NPTS];
+  for (int i = RECFAST_NPTS - 1; i >= 0; i--) {
+    float currz, trash, currTK;
+    if (std::fscanf(F, "%f %E %E %E", &currz, &trash, &trash, &currTK) != 4) {
+      std::fclose(F);
+      set_error("T_RECFAST: %s is truncated", fname.c_str());
+      return -1;
+    }
+    zt[i] = currz;
+    TK[i] = currTK;
+  }
+  std::fclose(F);
+  if (z > zt[RECFAST_NPTS - 1]) {
+    set_error("T_RECFAST called with z=%f beyond the table", (double)z);
+    return -1;
+  }
+  return natural_spline(zt, TK, RECFAST_NPTS, z);
+}
+
+int check_hii_params(const mxr_hii_params* p, int N)
+{
+  if (!p) {
+    set_error("params is NULL");
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (p->ReionGridDim != N) {
+    set_error("ReionGridDim %d does not match the context's %d", p->ReionGridDim, N);
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (!(p->BoxSize > 0) || !(p->ReionDeltaRFactor > 1.0) || !(p->UnitLength_in_cm > 0) ||
+      !(p->UnitVelocity_in_cm_per_s > 0) || !(p->UnitMass_in_g > 0) || !(p->redshift >= 0) ||
+      !(p->OmegaM > 0)) {
+    set_error("non-physical parameter (BoxSize, ReionDeltaRFactor, units, redshift or OmegaM)");
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (p->ReionFilterType < 0 || p->ReionFilterType > 2) {
+    set_error("ReionFilterType type %d is undefined", p->ReionFilterType);
+    return MXR_ERR_FILTER;
+  }
+  if (p->ReionRtoMFilterType < 0 || p->ReionRtoMFilterType > 1) {
+    set_error("Unrecognised RtoM filter (%d)", p->ReionRtoMFilterType);
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (p->Flag_IncludeRecombinations || p->Flag_BHFeedback) {
+    set_error("Flag_IncludeRecombinations / Flag_BHFeedback are not implemented by this library");
+    return MXR_ERR_UNSUPPORTED;
+  }
+  return MXR_OK;
+}
+
+static const char* const kclass_names[K_NCLASS] = {
+  "fwd_r2c_z", "fwd_fft_y", "fwd_fft_x_scale", "inv_fft_x_filter", "inv_fft_y", "inv_c2r_z_update",
+  "plain_copy", "plain_filter", "plain_clamp", "plain_criterion", "plain_scale_fill", "postloop_sums"};
+
+const char* kclass_name(int cls) { return (cls >= 0 && cls < K_NCLASS) ? kclass_names[cls] : "?"; }
+
+int prof_begin(Ctx* c)
+{
+  if (!c->prof)
+    return -1;
+  while ((int)c->evpool.size() < c->evnext + 2) {
+    hipEvent_t e;
+    MXR_HIP(hipEventCreate(&e));
+    c->evpool.push_back(e);
+  }
+  const int ev = c->evnext;
+  c->evnext += 2;
+  MXR_HIP(hipEventRecord(c->evpool[ev], c->stream));
+  return ev;
+}
+
+int prof_end(Ctx* c, int ev, int cls, double bytes)
+{
+  if (ev < 0)
+    return MXR_OK;
+  MXR_HIP(hipEventRecord(c->evpool[ev + 1], c->stream));
+  c->recs.push_back(Ctx::Rec{cls, ev, bytes});
+  return MXR_OK;
+}
+
+int prof_resolve(Ctx* c)
+{
+  for (const auto& r : c->recs) {
+    float ms = 0.f;
+    MXR_HIP(hipEventElapsedTime(&ms, c->evpool[r.ev], c->evpool[r.ev + 1]));
+    c->kstat[r.cls].launches++;
+    c->kstat[r.cls].total_ms += ms;
+    c->kstat[r.cls].alg_bytes += r.bytes;
+  }
+  c->recs.clear();
+  c->evnext = 0;
+  return MXR_OK;
+}
+
+}  // namespace mxr
+
+using namespace mxr;
+
+extern "C" {
+
+const char* mxr_strerror(int code)
+{
+  switch (code) {
+    case MXR_OK: return "success";
+    case MXR_ERR_INVALID_ARG: return "invalid argument";
+    case MXR_ERR_UNSUPPORTED: return "unsupported configuration";
+    case MXR_ERR_HIP: return "HIP runtime error";
+    case MXR_ERR_OOM: return "device out of memory";
+    case MXR_ERR_TABLES: return "physics table unavailable";
+    case MXR_ERR_NO_DEVICE: return "no usable gfx950 device";
+    case MXR_ERR_FILTER: return "undefined ReionFilterType";
+    default: return "unknown error";
+  }
+}
+
+const char* mxr_last_error(void) { return g_err; }
+
+int mxr_create(mxr_ctx** out, int device, int grid_dim)
+{
+  if (!out) {
+    set_error("ctx pointer is NULL");
+    return MXR_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  if (!supported_dim(grid_dim)) {
+    set_error("grid dimension %d is not supported (need one of 16,32,64,96,128,160,256,512)",
+              grid_dim);
+    return MXR_ERR_INVALID_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+    set_error("HIP device %d not available (%d visible)", device, ndev);
+    return MXR_ERR_NO_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  MXR_HIP(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_error("device %d is %s, this library is built for gfx950", device, prop.gcnArchName);
+    return MXR_ERR_NO_DEVICE;
+  }
+  mxr_ctx* h = new (std::nothrow) mxr_ctx();
+  if (!h) {
+    set_error("host allocation failed");
+    return MXR_ERR_OOM;
+  }
+  Ctx& c = h->c;
+  c.device = device;
+  c.N = grid_dim;
+  int rc = MXR_OK;
+  auto fail = [&](int code) {
+    mxr_destroy(h);
+    return code;
+  };
+  if (hipSetDevice(device) != hipSuccess)
+    return fail(hip_fail(hipErrorInvalidDevice, "hipSetDevice", __FILE__, __LINE__));
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking)) != hipSuccess)
+    return fail(hip_fail(e, "hipStreamCreate", __FILE__, __LINE__));
+  if ((e = hipEventCreate(&c.ev0)) != hipSuccess || (e = hipEventCreate(&c.ev1)) != hipSuccess)
+    return fail(hip_fail(e, "hipEventCreate", __FILE__, __LINE__));
+  std::vector<float2> tw(grid_dim);
+  for (int m = 0; m < grid_dim; m++) {
+    const double a = -2.0 * M_PI * (double)m / (double)grid_dim;
+    tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  if ((e = hipMalloc(&c.tw, sizeof(float2) * grid_dim)) != hipSuccess)
+    return fail(hip_fail(e, "hipMalloc(twiddles)", __FILE__, __LINE__));
+  if ((e = hipMemcpy(c.tw, tw.data(), sizeof(float2) * grid_dim, hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(hip_fail(e, "hipMemcpy(twiddles)", __FILE__, __LINE__));
+  if ((e = hipMalloc(&c.sums, sizeof(double) * 16)) != hipSuccess)
+    return fail(hip_fail(e, "hipMalloc(sums)", __FILE__, __LINE__));
+  if ((e = hipHostMalloc((void**)&c.sums_host, sizeof(double) * 16, hipHostMallocDefault)) != hipSuccess)
+    return fail(hip_fail(e, "hipHostMalloc(sums)", __FILE__, __LINE__));
+  (void)rc;
+  *out = h;
+  return MXR_OK;
+}
+
+void mxr_destroy(mxr_ctx* h)
+{
+  if (!h)
+    return;
+  Ctx& c = h->c;
+  hipSetDevice(c.device);
+  if (c.stream)
+    hipStreamSynchronize(c.stream);
+  hipFree(c.tw);
+  for (int g = 0; g < 4; g++) {
+    hipFree(c.spec[g]);
+    hipFree(c.work[g]);
+  }
+  for (int g = 0; g < 5; g++) {
+    hipFree(c.in_stage[g]);
+    hipFree(c.out_stage[g]);
+  }
+  hipFree(c.ion);
+  hipFree(c.partials);
+  hipFree(c.sums);
+  if (c.sums_host)
+    hipHostFree(c.sums_host);
+  for (auto e : c.evpool)
+    hipEventDestroy(e);
+  if (c.ev0)
+    hipEventDestroy(c.ev0);
+  if (c.ev1)
+    hipEventDestroy(c.ev1);
+  if (c.stream)
+    hipStreamDestroy(c.stream);
+  delete h;
+}
+
+int mxr_set_path(mxr_ctx* h, int path)
+{
+  if (!h || (path != MXR_PATH_FUSED && path != MXR_PATH_PLAIN && path != MXR_PATH_FUSED_DENSE)) {
+    set_error("bad context or path %d", path);
+    return MXR_ERR_INVALID_ARG;
+  }
+  h->c.path = path;
+  return MXR_OK;
+}
+
+// find_HII_bubbles.c:27-46
+double mxr_RtoM(const mxr_hii_params* p, double R)
+{
+  if (!p)
+    return -1;
+  const double OmegaM = p->OmegaM;
+  const double RhoCrit = rho_crit(p);
+  switch (p->ReionRtoMFilterType) {
+    case 0: return (4.0 / 3.0) * M_PI * std::pow(R, 3) * (OmegaM * RhoCrit);
+    case 1: return std::pow(2 * M_PI, 1.5) * OmegaM * RhoCrit * std::pow(R, 3);
+    default: set_error("Unrecognised filter (%d)", p->ReionRtoMFilterType); return -1;
+  }
+}
+
+// find_HII_bubbles.c:145-148, 256-285, 546
+int mxr_hii_radii(const mxr_hii_params* p, double* radii, int capacity)
+{
+  if (!p || p->ReionGridDim <= 0 || !(p->ReionDeltaRFactor > 1.0) || !(p->BoxSize > 0)) {
+    set_error("bad parameters for the radius schedule");
+    return MXR_ERR_INVALID_ARG;
+  }
+  const int N = p->ReionGridDim;
+  const double box_size = p->BoxSize;
+  const double clf = (box_size / (double)N) < 1.0 ? 1.0 : L_FACTOR;
+  double Rmax;
+  if (p->Flag_EvolvingReionRBubbleMax)
+    Rmax = (p->redshift > 6.) ? 25.483241248322766 : 112. * std::pow((1. + p->redshift) / 5., -4.4);
+  else
+    Rmax = p->Flag_IncludeRecombinations ? p->ReionRBubbleMaxRecomb : p->ReionRBubbleMax;
+  double R = std::fmin(Rmax, L_FACTOR * box_size);
+  int n = 0;
+  bool last = false;
+  while (!last) {
+    if (((R / p->ReionDeltaRFactor) <= (clf * box_size / (double)N)) ||
+        ((R / p->ReionDeltaRFactor) <= p->ReionRBubbleMin)) {
+      last = true;
+      R = clf * box_size / (double)N;
+    }
+    if (radii && n < capacity)
+      radii[n] = R;
+    n++;
+    R /= p->ReionDeltaRFactor;
+  }
+  return n;
+}
+
+int mxr_find_hii_bubbles(mxr_ctx* h, const mxr_hii_params* params, const mxr_hii_grids* grids,
+                         mxr_hii_globals* globals)
+{
+  if (!h || !grids || !globals) {
+    set_error("NULL context, grids or globals");
+    return MXR_ERR_INVALID_ARG;
+  }
+  int rc = check_hii_params(params, h->c.N);
+  if (rc)
+    return rc;
+  if (!grids->deltax || !grids->stars || !grids->weighted_sfr || !grids->xH || !grids->z_at_ionization ||
+      !grids->J_21_at_ionization || !grids->r_bubble || !grids->temp_kinetic_all_gas ||
+      (params->Flag_IncludeSpinTemp && (!grids->x_e_box || !grids->Tk_box)) ||
+      (grids->mem != MXR_MEM_HOST && grids->mem != MXR_MEM_DEVICE)) {
+    set_error("missing grid pointer or bad memory kind");
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (hipSetDevice(h->c.device) != hipSuccess) {
+    set_error("hipSetDevice(%d) failed", h->c.device);
+    return MXR_ERR_HIP;
+  }
+  return run_find_hii_bubbles(&h->c, params, grids, globals);
+}
+
+int mxr_filter(mxr_ctx* h, void* box, int mem, int local_ix_start, int slab_nx, int grid_dim, double BoxSize,
+               float R, int filter_type)
+{
+  if (!h || !box || slab_nx <= 0 || local_ix_start < 0 || local_ix_start + slab_nx > grid_dim ||
+      grid_dim != h->c.N || !(BoxSize > 0) || (mem != MXR_MEM_HOST && mem != MXR_MEM_DEVICE)) {
+    set_error("bad filter arguments");
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (filter_type < 0 || filter_type > 2) {
+    set_error("ReionFilterType type %d is undefined", filter_type);
+    return MXR_ERR_FILTER;
+  }
+  if (hipSetDevice(h->c.device) != hipSuccess) {
+    set_error("hipSetDevice(%d) failed", h->c.device);
+    return MXR_ERR_HIP;
+  }
+  // reionization.c:3376-3377: box size and delta_k in float
+  const float box_size = (float)BoxSize;
+  const float dk = (float)(2.0 * M_PI / box_size);
+  Ctx& c = h->c;
+  const long n = (long)slab_nx * grid_dim * (grid_dim / 2 + 1);
+  if (mem == MXR_MEM_DEVICE)
+    return run_filter(&c, (float2*)box, local_ix_start, slab_nx, grid_dim, dk, R, filter_type);
+  float2* d = nullptr;
+  MXR_HIP(hipMalloc(&d, sizeof(float2) * n));
+  int rc = MXR_OK;
+  hipError_t e = hipMemcpyAsync(d, box, sizeof(float2) * n, hipMemcpyHostToDevice, c.stream);
+  if (e == hipSuccess)
+    rc = run_filter(&c, d, local_ix_start, slab_nx, grid_dim, dk, R, filter_type);
+  else
+    rc = hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
+  if (rc == MXR_OK) {
+    e = hipMemcpyAsync(box, d, sizeof(float2) * n, hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess)
+      e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess)
+      rc = hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
+  }
+  hipFree(d);
+  return rc;
+}
+
+int mxr_profile_enable(mxr_ctx* h, int on)
+{
+  if (!h) {
+    set_error("NULL context");
+    return MXR_ERR_INVALID_ARG;
+  }
+  h->c.prof = on != 0;
+  h->c.recs.clear();
+  h->c.evnext = 0;
+  for (auto& k : h->c.kstat)
+    k = KStat{};
+  return MXR_OK;
+}
+
+int mxr_profile_read(mxr_ctx* h, mxr_kernel_stat* out, int capacity)
+{
+  if (!h || (capacity > 0 && !out)) {
+    set_error("bad profile_read arguments");
+    return MXR_ERR_INVALID_ARG;
+  }
+  for (int i = 0; i < K_NCLASS && i < capacity; i++) {
+    std::snprintf(out[i].name, sizeof(out[i].name), "%s", kclass_name(i));
+    out[i].launches = h->c.kstat[i].launches;
+    out[i].total_ms = h->c.kstat[i].total_ms;
+    out[i].alg_bytes = h->c.kstat[i].alg_bytes;
+  }
+  return K_NCLASS;
+}
+
+int mxr_brightness_temperature(mxr_ctx* h, const mxr_tb_params* p, const mxr_tb_grids* g, double* ave)
+{
+  if (!h || !p || !g || !ave || !g->xH || !g->deltax || !g->delta_T || p->ReionGridDim != h->c.N ||
+      (p->Flag_IncludeSpinTemp && !g->TS_box) || (g->mem != MXR_MEM_HOST && g->mem != MXR_MEM_DEVICE)) {
+    set_error("bad brightness-temperature arguments");
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (p->Flag_IncludePecVelsFor21cm) {
+    set_error("Flag_IncludePecVelsFor21cm > 0 is not implemented by this library");
+    return MXR_ERR_UNSUPPORTED;
+  }
+  if (hipSetDevice(h->c.device) != hipSuccess) {
+    set_error("hipSetDevice(%d) failed", h->c.device);
+    return MXR_ERR_HIP;
+  }
+  return run_brightness_temperature(&h->c, p, g, ave);
+}
+
+}  // extern "C"

@@ -1,0 +1,1178 @@
+// mxr_hii.hip -- find_HII_bubbles on one gfx950 GPU.
+//
+// Data layout in HBM (all per grid g in {deltax, stars, weighted_sfr[, x_e]}):
+//   spec[g]  unfiltered half spectrum, reference layout I = k + H (j + N i), H = N/2 + 1,
+//            divided by N^3 (find_HII_bubbles.c:235-253); written once per call.
+//   work[g]  same shape: per radius the x-pass writes filtered+transformed columns here, the
+//            y-pass transforms them in place, the z-pass (c2r) consumes them.
+//   ion      one byte per cell: ionised during this call (replaces the xH = 1 reset and the
+//            per-radius xH reads of find_HII_bubbles.c:150-153, 490-543).
+// FUSED path per radius (3 launches for all grids):
+//   x-pass  : W(kR) filter (reionization.c:3373-3433) fused into the loads, inverse FFT over i
+//   y-pass  : inverse FFT over j, in place
+//   z-pass  : per-row c2r (half-length complex FFT + Hermitian pre-twiddle), then the clamp
+//             (find_HII_bubbles.c:372-422), criterion and state update (:451-544) and, on the
+//             last radius, the temperatures and FP64 global sums (:549-645).
+// For the sharp-k filter every pass skips the modes the window zeroes (exact zeros in the
+// reference too): a column or row segment whose smallest |k| is cut is neither read nor
+// written, so the work per radius shrinks with the kept k-ball.  The skip tests evaluate the
+// window with the reference's float arithmetic, so the skipped set is exactly the zeroed set.
+// PLAIN path: the reference's one-kernel-per-step form (find_HII_bubbles_gpu.cu:256-521):
+//   per radius copy, filter, 3-pass c2r writing the real field, clamp, criterion kernel.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mxr_fft.hpp"
+#include "mxr_internal.hpp"
+
+namespace mxr {
+
+struct GridPtrs {
+  float2* p[4];
+};
+struct CGridPtrs {
+  const float2* p[4];
+};
+struct CRealPtrs {
+  const float* p[4];
+};
+
+// column c of a column pass starts at (c / cpp) * ps + (c % cpp); its elements are es apart
+struct ColGeom {
+  long cpp, ps, es, ncol;
+};
+
+// Window of one radius.  apply = 0 on the last (unfiltered) step.
+struct FiltArgs {
+  int N;
+  float dk;   // (float)(2 pi / (float)BoxSize)
+  float R;    // (float)R
+  int ftype;
+  int apply;
+  int prune;  // skip the modes the sharp-k window zeroes
+};
+
+enum { COL_PLAIN = 0, COL_SCALE = 1, COL_FILTER = 2, COL_YPRUNE = 3 };
+
+// grid pointer selection without dynamic indexing of a by-value kernel argument
+template <class P>
+__device__ __forceinline__ auto pick(const P& ps, int g)
+{
+  return g == 0 ? ps.p[0] : g == 1 ? ps.p[1] : g == 2 ? ps.p[2] : ps.p[3];
+}
+enum { C2R_STORE = 0, C2R_HII = 1 };
+
+// ---- the reference's window arithmetic (float, unfused) --------------------------------
+__device__ __forceinline__ float kmag_ref(int nx, int ny, int nz, int N, float dk)
+{
+  const int mid = N / 2;
+  const float kx = __fmul_rn((float)(nx > mid ? nx - N : nx), dk);
+  const float ky = __fmul_rn((float)(ny > mid ? ny - N : ny), dk);
+  const float kz = __fmul_rn((float)nz, dk);
+  return __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz, kz)));
+}
+
+// sharp-k (type 1): kR *= 0.413566994 (double product, float result); zero if kR > 1
+__device__ __forceinline__ bool sharpk_keep(int nx, int ny, int nz, const FiltArgs& fa)
+{
+  const float kR = __fmul_rn(kmag_ref(nx, ny, nz, fa.N, fa.dk), fa.R);
+  const float s = (float)((double)kR * 0.413566994);
+  return !(s > 1.0f);
+}
+
+// multiplicative window of types 0 (real-space top-hat) and 2 (Gaussian)
+__device__ __forceinline__ float window_mul(int nx, int ny, int nz, const FiltArgs& fa)
+{
+  float kR = __fmul_rn(kmag_ref(nx, ny, nz, fa.N, fa.dk), fa.R);
+  if (fa.ftype == 0) {
+    if ((double)kR > 1e-4) {
+      const float a = __fdiv_rn(sinf(kR), powf(kR, 3.0f));
+      const float b = __fdiv_rn(cosf(kR), powf(kR, 2.0f));
+      return (float)(3.0 * (double)__fsub_rn(a, b));
+    }
+    return 1.0f;
+  }
+  kR = (float)((double)kR * 0.643);
+  return powf(2.71828182845904523536f, (float)((double)__fmul_rn(-kR, kR) / 2.0));
+}
+
+// ---- column pass (x: columns = (j,k) flattened, elements i; y: columns = (i,k), elements j)
+template <int L, int NT, int TC, int S, int MODE>
+__global__ void __launch_bounds__(NT) k_col(CGridPtrs in, GridPtrs out, ColGeom geo, const float2* __restrict__ twg,
+                                            FiltArgs fa, double div)
+{
+  constexpr int T = NT / TC;
+  using SL = Slots<L, T>;
+  __shared__ float2 lds[L * TC];
+  __shared__ float2 tw[L];
+  const int g = blockIdx.y;
+  const int c = threadIdx.x % TC;
+  const int t = threadIdx.x / TC;
+  const long col = (long)blockIdx.x * TC + c;
+  const bool valid = col < geo.ncol;
+  const long base = valid ? (col / geo.cpp) * geo.ps + (col % geo.cpp) : 0;
+  for (int m = threadIdx.x; m < L; m += NT)
+    tw[m] = twg[m];
+
+  bool colnz = valid;
+  int fj = 0, fk = 0;
+  if constexpr (MODE == COL_FILTER) {
+    const int H = fa.N / 2 + 1;
+    fj = (int)(col / H);
+    fk = (int)(col % H);
+    if (valid && fa.apply && fa.prune && fa.ftype == 1)
+      colnz = sharpk_keep(0, fj, fk, fa);
+  } else if constexpr (MODE == COL_YPRUNE) {
+    fk = (int)(col % geo.cpp);
+    if (valid && fa.apply && fa.prune)
+      colnz = sharpk_keep(0, 0, fk, fa);
+  }
+  if constexpr (MODE == COL_FILTER || MODE == COL_YPRUNE) {
+    if (!__syncthreads_or(colnz))
+      return;  // every column of the tile is zero for this radius
+  }
+
+  const float2* __restrict__ src = pick(in, g);
+  float2 v[SL::NREG];
+#pragma unroll
+  for (int q = 0; q < SL::Q0; q++) {
+    const int j = t + T * q;
+#pragma unroll
+    for (int r = 0; r < SL::R0; r++) {
+      const int e = j + r * SL::B0;
+      float2 x = make_float2(0.f, 0.f);
+      if ((SL::B0 % T == 0 || j < SL::B0) && colnz) {
+        if constexpr (MODE == COL_FILTER) {
+          if (!fa.apply) {
+            x = src[base + (long)e * geo.es];
+          } else if (fa.ftype == 1) {
+            if (sharpk_keep(e, fj, fk, fa))
+              x = src[base + (long)e * geo.es];
+          } else {
+            const float w = window_mul(e, fj, fk, fa);
+            x = src[base + (long)e * geo.es];
+            x = make_float2(x.x * w, x.y * w);
+          }
+        } else if constexpr (MODE == COL_YPRUNE) {
+          if (!fa.apply || !fa.prune || sharpk_keep(0, e, fk, fa))
+            x = src[base + (long)e * geo.es];
+        } else {
+          x = src[base + (long)e * geo.es];
+        }
+      }
+      v[q * SL::R0 + r] = x;
+    }
+  }
+  tile_fft<L, T, S, L, LdsCol<TC>>(v, lds, tw, c, t);
+  if (!colnz)
+    return;
+  float2* __restrict__ dst = pick(out, g);
+#pragma unroll
+  for (int q = 0; q < SL::QL; q++) {
+    const int j = t + T * q;
+    if (SL::BL % T == 0 || j < SL::BL) {
+#pragma unroll
+      for (int r = 0; r < SL::RL; r++) {
+        const int e = j + r * SL::BL;
+        float2 y = v[q * SL::RL + r];
+        if constexpr (MODE == COL_SCALE)
+          y = make_float2((float)((double)y.x / div), (float)((double)y.y / div));
+        dst[base + (long)e * geo.es] = y;
+      }
+    }
+  }
+}
+
+// ---- z-pass forward: padded real rows -> half-spectrum rows ------------------------------
+// x[2n] + i x[2n+1] packed into an N/2-point complex FFT Z; then
+// X[k] = (Z[k] + conj Z[M-k]) / 2 + exp(-2 pi i k / N) (Z[k] - conj Z[M-k]) / (2i), k = 0..M.
+template <int N, int NT, int TR>
+__global__ void __launch_bounds__(NT) k_row_r2c(CRealPtrs in, GridPtrs out, long nrows, const float2* __restrict__ twg)
+{
+  constexpr int M = N / 2, H = N / 2 + 1, T = NT / TR, P = M + 1;
+  using SL = Slots<M, T>;
+  __shared__ float2 lds[TR * P];
+  __shared__ float2 tw[N];
+  const int g = blockIdx.y;
+  const int c = threadIdx.x / T, t = threadIdx.x % T;
+  const long row = (long)blockIdx.x * TR + c;
+  const bool valid = row < nrows;
+  for (int m = threadIdx.x; m < N; m += NT)
+    tw[m] = twg[m];
+  const float* __restrict__ src = pick(in, g) + (valid ? row : 0) * 2 * H;
+  float2 v[SL::NREG];
+#pragma unroll
+  for (int q = 0; q < SL::Q0; q++) {
+    const int j = t + T * q;
+#pragma unroll
+    for (int r = 0; r < SL::R0; r++) {
+      const int n = j + r * SL::B0;
+      float2 x = make_float2(0.f, 0.f);
+      if (valid && (SL::B0 % T == 0 || j < SL::B0))
+        x = *reinterpret_cast<const float2*>(src + 2 * n);
+      v[q * SL::R0 + r] = x;
+    }
+  }
+  __syncthreads();  // twiddles visible (single-stage transforms use them below)
+  tile_fft<M, T, -1, N, LdsRow<P>>(v, lds, tw, c, t);
+#pragma unroll
+  for (int q = 0; q < SL::QL; q++) {
+    const int j = t + T * q;
+    if (SL::BL % T == 0 || j < SL::BL)
+#pragma unroll
+      for (int r = 0; r < SL::RL; r++)
+        lds[c * P + j + r * SL::BL] = v[q * SL::RL + r];
+  }
+  __syncthreads();
+  if (!valid)
+    return;
+  float2* __restrict__ dst = pick(out, g) + row * H;
+  for (int k = t; k <= M; k += T) {
+    const float2 zk = lds[c * P + (k % M)];
+    const float2 zm0 = lds[c * P + ((M - k) % M)];
+    const float2 zm = make_float2(zm0.x, -zm0.y);
+    const float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
+    const float2 d = make_float2(0.5f * (zk.x - zm.x), 0.5f * (zk.y - zm.y));
+    const float2 o = make_float2(d.y, -d.x);  // d / i
+    dst[k] = cadd(e, cmul(tw[k], o));
+  }
+}
+
+// ---- fused epilogue state -------------------------------------------------------------
+struct HiiEpi {
+  double f_coll_prefactor;
+  float J21c;     // J_21_aux_constant (float, find_HII_bubbles.c:434-435)
+  float z;        // (float)redshift
+  float TK;       // T_RECFAST(z)
+  float cT_ad;    // cT_approx(z)
+  int first, last, uvb, spin;
+  uint8_t* ion;
+  float* xH;
+  float* z_in;
+  float* J21;
+  float* r_bubble;
+  float* temp;
+  const float* deltax;  // unfiltered input (padded)
+  const float* sfr;     // unfiltered input (padded)
+  const float* Tk_box;  // real (spin only)
+  double* partials;     // 6 per block on the last radius
+};
+
+// find_HII_bubbles.c:48-74
+__device__ __noinline__ float fully_ionized_T(float z_re, float z, float delta)
+{
+#pragma clang fp contract(off)
+  float result, delta_re;
+  if (fabs((double)(z - z_re)) < 1e-4)
+    result = 1;
+  else {
+    delta_re = (float)((double)delta * (1. + (double)z) / (1. + (double)z_re));
+    if (delta_re <= -1)
+      delta_re = (float)(-1. + (double)ABS_TOL);
+    if (delta <= -1)
+      delta = (float)(-1. + (double)ABS_TOL);
+    result = (float)pow((1. + (double)delta) / (1. + (double)delta_re), 1.1333);
+    result = (float)((double)result * pow((1. + (double)z) / (1. + (double)z_re), 3.4));
+    result = (float)((double)result *
+                     (double)expf((float)(pow((1. + (double)z) / 7.1, 2.5) - pow((1. + (double)z_re) / 7.1, 2.5))));
+  }
+  result = (float)((double)result * pow(T_RE, 1.7));
+  result = (float)((double)result + pow(1e4 * ((1. + (double)z) / 4.), 1.7) * (double)(1 + delta));
+  result = (float)pow((double)result, 0.5882);
+  return result;
+}
+
+// T_HI * (1 + cT delta) with the reference's promotions: float product, double sum
+__device__ __forceinline__ float thermal_T(float TK, float cT, float dx)
+{
+  return (float)((double)TK * (1. + (double)__fmul_rn(cT, dx)));
+}
+
+// wave-wide then block-wide FP64 sum of NV values; thread 0 writes them to out
+template <int NT, int NV>
+__device__ __forceinline__ void block_sum(double* acc, double* out)
+{
+  __shared__ double red[NT / 64][NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    double x = acc[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+      x += __shfl_down(x, o, 64);
+    acc[q] = x;
+  }
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NV; q++)
+      red[w][q] = acc[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      double s = 0.0;
+      for (int ww = 0; ww < NT / 64; ww++)
+        s += red[ww][q];
+      out[q] = s;
+    }
+  }
+}
+
+// One cell of the fused update.  df/sf/xf: filtered deltax/stars/x_e (already c2r'd).
+__device__ __forceinline__ void hii_cell(const HiiEpi& ea, long ir, long ip, float df, float sf, float wf, float xf,
+                                         double* acc)
+{
+#pragma clang fp contract(off)
+  // aliasing clamp, find_HII_bubbles.c:377-411
+  df = fmaxf(df, -1.0f + REL_TOL);
+  sf = fmaxf(sf, 0.0f);
+  if (sf < ABS_TOL)
+    sf = 0.0f;
+  wf = fmaxf(wf, 0.0f);
+  if (wf < ABS_TOL)
+    wf = 0.0f;
+  (void)wf;  // weighted_sfr_filtered enters only the recombination branch
+  if (ea.spin) {
+    xf = fmaxf(xf, 0.0f);
+    if (xf < ABS_TOL)
+      xf = 0.0f;
+    xf = fminf(xf, 0.999f);
+  }
+  // criterion, :457-485 (double)
+  const double density_over_mean = 1.0 + (double)df;
+  const double f_coll_stars = (double)sf / density_over_mean;
+  const double neutral_fraction = ea.spin ? 1.0 - (double)xf : 1.0;
+  const double lhs = (0.0 + f_coll_stars) * ea.f_coll_prefactor;
+  const bool ionise = lhs > neutral_fraction * (1. + 0.0);
+
+  const bool was = ea.first ? false : (ea.ion[ir] != 0);
+  float xh = was ? 0.0f : 1.0f;
+  bool partial = false;
+  float T_HI = 0.0f;
+  if (!was) {
+    if (ionise) {
+      xh = 0.0f;
+      ea.ion[ir] = 1;
+    } else {
+      if (ea.first)
+        ea.ion[ir] = 0;
+      if (ea.last) {  // :516-528
+        T_HI = ea.spin ? ea.Tk_box[ir] : thermal_T(ea.TK, ea.cT_ad, ea.deltax[ip]);
+        partial = true;
+        xh = (float)(neutral_fraction - lhs);
+        if (xh < 0.0f)
+          xh = 0.0f;
+        else if (xh > 1.0f)
+          xh = 1.0f;
+      }
+    }
+    if (xh < REL_TOL) {  // :531-543 (z_in read only on a new crossing)
+      if (ea.z_in[ir] < 0) {
+        ea.z_in[ir] = ea.z;
+        if (ea.uvb == 1)
+          ea.J21[ir] = __fmul_rn(sf, ea.J21c);
+      }
+    }
+  }
+  if (!ea.last)
+    return;
+  // post-loop, :577-611
+  const float dx_in = ea.deltax[ip];
+  const float zin = ea.z_in[ir];
+  float temp;
+  if (zin > 0 && xh < REL_TOL)
+    temp = fully_ionized_T(zin, ea.z, dx_in);
+  else if (partial)
+    temp = T_HI;  // ComputePartiallyIoinizedTemperature(T_HI, xH = 1) = T_HI
+  else
+    temp = ea.temp[ir];
+  const float floor_T = ea.spin ? ea.Tk_box[ir] : thermal_T(ea.TK, ea.cT_ad, dx_in);
+  if (temp < floor_T)
+    temp = floor_T;
+  ea.xH[ir] = xh;
+  ea.temp[ir] = temp;
+  ea.r_bubble[ir] = 0.0f;
+  const double dom = 1.0 + (double)dx_in;
+  acc[0] += (double)xh;
+  acc[1] += (double)xh * dom;
+  acc[2] += dom;
+  acc[3] += (double)ea.sfr[ip];
+  acc[4] += (double)temp;
+  acc[5] += (double)temp * dom;
+}
+
+// One grid's rows through the inverse z-pass: load the half-spectrum rows into LDS (only the
+// k the window keeps when pruning), pre-twiddle, M-point inverse FFT.  out receives the
+// last-stage slots: slot (q, r) = y[2n] + i y[2n+1], n = t + T q + r BL.
+template <int N, int T, int TR>
+__device__ __forceinline__ void row_c2r_grid(const float2* __restrict__ base, bool valid, long row, float2* lds,
+                                             const float2* tw, const FiltArgs& fa, int c, int t, float2* out)
+{
+  constexpr int M = N / 2, H = N / 2 + 1, P = M + 1;
+  using SL = Slots<M, T>;
+  const float2* __restrict__ src = base + (valid ? row : 0) * H;
+  for (int k = t; k < H; k += T) {
+    float2 x = make_float2(0.f, 0.f);
+    if (valid && (!fa.apply || !fa.prune || sharpk_keep(0, 0, k, fa)))
+      x = src[k];
+    lds[c * P + k] = x;
+  }
+  __syncthreads();
+  float2 v[SL::NREG];
+#pragma unroll
+  for (int q = 0; q < SL::Q0; q++) {
+    const int j = t + T * q;
+#pragma unroll
+    for (int r = 0; r < SL::R0; r++) {
+      const int n = j + r * SL::B0;
+      float2 z = make_float2(0.f, 0.f);
+      if (SL::B0 % T == 0 || j < SL::B0) {
+        const float2 xn = lds[c * P + n];
+        const float2 xm0 = lds[c * P + M - n];
+        const float2 xm = make_float2(xm0.x, -xm0.y);
+        const float2 s = cadd(xn, xm);
+        const float2 d = csub(xn, xm);
+        const float2 w = make_float2(tw[n].x, -tw[n].y);  // exp(+2 pi i n / N)
+        const float2 u = cmul(w, d);
+        z = make_float2(s.x - u.y, s.y + u.x);  // s + i u
+      }
+      v[q * SL::R0 + r] = z;
+    }
+  }
+  __syncthreads();
+  tile_fft<M, T, +1, N, LdsRow<P>>(v, lds, tw, c, t);
+#pragma unroll
+  for (int s = 0; s < SL::QL * SL::RL; s++)
+    out[s] = v[s];
+  __syncthreads();
+}
+
+// ---- z-pass inverse: half-spectrum rows -> real rows, + store or fused update -----------
+// Z[n] = (X[n] + conj X[M-n]) + i exp(+2 pi i n / N) (X[n] - conj X[M-n]), n < M;
+// an M-point inverse FFT of Z gives y[2n] + i y[2n+1] (unnormalised c2r).
+template <int N, int NT, int TR, int EPI>
+__global__ void __launch_bounds__(NT) k_row_c2r(CGridPtrs in, int ngrid, long nrows, const float2* __restrict__ twg,
+                                                FiltArgs fa, HiiEpi ea)
+{
+  constexpr int M = N / 2, H = N / 2 + 1, T = NT / TR, P = M + 1;
+  using SL = Slots<M, T>;
+  constexpr int NG = (EPI == C2R_HII) ? 4 : 1;
+  __shared__ float2 lds[TR * P];
+  __shared__ float2 tw[N];
+  const int c = threadIdx.x / T, t = threadIdx.x % T;
+  const long row = (long)blockIdx.x * TR + c;
+  const bool valid = row < nrows;
+  for (int m = threadIdx.x; m < N; m += NT)
+    tw[m] = twg[m];
+  float2 res[NG][SL::QL * SL::RL];
+  const int g0 = (EPI == C2R_HII) ? 0 : blockIdx.y;
+  row_c2r_grid<N, T, TR>(pick(in, g0), valid, row, lds, tw, fa, c, t, res[0]);
+  if constexpr (NG == 4) {
+    row_c2r_grid<N, T, TR>(in.p[1], valid, row, lds, tw, fa, c, t, res[1]);
+    row_c2r_grid<N, T, TR>(in.p[2], valid, row, lds, tw, fa, c, t, res[2]);
+    if (ngrid > 3)  // uniform across the block
+      row_c2r_grid<N, T, TR>(in.p[3], valid, row, lds, tw, fa, c, t, res[3]);
+    else
+#pragma unroll
+      for (int s = 0; s < SL::QL * SL::RL; s++)
+        res[3][s] = make_float2(0.f, 0.f);
+  }
+
+  if constexpr (EPI == C2R_STORE) {
+    if (!valid)
+      return;
+    float* __restrict__ dst = const_cast<float*>(reinterpret_cast<const float*>(pick(in, g0))) + row * 2 * H;
+#pragma unroll
+    for (int q = 0; q < SL::QL; q++) {
+      const int j = t + T * q;
+      if (SL::BL % T == 0 || j < SL::BL)
+#pragma unroll
+        for (int r = 0; r < SL::RL; r++) {
+          const int n = j + r * SL::BL;
+          *reinterpret_cast<float2*>(dst + 2 * n) = res[0][q * SL::RL + r];
+        }
+    }
+  } else {
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (valid) {
+#pragma unroll
+      for (int q = 0; q < SL::QL; q++) {
+        const int j = t + T * q;
+        if (SL::BL % T == 0 || j < SL::BL)
+#pragma unroll
+          for (int r = 0; r < SL::RL; r++) {
+            const int n = j + r * SL::BL;
+            const int s = q * SL::RL + r;
+            const float2 d = res[0][s], st = res[1][s], w = res[2][s];
+            const float2 xe = ea.spin ? res[3][s] : make_float2(0.f, 0.f);
+            const long ir = row * N + 2 * n;
+            const long ip = row * 2 * H + 2 * n;
+            hii_cell(ea, ir, ip, d.x, st.x, w.x, xe.x, acc);
+            hii_cell(ea, ir + 1, ip + 1, d.y, st.y, w.y, xe.y, acc);
+          }
+      }
+    }
+    if (ea.last)
+      block_sum<NT, 6>(acc, ea.partials + (long)blockIdx.x * 6);
+  }
+}
+
+// ---- plain-path kernels (one kernel per reference step) -------------------------------
+__global__ void k_fill(float* __restrict__ p, float v, long n)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// complex_vector_times_scalar: spectrum /= N^3 (double divisor)
+__global__ void k_scale(float2* __restrict__ p, long n, double div)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    float2 x = p[i];
+    p[i] = make_float2((float)((double)x.x / div), (float)((double)x.y / div));
+  }
+}
+
+// filter(): the window on a slab of a half spectrum
+__global__ void k_filter(float2* __restrict__ box, int local_ix_start, int slab_nx, FiltArgs fa)
+{
+  const int N = fa.N, H = N / 2 + 1;
+  const long n = (long)slab_nx * N * H;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (long)gridDim.x * blockDim.x) {
+    const int nz = (int)(idx % H);
+    const int ny = (int)((idx / H) % N);
+    const int nx = (int)(idx / ((long)H * N)) + local_ix_start;
+    if (fa.ftype == 1) {
+      if (!sharpk_keep(nx, ny, nz, fa))
+        box[idx] = make_float2(0.f, 0.f);
+    } else {
+      const float w = window_mul(nx, ny, nz, fa);
+      const float2 x = box[idx];
+      box[idx] = make_float2(x.x * w, x.y * w);
+    }
+  }
+}
+
+// sanity_check_aliasing on one padded real grid: kind 0 deltax, 1 stars/sfr, 2 x_e
+__global__ void k_clamp(float* __restrict__ f, int N, int kind)
+{
+  const int H = N / 2 + 1;
+  const long n = (long)N * N * N;
+  for (long ir = (long)blockIdx.x * blockDim.x + threadIdx.x; ir < n; ir += (long)gridDim.x * blockDim.x) {
+    const long row = ir / N;
+    const long ip = row * 2 * H + ir % N;
+    float x = f[ip];
+    if (kind == 0) {
+      x = fmaxf(x, -1.0f + REL_TOL);
+    } else {
+      x = fmaxf(x, 0.0f);
+      if (x < ABS_TOL)
+        x = 0.0f;
+      if (kind == 2)
+        x = fminf(x, 0.999f);
+    }
+    f[ip] = x;
+  }
+}
+
+// find_HII_bubbles_gpu_main_loop shape, with the CPU path's semantics (:451-544)
+struct PlainArgs {
+  int N;
+  const float* dfl;
+  const float* sfl;
+  const float* xfl;
+};
+
+__global__ void k_criterion(PlainArgs pa, HiiEpi ea)
+{
+#pragma clang fp contract(off)
+  const int N = pa.N, H = N / 2 + 1;
+  const long n = (long)N * N * N;
+  for (long ir = (long)blockIdx.x * blockDim.x + threadIdx.x; ir < n; ir += (long)gridDim.x * blockDim.x) {
+    const long ip = (ir / N) * 2 * H + ir % N;
+    const double density_over_mean = 1.0 + (double)pa.dfl[ip];
+    const float sf = pa.sfl[ip];
+    const double f_coll_stars = (double)sf / density_over_mean;
+    const double neutral_fraction = ea.spin ? 1.0 - (double)pa.xfl[ip] : 1.0;
+    const double lhs = (0.0 + f_coll_stars) * ea.f_coll_prefactor;
+    float xh = ea.xH[ir];
+    if (lhs > neutral_fraction * (1. + 0.0)) {
+      xh = 0.0f;
+    } else if (ea.last && (xh > REL_TOL)) {
+      const float T_HI = ea.spin ? ea.Tk_box[ir] : thermal_T(ea.TK, ea.cT_ad, ea.deltax[ip]);
+      // ComputePartiallyIoinizedTemperature (find_HII_bubbles.c:76-84)
+      float tp;
+      if (xh <= 0.f)
+        tp = (float)T_RE;
+      else if (xh >= 1.f)
+        tp = T_HI;
+      else
+        tp = (float)((double)__fmul_rn(T_HI, xh) + T_RE * (1. - (double)xh));
+      ea.temp[ir] = tp;
+      xh = (float)(neutral_fraction - lhs);
+      if (xh < 0.0f)
+        xh = 0.0f;
+      else if (xh > 1.0f)
+        xh = 1.0f;
+    }
+    ea.xH[ir] = xh;
+    if ((xh < REL_TOL) && (ea.z_in[ir] < 0)) {
+      ea.z_in[ir] = ea.z;
+      if (ea.uvb == 1)
+        ea.J21[ir] = __fmul_rn(sf, ea.J21c);
+    }
+  }
+}
+
+// post-loop temperatures and FP64 partial sums (find_HII_bubbles.c:577-645)
+template <int NT>
+__global__ void __launch_bounds__(NT) k_postloop(int N, HiiEpi ea)
+{
+#pragma clang fp contract(off)
+  const int H = N / 2 + 1;
+  const long n = (long)N * N * N;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (long ir = (long)blockIdx.x * NT + threadIdx.x; ir < n; ir += (long)gridDim.x * NT) {
+    const long ip = (ir / N) * 2 * H + ir % N;
+    const float xh = ea.xH[ir];
+    const float dx_in = ea.deltax[ip];
+    const double dom = 1.0 + (double)dx_in;
+    const float zin = ea.z_in[ir];
+    float temp = ea.temp[ir];
+    if (zin > 0 && xh < REL_TOL)
+      temp = fully_ionized_T(zin, ea.z, dx_in);
+    const float floor_T = ea.spin ? ea.Tk_box[ir] : thermal_T(ea.TK, ea.cT_ad, dx_in);
+    if (temp < floor_T)
+      temp = floor_T;
+    ea.temp[ir] = temp;
+    acc[0] += (double)xh;
+    acc[1] += (double)xh * dom;
+    acc[2] += dom;
+    acc[3] += (double)ea.sfr[ip];
+    acc[4] += (double)temp;
+    acc[5] += (double)temp * dom;
+  }
+  block_sum<NT, 6>(acc, ea.partials + (long)blockIdx.x * 6);
+}
+
+// fixed-order final sum of per-block partials (deterministic)
+__global__ void k_final_sum(const double* __restrict__ partials, int nblk, int nv, double* __restrict__ out)
+{
+  __shared__ double red[256];
+  for (int q = 0; q < nv; q++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 256)
+      s += partials[(long)b * nv + q];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (threadIdx.x < o)
+        red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0)
+      out[q] = red[0];
+    __syncthreads();
+  }
+}
+
+// ---- brightness temperature (BrightnessTemperature.c:44-124, 512-545) ------------------
+struct TbArgs {
+  int N;
+  float const_factor;
+  double T_rad;
+  int spin;
+  const float* xH;
+  const float* deltax;
+  const float* TS;
+  float* dT;
+  double* partials;
+};
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_tb(TbArgs a)
+{
+#pragma clang fp contract(off)
+  const int N = a.N, H = N / 2 + 1;
+  const long n = (long)N * N * N;
+  double acc[1] = {0.0};
+  for (long ir = (long)blockIdx.x * NT + threadIdx.x; ir < n; ir += (long)gridDim.x * NT) {
+    const long ip = (ir / N) * 2 * H + ir % N;
+    const double pixel_deltax = a.deltax[ip];
+    const double pixel_x_HI = a.xH[ir];
+    float dT = (float)((double)a.const_factor * pixel_x_HI * (1.0 + pixel_deltax));
+    if (a.spin) {
+      const double f = (1 - a.T_rad / (double)a.TS[ir]);
+      dT = (float)((double)dT * f);
+    }
+    a.dT[ir] = dT;
+    acc[0] += (double)dT;
+  }
+  block_sum<NT, 1>(acc, a.partials + blockIdx.x);
+}
+
+// ---- launch configuration per grid dimension ------------------------------------------
+constexpr int pow2_floor(int x)
+{
+  int p = 1;
+  while (p * 2 <= x)
+    p *= 2;
+  return p;
+}
+
+template <int N>
+struct Cfg {
+  static constexpr int M = N / 2;
+  static constexpr int H = N / 2 + 1;
+  static constexpr int TC = 16;
+  static constexpr int COL_T = (N / 8 >= 16) ? 16 : (N / 8 < 1 ? 1 : N / 8);
+  static constexpr int COL_NT = TC * COL_T;
+  static constexpr int ROW_T = pow2_floor((M / 8 >= 16) ? 16 : (M / 8 < 1 ? 1 : M / 8));
+  static constexpr int ROW_NT = 256;
+  static constexpr int TR = ROW_NT / ROW_T;
+};
+
+template <int N>
+struct Hii {
+  using C = Cfg<N>;
+
+  static ColGeom xgeo() { return ColGeom{(long)N * C::H, 0, (long)N * C::H, (long)N * C::H}; }
+  static ColGeom ygeo() { return ColGeom{(long)C::H, (long)N * C::H, (long)C::H, (long)N * C::H}; }
+
+  template <int S, int MODE>
+  static int col(Ctx* c, CGridPtrs in, GridPtrs out, int ngrid, ColGeom geo, FiltArgs fa, double div)
+  {
+    dim3 grid((unsigned)((geo.ncol + C::TC - 1) / C::TC), (unsigned)ngrid);
+    hipLaunchKernelGGL((k_col<N, C::COL_NT, C::TC, S, MODE>), grid, dim3(C::COL_NT), 0, c->stream, in, out, geo, c->tw,
+                       fa, div);
+    MXR_HIP(hipGetLastError());
+    return MXR_OK;
+  }
+
+  static int r2c(Ctx* c, CRealPtrs in, GridPtrs out, int ngrid)
+  {
+    const long nrows = (long)N * N;
+    dim3 grid((unsigned)((nrows + C::TR - 1) / C::TR), (unsigned)ngrid);
+    hipLaunchKernelGGL((k_row_r2c<N, C::ROW_NT, C::TR>), grid, dim3(C::ROW_NT), 0, c->stream, in, out, nrows, c->tw);
+    MXR_HIP(hipGetLastError());
+    return MXR_OK;
+  }
+
+  template <int EPI>
+  static int c2r(Ctx* c, CGridPtrs in, int ngrid, FiltArgs fa, const HiiEpi& ea)
+  {
+    const long nrows = (long)N * N;
+    dim3 grid((unsigned)((nrows + C::TR - 1) / C::TR), (unsigned)(EPI == C2R_HII ? 1 : ngrid));
+    hipLaunchKernelGGL((k_row_c2r<N, C::ROW_NT, C::TR, EPI>), grid, dim3(C::ROW_NT), 0, c->stream, in, ngrid, nrows,
+                       c->tw, fa, ea);
+    MXR_HIP(hipGetLastError());
+    return MXR_OK;
+  }
+
+  static long c2r_blocks() { return ((long)N * N + C::TR - 1) / C::TR; }
+};
+
+static unsigned ew_blocks(long n)
+{
+  long b = (n + 255) / 256;
+  if (b > 8192)
+    b = 8192;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+int ensure_buffers(Ctx* c, int ngrid)
+{
+  const long Nc = n_complex(c->N);
+  for (int g = 0; g < ngrid; g++) {
+    if (!c->spec[g])
+      MXR_HIP(hipMalloc(&c->spec[g], sizeof(float2) * Nc));
+    if (!c->work[g])
+      MXR_HIP(hipMalloc(&c->work[g], sizeof(float2) * Nc));
+  }
+  if (!c->ion)
+    MXR_HIP(hipMalloc(&c->ion, (size_t)n_real(c->N)));
+  const size_t need = 6 * 8192 + 6 * (size_t)(((long)c->N * c->N + 15) / 16 + 16);
+  if (c->partials_cap < need) {
+    hipFree(c->partials);
+    c->partials = nullptr;
+    MXR_HIP(hipMalloc(&c->partials, sizeof(double) * need));
+    c->partials_cap = need;
+  }
+  return MXR_OK;
+}
+
+static int stage_in(Ctx* c, int slot, const float* host, size_t n, const float** dev)
+{
+  if (!c->in_stage[slot])
+    MXR_HIP(hipMalloc(&c->in_stage[slot], sizeof(float) * 2 * n_complex(c->N)));
+  MXR_HIP(hipMemcpyAsync(c->in_stage[slot], host, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+  *dev = c->in_stage[slot];
+  return MXR_OK;
+}
+
+static int stage_out(Ctx* c, int slot, const float* host, size_t n, float** dev)
+{
+  if (!c->out_stage[slot])
+    MXR_HIP(hipMalloc(&c->out_stage[slot], sizeof(float) * 2 * n_complex(c->N)));
+  MXR_HIP(hipMemcpyAsync(c->out_stage[slot], host, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+  *dev = c->out_stage[slot];
+  return MXR_OK;
+}
+
+#define RC(x)                                                                                                        \
+  do {                                                                                                               \
+    int rc_ = (x);                                                                                                   \
+    if (rc_)                                                                                                         \
+      return rc_;                                                                                                    \
+  } while (0)
+
+// Modes the sharp-k window keeps at radius R (host restatement of sharpk_keep, used only to
+// count algorithmic bytes): k1 = kept k_z on the k_x = k_y = 0 line, c2 = kept (k_y, k_z)
+// columns at k_x = 0, k3 = kept modes overall.
+struct KeptModes {
+  long k1, c2, k3;
+  static KeptModes all(int N)
+  {
+    const long H = N / 2 + 1;
+    return KeptModes{H, (long)N * H, (long)N * N * H};
+  }
+};
+
+static bool keep_host(int nx, int ny, int nz, int N, float dk, float R)
+{
+  const int mid = N / 2;
+  volatile float kx = (float)(nx > mid ? nx - N : nx) * dk;
+  volatile float ky = (float)(ny > mid ? ny - N : ny) * dk;
+  volatile float kz = (float)nz * dk;
+  volatile float x2 = kx * kx, y2 = ky * ky, z2 = kz * kz;
+  volatile float s = x2 + y2;
+  s = s + z2;
+  volatile float kR = std::sqrt((float)s) * R;
+  return !((float)((double)kR * 0.413566994) > 1.0f);
+}
+
+static KeptModes kept_modes(int N, float dk, float R)
+{
+  const int H = N / 2 + 1;
+  KeptModes km{0, 0, 0};
+  for (int k = 0; k < H; k++) {
+    if (!keep_host(0, 0, k, N, dk, R))
+      break;
+    km.k1++;
+    for (int j = 0; j < N; j++) {
+      if (!keep_host(0, j, k, N, dk, R))
+        continue;
+      km.c2++;
+      // kept |n_x| form a prefix of 0..N/2 (|k| grows with |n_x|)
+      int lo = 0, hi = N / 2;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (keep_host(mid, j, k, N, dk, R))
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      km.k3 += 1 + 2L * lo - ((N % 2 == 0 && lo == N / 2) ? 1 : 0);
+    }
+  }
+  return km;
+}
+
+template <int N>
+static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_hii_globals* out)
+{
+  using K = Hii<N>;
+  constexpr int H = N / 2 + 1;
+  const long Nc = n_complex(N), nreal = n_real(N);
+  const int spin = p->Flag_IncludeSpinTemp ? 1 : 0;
+  const int ngrid = spin ? 4 : 3;
+  RC(ensure_buffers(c, ngrid));
+
+  // ---- inputs and state on the device
+  const float* din[4] = {g->deltax, g->stars, g->weighted_sfr, spin ? g->x_e_box : nullptr};
+  const float* dTk = spin ? g->Tk_box : nullptr;
+  float* dst[5] = {g->xH, g->z_at_ionization, g->J_21_at_ionization, g->r_bubble, g->temp_kinetic_all_gas};
+  if (g->mem == MXR_MEM_HOST) {
+    for (int i = 0; i < ngrid; i++)
+      RC(stage_in(c, i, din[i], 2 * Nc, &din[i]));
+    if (spin)
+      RC(stage_in(c, 4, dTk, nreal, &dTk));
+    for (int i = 0; i < 5; i++)
+      RC(stage_out(c, i, dst[i], nreal, &dst[i]));
+  }
+
+  // ---- scalars (find_HII_bubbles.c:90-148)
+  const double box_size = p->BoxSize;
+  const double pixel_volume = std::pow(box_size / (double)N, 3);
+  const double total_n_cells = std::pow((double)N, 3);
+  const double UL = p->UnitLength_in_cm, UM = p->UnitMass_in_g;
+  const double UT = p->UnitLength_in_cm / p->UnitVelocity_in_cm_per_s;
+  const double sfr_timescale = p->ReionSfrTimescale * hubble_time(p);
+  const double redshift = p->redshift;
+  double TKd = p->TK_recfast;
+  if (TKd < 0)
+    TKd = T_RECFAST(p->TablesForXHeatingDir, (float)redshift);
+  if (TKd < 0)
+    return MXR_ERR_TABLES;
+  const int nR = mxr_hii_radii(p, nullptr, 0);
+  std::vector<double> radii(nR);
+  mxr_hii_radii(p, radii.data(), nR);
+
+  HiiEpi ea{};
+  ea.z = (float)redshift;
+  ea.TK = (float)TKd;
+  ea.cT_ad = cT_approx((float)redshift);
+  ea.uvb = p->ReionUVBFlag;
+  ea.spin = spin;
+  ea.ion = c->ion;
+  ea.xH = dst[0];
+  ea.z_in = dst[1];
+  ea.J21 = dst[2];
+  ea.r_bubble = dst[3];
+  ea.temp = dst[4];
+  ea.deltax = din[0];
+  ea.sfr = din[2];
+  ea.Tk_box = dTk;
+  ea.partials = c->partials;
+
+  FiltArgs fa{};
+  fa.N = N;
+  {
+    const float bs = (float)box_size;  // reionization.c:3376-3377
+    fa.dk = (float)(2.0 * M_PI / bs);
+  }
+  fa.ftype = p->ReionFilterType;
+  fa.prune = (c->path == MXR_PATH_FUSED) ? 1 : 0;
+
+  CRealPtrs rin{};
+  GridPtrs spec{}, work{};
+  CGridPtrs cspec{}, cwork{};
+  for (int i = 0; i < ngrid; i++) {
+    rin.p[i] = din[i];
+    spec.p[i] = c->spec[i];
+    work.p[i] = c->work[i];
+    cspec.p[i] = c->spec[i];
+    cwork.p[i] = c->work[i];
+  }
+  const bool plain = (c->path == MXR_PATH_PLAIN);
+
+  // timed launch: event pair around fn() when profiling, plus its algorithmic bytes
+  auto T = [&](int cls, double bytes, auto&& fn) -> int {
+    const int ev = prof_begin(c);
+    if (ev < -1)
+      return ev;
+    const int rc = fn();
+    if (rc)
+      return rc;
+    return prof_end(c, ev, cls, bytes);
+  };
+  const double cplx = 8.0 * (double)Nc;  // bytes of one half spectrum
+  const double G = (double)ngrid;
+
+  MXR_HIP(hipEventRecord(c->ev0, c->stream));
+  // ---- forward transforms (:163-253)
+  FiltArgs none{};
+  none.N = N;
+  RC(T(K_FWD_R2C, G * 2 * cplx, [&] { return K::r2c(c, rin, spec, ngrid); }));
+  RC(T(K_FWD_Y, G * 2 * cplx, [&] { return K::template col<-1, COL_PLAIN>(c, cspec, spec, ngrid, K::ygeo(), none, 1.0); }));
+  if (plain) {
+    RC(T(K_FWD_X, G * 2 * cplx,
+         [&] { return K::template col<-1, COL_PLAIN>(c, cspec, spec, ngrid, K::xgeo(), none, 1.0); }));
+    for (int i = 0; i < ngrid; i++)
+      RC(T(K_SCALE, 2 * cplx, [&] {
+        hipLaunchKernelGGL(k_scale, dim3(ew_blocks(Nc)), dim3(256), 0, c->stream, c->spec[i], Nc, total_n_cells);
+        MXR_HIP(hipGetLastError());
+        return MXR_OK;
+      }));
+    for (int f = 0; f < 2; f++)
+      RC(T(K_SCALE, 4.0 * nreal, [&] {
+        hipLaunchKernelGGL(k_fill, dim3(ew_blocks(nreal)), dim3(256), 0, c->stream, dst[f == 0 ? 0 : 3],
+                           f == 0 ? 1.0f : 0.0f, nreal);
+        MXR_HIP(hipGetLastError());
+        return MXR_OK;
+      }));
+  } else {
+    RC(T(K_FWD_X, G * 2 * cplx,
+         [&] { return K::template col<-1, COL_SCALE>(c, cspec, spec, ngrid, K::xgeo(), none, total_n_cells); }));
+  }
+
+  // ---- radius loop (:279-547)
+  for (int ri = 0; ri < nR; ri++) {
+    const double R = radii[ri];
+    const bool last = (ri == nR - 1);
+    const double M_mean = mxr_RtoM(p, R);
+    double Gamma_R_prefactor = (1.0 + redshift) * (1.0 + redshift) * R * UL / pixel_volume * std::pow(UL, -3.);
+    Gamma_R_prefactor *= (UM / UT) / PROTONMASS * p->ReionNionPhotPerBary;
+    ea.J21c = (float)(Gamma_R_prefactor * PLANCK * 1e21 / (4.0 * M_PI) * p->ReionGammaHaloBias / sfr_timescale *
+                      p->ReionAlphaUV);
+    ea.f_coll_prefactor = p->ReionEfficiency * (4.0 / 3.0) * M_PI / pixel_volume / M_mean * (R * R * R);
+    ea.first = (ri == 0);
+    ea.last = last;
+    fa.R = (float)R;
+    fa.apply = last ? 0 : 1;
+
+    if (!plain) {
+      // kept-mode counts of this radius (all modes when not pruning)
+      KeptModes km = (fa.apply && fa.prune && fa.ftype == 1) ? kept_modes(N, fa.dk, fa.R) : KeptModes::all(N);
+      const double state_b = last ? (double)nreal * (1 + 8 + 4 + 12 + (spin ? 4 : 0)) : (double)nreal * 1.0;
+      RC(T(K_INV_X, G * 8.0 * ((double)km.k3 + (double)N * km.c2),
+           [&] { return K::template col<+1, COL_FILTER>(c, cspec, work, ngrid, K::xgeo(), fa, 1.0); }));
+      RC(T(K_INV_Y, G * 8.0 * ((double)N * km.c2 + (double)N * N * km.k1),
+           [&] { return K::template col<+1, COL_YPRUNE>(c, cwork, work, ngrid, K::ygeo(), fa, 1.0); }));
+      RC(T(K_INV_Z, G * 8.0 * (double)N * N * km.k1 + state_b,
+           [&] { return K::template c2r<C2R_HII>(c, cwork, ngrid, fa, ea); }));
+    } else {
+      for (int i = 0; i < ngrid; i++) {
+        RC(T(K_COPY, 2 * cplx, [&] {
+          MXR_HIP(hipMemcpyAsync(c->work[i], c->spec[i], sizeof(float2) * Nc, hipMemcpyDeviceToDevice, c->stream));
+          return MXR_OK;
+        }));
+        if (!last)
+          RC(T(K_FILTER, 2 * cplx, [&] {
+            hipLaunchKernelGGL(k_filter, dim3(ew_blocks(Nc)), dim3(256), 0, c->stream, c->work[i], 0, N, fa);
+            MXR_HIP(hipGetLastError());
+            return MXR_OK;
+          }));
+      }
+      for (int i = 0; i < ngrid; i++) {
+        CGridPtrs ci{};
+        GridPtrs wi{};
+        ci.p[0] = c->work[i];
+        wi.p[0] = c->work[i];
+        RC(T(K_INV_X, 2 * cplx, [&] { return K::template col<+1, COL_PLAIN>(c, ci, wi, 1, K::xgeo(), none, 1.0); }));
+        RC(T(K_INV_Y, 2 * cplx, [&] { return K::template col<+1, COL_PLAIN>(c, ci, wi, 1, K::ygeo(), none, 1.0); }));
+        RC(T(K_INV_Z, 2 * cplx, [&] { return K::template c2r<C2R_STORE>(c, ci, 1, none, ea); }));
+        RC(T(K_CLAMP, 8.0 * nreal, [&] {
+          hipLaunchKernelGGL(k_clamp, dim3(ew_blocks(nreal)), dim3(256), 0, c->stream, (float*)c->work[i], N,
+                             i == 0 ? 0 : (i == 3 ? 2 : 1));
+          MXR_HIP(hipGetLastError());
+          return MXR_OK;
+        }));
+      }
+      PlainArgs pa{N, (const float*)c->work[0], (const float*)c->work[1], spin ? (const float*)c->work[3] : nullptr};
+      RC(T(K_CRIT, (double)nreal * (4 + 4 + (spin ? 4 : 0) + 4 + 4 + 4), [&] {
+        hipLaunchKernelGGL(k_criterion, dim3(ew_blocks(nreal)), dim3(256), 0, c->stream, pa, ea);
+        MXR_HIP(hipGetLastError());
+        return MXR_OK;
+      }));
+    }
+  }
+
+  // ---- reductions (:549-664)
+  int nblk = plain ? (int)ew_blocks(nreal) : (int)K::c2r_blocks();
+  RC(T(K_POST, plain ? (double)nreal * (4 + 4 + 4 + 4 + 4 + 4) : 0.0, [&] {
+    if (plain)
+      hipLaunchKernelGGL((k_postloop<256>), dim3(nblk), dim3(256), 0, c->stream, N, ea);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, c->stream, c->partials, nblk, 6, c->sums);
+    MXR_HIP(hipGetLastError());
+    return MXR_OK;
+  }));
+  MXR_HIP(hipEventRecord(c->ev1, c->stream));
+  MXR_HIP(hipMemcpyAsync(c->sums_host, c->sums, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
+  if (g->mem == MXR_MEM_HOST) {
+    float* hdst[5] = {g->xH, g->z_at_ionization, g->J_21_at_ionization, g->r_bubble, g->temp_kinetic_all_gas};
+    for (int i = 0; i < 5; i++)
+      MXR_HIP(hipMemcpyAsync(hdst[i], dst[i], sizeof(float) * nreal, hipMemcpyDeviceToHost, c->stream));
+  }
+  MXR_HIP(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  MXR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  RC(prof_resolve(c));
+
+  const double* s = c->sums_host;
+  std::memset(out, 0, sizeof(*out));
+  out->volume_weighted_global_xH = s[0] / total_n_cells;
+  out->mass_weighted_global_xH = s[1] / s[2];
+  out->volume_weighted_global_weighted_sfr = s[3] / total_n_cells * (UM / UT * SEC_PER_YEAR / SOLAR_MASS);
+  out->volume_weighted_global_temp_kinetic_all_gas = s[4] / total_n_cells;
+  out->mass_weighted_global_temp_kinetic_all_gas = s[5] / s[2];
+  out->n_radii = nR;
+  out->device_ms = ms;
+  return MXR_OK;
+}
+
+#define MXR_DIMS(X) X(16) X(32) X(64) X(96) X(128) X(160) X(256) X(512)
+
+bool supported_dim(int N)
+{
+#define X(n) if (N == n) return true;
+  MXR_DIMS(X)
+#undef X
+  return false;
+}
+
+int run_find_hii_bubbles(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_hii_globals* out)
+{
+  switch (c->N) {
+#define X(n)                                                                                                         \
+  case n:                                                                                                            \
+    return hii_run<n>(c, p, g, out);
+    MXR_DIMS(X)
+#undef X
+    default: set_error("unsupported grid dimension %d", c->N); return MXR_ERR_INVALID_ARG;
+  }
+}
+
+int run_filter(Ctx* c, float2* box, int ix0, int nx, int N, float dk, float R, int ftype)
+{
+  FiltArgs fa{};
+  fa.N = N;
+  fa.dk = dk;
+  fa.R = R;
+  fa.ftype = ftype;
+  fa.apply = 1;
+  const long n = (long)nx * N * (N / 2 + 1);
+  hipLaunchKernelGGL(k_filter, dim3(ew_blocks(n)), dim3(256), 0, c->stream, box, ix0, nx, fa);
+  MXR_HIP(hipGetLastError());
+  MXR_HIP(hipStreamSynchronize(c->stream));
+  return MXR_OK;
+}
+
+int run_brightness_temperature(Ctx* c, const mxr_tb_params* p, const mxr_tb_grids* g, double* ave)
+{
+  const int N = c->N;
+  const long nreal = n_real(N), npad = 2 * n_complex(N);
+  RC(ensure_buffers(c, 0));
+  const float* xH = g->xH;
+  const float* dx = g->deltax;
+  const float* TS = g->TS_box;
+  float* dT = g->delta_T;
+  if (g->mem == MXR_MEM_HOST) {
+    RC(stage_in(c, 0, xH, nreal, &xH));
+    RC(stage_in(c, 1, dx, npad, &dx));
+    if (p->Flag_IncludeSpinTemp)
+      RC(stage_in(c, 2, TS, nreal, &TS));
+    if (!c->out_stage[0])
+      MXR_HIP(hipMalloc(&c->out_stage[0], sizeof(float) * npad));
+    dT = c->out_stage[0];
+  }
+  // BrightnessTemperature.c:47-55, 82
+  const float redshift = (float)p->redshift;
+  const float Hubble_h = (float)p->Hubble_h;
+  const float OmegaM = (float)p->OmegaM;
+  const float OmegaB = (float)(OmegaM * p->BaryonFrac);
+  TbArgs a{};
+  a.N = N;
+  a.const_factor = (float)(27.0 * (OmegaB * Hubble_h * Hubble_h / 0.023) *
+                           std::sqrt((0.15 / OmegaM / Hubble_h / Hubble_h) * (1.0 + redshift) / 10.0));
+  a.T_rad = TCMB * (1. + redshift);
+  a.spin = p->Flag_IncludeSpinTemp;
+  a.xH = xH;
+  a.deltax = dx;
+  a.TS = TS;
+  a.dT = dT;
+  a.partials = c->partials;
+  const int nblk = (int)ew_blocks(nreal);
+  hipLaunchKernelGGL((k_tb<256>), dim3(nblk), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, c->stream, c->partials, nblk, 1, c->sums);
+  MXR_HIP(hipGetLastError());
+  MXR_HIP(hipMemcpyAsync(c->sums_host, c->sums, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (g->mem == MXR_MEM_HOST)
+    MXR_HIP(hipMemcpyAsync(g->delta_T, dT, sizeof(float) * nreal, hipMemcpyDeviceToHost, c->stream));
+  MXR_HIP(hipStreamSynchronize(c->stream));
+  *ave = c->sums_host[0] / std::pow((double)N, 3);
+  return MXR_OK;
+}
+
+}  // namespace mxr

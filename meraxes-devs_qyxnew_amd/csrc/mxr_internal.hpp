@@ -1,0 +1,119 @@
+// mxr_internal.hpp -- context, error plumbing and host-side physics shared by the library's
+// translation units.  Not part of the public ABI (include/mxr.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/mxr.h"
+
+namespace mxr {
+
+// Constants restated from src/meraxes.h:21-67 and src/core/reionization.h:10-47.
+constexpr double GRAVITY = 6.672e-8;
+constexpr double SOLAR_MASS = 1.989e33;
+constexpr double PLANCK = 6.6262e-27;
+constexpr double PROTONMASS = 1.6726e-24;
+constexpr double HUBBLE = 3.2407789e-18;
+constexpr double SEC_PER_YEAR = 3.155e7;
+constexpr double TCMB = 2.728;
+constexpr double T_RE = 1e4;
+constexpr float REL_TOL = 1e-5f;
+constexpr float ABS_TOL = 1e-8f;
+constexpr double L_FACTOR = 0.620350491;
+constexpr int RECFAST_NPTS = 501;
+
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define MXR_HIP(call)                                                                                                \
+  do {                                                                                                               \
+    hipError_t e_ = (call);                                                                                          \
+    if (e_ != hipSuccess)                                                                                            \
+      return ::mxr::hip_fail(e_, #call, __FILE__, __LINE__);                                                         \
+  } while (0)
+
+// Kernel classes timed by the profiler.
+enum KClass {
+  K_FWD_R2C = 0,  // z-pass r2c of the inputs
+  K_FWD_Y,        // forward y-pass
+  K_FWD_X,        // forward x-pass (+ /N^3 when fused)
+  K_INV_X,        // inverse x-pass (+ window when fused)
+  K_INV_Y,        // inverse y-pass
+  K_INV_Z,        // inverse z-pass c2r (+ clamp/criterion/state when fused)
+  K_COPY,         // plain: unfiltered -> filtered copy
+  K_FILTER,       // plain: filter()
+  K_CLAMP,        // plain: sanity_check_aliasing
+  K_CRIT,         // plain: criterion / state kernel
+  K_SCALE,        // plain: /N^3, xH and r_bubble fills
+  K_POST,         // post-loop temperatures and FP64 sums
+  K_NCLASS
+};
+
+struct KStat {
+  int launches = 0;
+  double total_ms = 0.0;
+  double alg_bytes = 0.0;
+};
+
+// One grid dimension's worth of persistent device state.
+struct Ctx {
+  // profiling: an event pair per timed launch, resolved after the call's final sync
+  bool prof = false;
+  std::vector<hipEvent_t> evpool;
+  struct Rec {
+    int cls;
+    int ev;
+    double bytes;
+  };
+  std::vector<Rec> recs;
+  int evnext = 0;
+  KStat kstat[K_NCLASS];
+
+  int device = 0;
+  int N = 0;
+  int path = MXR_PATH_FUSED;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float2* tw = nullptr;            // tw[m] = exp(-2 pi i m / N), m < N (FP64-evaluated, FP32-stored)
+  float2* spec[4] = {};            // unfiltered half spectra (deltax, stars, weighted_sfr, x_e)
+  float2* work[4] = {};            // filtered spectra / FFT work (same size)
+  float* in_stage[5] = {};         // device copies of host inputs: 4 padded grids + Tk_box
+  float* out_stage[5] = {};        // device copies of host state grids
+  uint8_t* ion = nullptr;          // per-cell "ionised during this call" flags
+  double* partials = nullptr;      // per-block FP64 partial sums
+  size_t partials_cap = 0;
+  double* sums = nullptr;          // final FP64 sums (device)
+  double* sums_host = nullptr;     // pinned copy
+};
+
+inline long n_complex(int N) { return (long)N * N * (N / 2 + 1); }
+inline long n_real(int N) { return (long)N * N * N; }
+
+// host physics (mxr_api.hip)
+double rho_crit(const mxr_hii_params* p);
+double hubble_time(const mxr_hii_params* p);
+double T_RECFAST(const char* tables_dir, float z);
+float cT_approx(float z);
+int check_hii_params(const mxr_hii_params* p, int N);
+
+// grid-dimension dispatch (mxr_hii.hip)
+bool supported_dim(int N);
+int run_find_hii_bubbles(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_hii_globals* out);
+int run_filter(Ctx* c, float2* box, int local_ix_start, int slab_nx, int grid_dim, float dk, float R, int ftype);
+int run_brightness_temperature(Ctx* c, const mxr_tb_params* p, const mxr_tb_grids* g, double* ave);
+int ensure_buffers(Ctx* c, int ngrid);
+int prof_begin(Ctx* c);                              // returns the event index (or -1)
+int prof_end(Ctx* c, int ev, int cls, double bytes); // records the launch
+int prof_resolve(Ctx* c);                            // after a stream sync
+const char* kclass_name(int cls);
+
+}  // namespace mxr
+
+struct mxr_ctx {
+  mxr::Ctx c;
+};

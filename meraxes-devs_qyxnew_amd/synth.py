@@ -1,0 +1,129 @@
+"""Seeded synthetic input grids for the reionization grid path.
+
+Not part of the hot path: it makes the deltax / stars / weighted_sfr grids that the
+benchmark and the parity tests feed, identically, to the HIP path and to the CPU oracle.
+torch is used for its FFT and RNG on either device; for a given (seed, device) the output is
+deterministic.  Grids come back in the reference's padded real layout
+(I = k + 2(N/2+1)(j + N i), src/core/misc_tools.c:244-264) with zero padding.
+
+Recipe (one realisation):
+  * g: Gaussian random field, white noise coloured in k-space with P(k) ~ k^-2.5, P(0) = 0,
+    rescaled to a per-cell standard deviation of 0.5;
+  * deltax = exp(g - var/2) - 1 (lognormal, <1 + deltax> ~ 1);
+  * halo counts per cell are Poisson with mean lambda = nbar exp(1.5 deltax) / <exp(1.5 deltax)>,
+    split over 24 log-spaced mass bins (8 per dex) of the power law dn/dM ~ M^-1.9 on
+    [1e8, 1e11] Msun; each bin contributes Poisson(lambda p_b) haloes of the bin's mean mass;
+  * stars = summed halo mass in 1e10 Msun/h; weighted_sfr = stars / 100 Myr in internal units;
+  * nbar is set by `stars_to_matter`: mean stars per cell = stars_to_matter x the mean matter
+    mass of a cell (Omega_m rho_crit V_pix), so the ionised fraction depends on the ratio, not
+    on N or the box.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+SEC_PER_MEGAYEAR = 3.155e13
+GRAVITY = 6.672e-8
+HUBBLE = 3.2407789e-18
+
+
+def _units(p):
+    UL, UM, UV = p["UnitLength_in_cm"], p["UnitMass_in_g"], p["UnitVelocity_in_cm_per_s"]
+    UT = UL / UV
+    G = GRAVITY / UL ** 3 * UM * UT ** 2
+    Hub = HUBBLE * UT
+    rho_crit = 3 * Hub * Hub / (8 * math.pi * G)
+    return UT, rho_crit
+
+
+def mass_bins(m_lo=1e8, m_hi=1e11, slope=-1.9, per_dex=8):
+    ndex = round(math.log10(m_hi / m_lo))
+    edges = m_lo * 10.0 ** (np.arange(ndex * per_dex + 1) / per_dex)
+    a = slope + 1.0  # integral of M^slope
+    b = slope + 2.0  # integral of M^(slope+1)
+    n_int = (edges[1:] ** a - edges[:-1] ** a) / a
+    m_int = (edges[1:] ** b - edges[:-1] ** b) / b
+    p = n_int / n_int.sum()
+    mbar = m_int / n_int
+    return p, mbar
+
+
+def gaussian_field(N: int, seed: int, device="cpu", sigma=0.5, slope=-2.5) -> torch.Tensor:
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    white = torch.randn((N, N, N), generator=g, device=device, dtype=torch.float32)
+    wk = torch.fft.rfftn(white)
+    kx = torch.fft.fftfreq(N, device=device) * N
+    kz = torch.fft.rfftfreq(N, device=device) * N
+    k2 = kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + kz[None, None, :] ** 2
+    amp = torch.where(k2 > 0, k2.clamp_min(1e-30) ** (slope / 4.0), torch.zeros_like(k2))
+    field = torch.fft.irfftn(wk * amp, s=(N, N, N))
+    field = field - field.mean()
+    field = field * (sigma / field.std())
+    del white, wk, amp, k2
+    return field
+
+
+def to_padded(x: torch.Tensor) -> torch.Tensor:
+    N = x.shape[0]
+    H = N // 2 + 1
+    out = torch.zeros((N, N, 2 * H), dtype=torch.float32, device=x.device)
+    out[:, :, :N] = x
+    return out.reshape(-1)
+
+
+def realisation(N: int, box: float, seed: int, device="cpu", stars_to_matter: float = 1.0, params=None,
+                padded=True):
+    """Return dict(deltax, stars, weighted_sfr) as float32 tensors on `device`."""
+    from mxr import GENESIS  # noqa: E402  (sibling module)
+
+    p = dict(GENESIS)
+    if params:
+        p.update(params)
+    UT, rho_crit = _units(p)
+    h = p["Hubble_h"]
+    v_pix = (box / N) ** 3
+    matter_cell = p["OmegaM"] * rho_crit * v_pix  # 1e10 Msun/h
+    prob, mbar = mass_bins()
+    mean_mass = float((prob * mbar).sum()) * h / 1e10  # 1e10 Msun/h
+    nbar = stars_to_matter * matter_cell / mean_mass
+
+    g = gaussian_field(N, seed, device)
+    var = float(g.var())
+    deltax = torch.exp(g - var / 2.0) - 1.0
+    del g
+    w = torch.exp(1.5 * deltax.double())
+    lam = (nbar * w / w.mean()).float()
+    del w
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed) * 7919 + 17)
+    stars = torch.zeros_like(lam, dtype=torch.float64)
+    for pb, mb in zip(prob, mbar):
+        n = torch.poisson(lam * float(pb), generator=gen)
+        stars += n.double() * float(mb * h / 1e10)
+        del n
+    del lam
+    t100 = 100.0 / (UT / SEC_PER_MEGAYEAR)
+    sfr = stars / t100
+    out = dict(deltax=deltax.float(), stars=stars.float(), weighted_sfr=sfr.float())
+    if padded:
+        out = {k: to_padded(v) for k, v in out.items()}
+    return out
+
+
+def numpy_realisation(N: int, box: float, seed: int, **kw):
+    return {k: v.cpu().numpy() for k, v in realisation(N, box, seed, device="cpu", **kw).items()}
+
+
+def history_sfr(N: int, box: float, seed: int, redshifts, device="cpu", **kw):
+    """sfr grids for a list of prior snapshots: seed + i, amplitude ~ 1/(1+z) normalised to
+    the last redshift (the spin-temperature history set-up)."""
+    z_last = redshifts[-1]
+    out = []
+    for i, z in enumerate(redshifts):
+        r = realisation(N, box, seed + i, device=device, **kw)
+        out.append(r["weighted_sfr"] * float((1 + z_last) / (1 + z)))
+    return out

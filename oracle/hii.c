@@ -1,0 +1,438 @@
+/*
+ * hii.c -- oracle restatement of find_HII_bubbles, filter, RtoM, T_RECFAST and the 21-cm
+ * brightness box (TEST INFRASTRUCTURE ONLY; see oracle.h).
+ *
+ * Reference files followed:
+ *   src/core/find_HII_bubbles.c:27-710      RtoM, ionised-gas temperatures, _find_HII_bubbles
+ *   src/core/reionization.c:3373-3433       filter
+ *   src/core/XRayHeatingFunctions.c:134-192 T_RECFAST (GSL natural cubic spline of
+ *                                           recfast_LCDM.dat), :566-569 cT_approx
+ *   src/core/init.c:203-225                 set_units (G, Hubble, RhoCrit)
+ *   src/core/virial_properties.c:6-96       E_z, hubble_at_z, hubble_time
+ *   src/core/BrightnessTemperature.c:24-549 ComputeBrightnessTemperatureBox
+ *   src/meraxes.h:21-67, src/core/reionization.h:10-47, src/core/misc_tools.h:14-20 constants
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "oracle.h"
+
+/* constants, src/meraxes.h:21-67 and src/core/reionization.h:10-47 */
+#define GRAVITY 6.672e-8
+#define SOLAR_MASS 1.989e33
+#define PLANCK 6.6262e-27
+#define PROTONMASS 1.6726e-24
+#define HUBBLE 3.2407789e-18
+#define SEC_PER_YEAR 3.155e7
+#define TCMB 2.728
+#define T_RE (double)(1e4)
+#define REL_TOL (float)1e-5
+#define ABS_TOL (float)1e-8
+#define L_FACTOR 0.620350491
+#define RECFAST_NPTS 501
+
+/* misc_tools.h:14-20 */
+#define CLAMP_0_1(x)                                                                                                   \
+  do {                                                                                                                 \
+    if ((x) < 0.0)                                                                                                     \
+      (x) = 0.0;                                                                                                       \
+    else if ((x) > 1.0)                                                                                                \
+      (x) = 1.0;                                                                                                       \
+  } while (0)
+
+/* FFT kernels (shared with fft.c through the template) */
+#define REAL float
+#define SUFFIX f32
+#include "fft_tmpl.h"
+#undef REAL
+#undef SUFFIX
+#define REAL double
+#define SUFFIX f64
+#include "fft_tmpl.h"
+#undef REAL
+#undef SUFFIX
+
+/* ---------------------------------------------------------------- units and cosmology */
+
+static double unit_time_s(const mxr_hii_params* p) { return p->UnitLength_in_cm / p->UnitVelocity_in_cm_per_s; }
+
+/* init.c:203-225 */
+double oracle_rho_crit(const mxr_hii_params* p)
+{
+  const double UT = unit_time_s(p);
+  const double G = GRAVITY / pow(p->UnitLength_in_cm, 3) * p->UnitMass_in_g * pow(UT, 2);
+  const double Hubble = HUBBLE * UT;
+  return 3 * Hubble * Hubble / (8 * M_PI * G);
+}
+
+/* virial_properties.c:6-20, 79-96 */
+double oracle_hubble_time(const mxr_hii_params* p)
+{
+  const double z = p->redshift;
+  const double opz = 1. + z;
+  const double Ez = sqrt(p->OmegaM * opz * opz * opz + p->OmegaK * opz * opz + p->OmegaLambda);
+  const double Hubble = HUBBLE * unit_time_s(p);
+  return 1.0 / (Hubble * Ez);
+}
+
+/* find_HII_bubbles.c:27-46 */
+double oracle_RtoM(const mxr_hii_params* p, double R)
+{
+  const double OmegaM = p->OmegaM;
+  const double RhoCrit = oracle_rho_crit(p);
+  switch (p->ReionRtoMFilterType) {
+    case 0:
+      return (4.0 / 3.0) * M_PI * pow(R, 3) * (OmegaM * RhoCrit);
+    case 1:
+      return pow(2 * M_PI, 1.5) * OmegaM * RhoCrit * pow(R, 3);
+    default:
+      return -1;
+  }
+}
+
+/* find_HII_bubbles.c:145-148, 256-285, 546 */
+int oracle_hii_radii(const mxr_hii_params* p, double* radii, int capacity)
+{
+  const int N = p->ReionGridDim;
+  const double box_size = p->BoxSize;
+  double cell_length_factor = L_FACTOR;
+  if ((box_size / (double)N) < 1.0)
+    cell_length_factor = 1.0;
+  double Rmax;
+  if (p->Flag_EvolvingReionRBubbleMax)
+    Rmax = (p->redshift > 6.) ? 25.483241248322766 : 112. * pow((1. + p->redshift) / 5., -4.4);
+  else
+    Rmax = p->Flag_IncludeRecombinations ? p->ReionRBubbleMaxRecomb : p->ReionRBubbleMax;
+  double R = fmin(Rmax, L_FACTOR * box_size);
+  int n = 0, last = 0;
+  while (!last) {
+    if (((R / p->ReionDeltaRFactor) <= (cell_length_factor * box_size / (double)N)) ||
+        ((R / p->ReionDeltaRFactor) <= p->ReionRBubbleMin)) {
+      last = 1;
+      R = cell_length_factor * box_size / (double)N;
+    }
+    if (radii && n < capacity)
+      radii[n] = R;
+    n++;
+    R /= p->ReionDeltaRFactor;
+  }
+  return n;
+}
+
+/* XRayHeatingFunctions.c:566-569 */
+static float cT_approx(float z) { return 0.58 - 0.006 * (z - 10.0); }
+
+/* find_HII_bubbles.c:48-74 */
+static float fully_ionized_T(float z_re, float z, float delta)
+{
+  float result, delta_re;
+  if (fabs(z - z_re) < 1e-4)
+    result = 1;
+  else {
+    delta_re = delta * (1. + z) / (1. + z_re);
+    if (delta_re <= -1)
+      delta_re = -1. + ABS_TOL;
+    if (delta <= -1)
+      delta = -1. + ABS_TOL;
+    result = pow((1. + delta) / (1. + delta_re), 1.1333);
+    result *= pow((1. + z) / (1. + z_re), 3.4);
+    result *= expf(pow((1. + z) / 7.1, 2.5) - pow((1. + z_re) / 7.1, 2.5));
+  }
+  result *= pow(T_RE, 1.7);
+  result += pow(1e4 * ((1. + z) / 4.), 1.7) * (1 + delta);
+  result = pow(result, 0.5882);
+  return result;
+}
+
+/* find_HII_bubbles.c:76-84 */
+static float partially_ionized_T(float T_HI, float res_xH)
+{
+  if (res_xH <= 0.)
+    return T_RE;
+  if (res_xH >= 1)
+    return T_HI;
+  return T_HI * res_xH + T_RE * (1. - res_xH);
+}
+
+/* ------------------------------------------------------------ natural cubic spline */
+/* The interpolant GSL's gsl_interp_cspline builds (natural end conditions, S'' = 0 at both
+ * ends): S_i(x) = y_i + b_i t + c_i t^2 + d_i t^3, t = x - x_i, with c_i = M_i / 2 from the
+ * tridiagonal system h_{i-1} M_{i-1} + 2 (h_{i-1} + h_i) M_i + h_i M_{i+1} = 6 (s_i - s_{i-1}).
+ * Interval lookup: x_i <= x < x_{i+1}, the last interval for x = x_{n-1}. */
+double oracle_spline_natural(const double* x, const double* y, int n, double xq)
+{
+  double* M = (double*)calloc((size_t)n, sizeof(double));
+  double* cp = (double*)calloc((size_t)n, sizeof(double));
+  double* dp = (double*)calloc((size_t)n, sizeof(double));
+  /* Thomas algorithm on the n-2 interior unknowns */
+  for (int i = 1; i < n - 1; i++) {
+    const double h0 = x[i] - x[i - 1], h1 = x[i + 1] - x[i];
+    const double a = h0, b = 2.0 * (h0 + h1), c = h1;
+    const double r = 6.0 * ((y[i + 1] - y[i]) / h1 - (y[i] - y[i - 1]) / h0);
+    const double den = b - (i > 1 ? a * cp[i - 1] : 0.0);
+    cp[i] = c / den;
+    dp[i] = (r - (i > 1 ? a * dp[i - 1] : 0.0)) / den;
+  }
+  for (int i = n - 2; i >= 1; i--)
+    M[i] = dp[i] - ((i < n - 2) ? cp[i] * M[i + 1] : 0.0);
+  int lo = 0, hi = n - 1;
+  while (hi > lo + 1) {
+    const int mid = (lo + hi) / 2;
+    if (x[mid] > xq)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  const double h = x[lo + 1] - x[lo];
+  const double t = xq - x[lo];
+  const double b = (y[lo + 1] - y[lo]) / h - h * (2.0 * M[lo] + M[lo + 1]) / 6.0;
+  const double c = M[lo] / 2.0;
+  const double d = (M[lo + 1] - M[lo]) / (6.0 * h);
+  const double v = y[lo] + t * (b + t * (c + t * d));
+  free(M);
+  free(cp);
+  free(dp);
+  return v;
+}
+
+/* XRayHeatingFunctions.c:134-192: recfast_LCDM.dat rows "z x_e ? T_K", z descending from
+ * 500; read as floats, stored reversed so z ascends. */
+double oracle_T_RECFAST(const char* tables_dir, float z)
+{
+  if (!tables_dir)
+    return -1;
+  char fname[1024];
+  snprintf(fname, sizeof(fname), "%s/recfast_LCDM.dat", tables_dir);
+  FILE* F = fopen(fname, "r");
+  if (!F)
+    return -1;
+  double zt[RECFAST_NPTS], TKt[RECFAST_NPTS];
+  for (int i = RECFAST_NPTS - 1; i >= 0; i--) {
+    float currz, trash, currTK;
+    if (fscanf(F, "%f %E %E %E", &currz, &trash, &trash, &currTK) != 4) {
+      fclose(F);
+      return -1;
+    }
+    zt[i] = currz;
+    TKt[i] = currTK;
+  }
+  fclose(F);
+  if (z > zt[RECFAST_NPTS - 1])
+    return -1;
+  return oracle_spline_natural(zt, TKt, RECFAST_NPTS, z);
+}
+
+/* ---------------------------------------------------------------------------- filter */
+/* reionization.c:3373-3433, float arithmetic as written (build with -ffp-contract=off). */
+static void filter_f32(float* boxf, int local_ix_start, int slab_nx, int grid_dim, double BoxSize, float R,
+                       int filter_type)
+{
+  float complex* box = (float complex*)boxf;
+  int middle = grid_dim / 2;
+  float box_size = (float)BoxSize;
+  float delta_k = (float)(2.0 * M_PI / box_size);
+#pragma omp parallel for schedule(static)
+  for (int n_x = 0; n_x < slab_nx; n_x++) {
+    float k_x;
+    int n_x_global = n_x + local_ix_start;
+    if (n_x_global > middle)
+      k_x = (n_x_global - grid_dim) * delta_k;
+    else
+      k_x = n_x_global * delta_k;
+    for (int n_y = 0; n_y < grid_dim; n_y++) {
+      float k_y;
+      if (n_y > middle)
+        k_y = (n_y - grid_dim) * delta_k;
+      else
+        k_y = n_y * delta_k;
+      for (int n_z = 0; n_z <= middle; n_z++) {
+        float k_z = n_z * delta_k;
+        float k_mag = sqrtf(k_x * k_x + k_y * k_y + k_z * k_z);
+        float kR = k_mag * R;
+        const long idx = n_z + (long)(middle + 1) * (n_y + (long)grid_dim * n_x);
+        switch (filter_type) {
+          case 0:
+            if (kR > 1e-4)
+              box[idx] *= (float complex)(3.0 * (sinf(kR) / powf(kR, 3) - cosf(kR) / powf(kR, 2)));
+            break;
+          case 1:
+            kR *= 0.413566994;
+            if (kR > 1)
+              box[idx] = (float complex)0.0;
+            break;
+          case 2:
+            kR *= 0.643;
+            box[idx] *= (float complex)(powf((float)M_E, (float)(-kR * kR / 2.0)));
+            break;
+          default:
+            break;
+        }
+      }
+    }
+  }
+}
+
+/* The same window evaluated entirely in double (FP64 oracle). */
+static void filter_f64(double* boxd, int local_ix_start, int slab_nx, int grid_dim, double BoxSize, double R,
+                       int filter_type)
+{
+  double complex* box = (double complex*)boxd;
+  int middle = grid_dim / 2;
+  double delta_k = 2.0 * M_PI / BoxSize;
+#pragma omp parallel for schedule(static)
+  for (int n_x = 0; n_x < slab_nx; n_x++) {
+    int n_x_global = n_x + local_ix_start;
+    double k_x = (n_x_global > middle) ? (n_x_global - grid_dim) * delta_k : n_x_global * delta_k;
+    for (int n_y = 0; n_y < grid_dim; n_y++) {
+      double k_y = (n_y > middle) ? (n_y - grid_dim) * delta_k : n_y * delta_k;
+      for (int n_z = 0; n_z <= middle; n_z++) {
+        double k_z = n_z * delta_k;
+        double kR = sqrt(k_x * k_x + k_y * k_y + k_z * k_z) * R;
+        const long idx = n_z + (long)(middle + 1) * (n_y + (long)grid_dim * n_x);
+        switch (filter_type) {
+          case 0:
+            if (kR > 1e-4)
+              box[idx] *= 3.0 * (sin(kR) / pow(kR, 3) - cos(kR) / pow(kR, 2));
+            break;
+          case 1:
+            if (kR * 0.413566994 > 1)
+              box[idx] = 0.0;
+            break;
+          case 2:
+            box[idx] *= exp(-(0.643 * kR) * (0.643 * kR) / 2.0);
+            break;
+          default:
+            break;
+        }
+      }
+    }
+  }
+}
+
+int oracle_filter_f32(float* box, int local_ix_start, int slab_nx, int grid_dim, double BoxSize, float R,
+                      int filter_type)
+{
+  if (filter_type < 0 || filter_type > 2)
+    return MXR_ERR_FILTER;
+  filter_f32(box, local_ix_start, slab_nx, grid_dim, BoxSize, R, filter_type);
+  return 0;
+}
+
+int oracle_filter_f64(double* box, int local_ix_start, int slab_nx, int grid_dim, double BoxSize, double R,
+                      int filter_type)
+{
+  if (filter_type < 0 || filter_type > 2)
+    return MXR_ERR_FILTER;
+  filter_f64(box, local_ix_start, slab_nx, grid_dim, BoxSize, R, filter_type);
+  return 0;
+}
+
+/* ------------------------------------------------------------------- the pipeline */
+#define REAL float
+#define SUFFIX f32
+#define FILTER_R_T float
+#define FMAXR fmaxf
+#define FMINR fminf
+#include "hii_tmpl.h"
+#undef REAL
+#undef SUFFIX
+#undef FILTER_R_T
+#undef FMAXR
+#undef FMINR
+
+#define REAL double
+#define SUFFIX f64
+#define FILTER_R_T double
+#define FMAXR fmax
+#define FMINR fmin
+#include "hii_tmpl.h"
+#undef REAL
+#undef SUFFIX
+#undef FILTER_R_T
+#undef FMAXR
+#undef FMINR
+
+static int check_args(const mxr_hii_params* p, const mxr_hii_grids* g)
+{
+  if (!p || !g || !g->deltax || !g->stars || !g->weighted_sfr || !g->xH || !g->z_at_ionization ||
+      !g->J_21_at_ionization || !g->r_bubble || !g->temp_kinetic_all_gas)
+    return MXR_ERR_INVALID_ARG;
+  if (p->ReionGridDim < 2 || (p->ReionGridDim % 2))
+    return MXR_ERR_INVALID_ARG;
+  if (p->Flag_IncludeSpinTemp && (!g->x_e_box || !g->Tk_box))
+    return MXR_ERR_INVALID_ARG;
+  return 0;
+}
+
+int oracle_find_hii_bubbles_f32(const mxr_hii_params* p, const mxr_hii_grids* g, mxr_hii_globals* out,
+                                double* margin)
+{
+  int rc = check_args(p, g);
+  if (rc)
+    return rc;
+  rc = hii_run_f32(p, g, out, margin, 0, NULL, NULL);
+  return rc < 0 ? rc : 0;
+}
+
+int oracle_find_hii_bubbles_f64(const mxr_hii_params* p, const mxr_hii_grids* g, mxr_hii_globals* out,
+                                double* margin)
+{
+  int rc = check_args(p, g);
+  if (rc)
+    return rc;
+  rc = hii_run_f64(p, g, out, margin, 0, NULL, NULL);
+  return rc < 0 ? rc : 0;
+}
+
+int oracle_hii_sample_f32(const mxr_hii_params* p, const mxr_hii_grids* g, int n_radii, double* seconds,
+                          double* seconds_fwd)
+{
+  int rc = check_args(p, g);
+  if (rc)
+    return rc;
+  return hii_run_f32(p, g, NULL, NULL, n_radii > 0 ? n_radii : 0, seconds, seconds_fwd);
+}
+
+/* ---------------------------------------------------------------- brightness box */
+/* BrightnessTemperature.c:24-124, 512-545 (Flag_IncludePecVelsFor21cm = 0) */
+int oracle_brightness_temperature(const mxr_tb_params* p, const mxr_tb_grids* g, double* volume_ave_Tb)
+{
+  if (!p || !g || !g->xH || !g->deltax || !g->delta_T || (p->Flag_IncludeSpinTemp && !g->TS_box))
+    return MXR_ERR_INVALID_ARG;
+  if (p->Flag_IncludePecVelsFor21cm)
+    return MXR_ERR_UNSUPPORTED;
+  const int N = p->ReionGridDim;
+  const long H = N / 2 + 1;
+  const double total_n_cells = pow((double)N, 3);
+  float redshift = (float)(float)p->redshift;
+  float Hubble_h = (float)(float)p->Hubble_h;
+  float OmegaM = (float)(float)p->OmegaM;
+  float OmegaB = (float)(OmegaM * p->BaryonFrac);
+  float const_factor = (float)(27.0 * (OmegaB * Hubble_h * Hubble_h / 0.023) *
+                               sqrt((0.15 / OmegaM / Hubble_h / Hubble_h) * (1.0 + redshift) / 10.0));
+  double T_rad = TCMB * (1. + redshift);
+  double Ave_Tb = 0.0;
+  for (int ii = 0; ii < N; ii++)
+    for (int jj = 0; jj < N; jj++)
+      for (int kk = 0; kk < N; kk++) {
+        const long i_real = kk + (long)N * (jj + (long)N * ii);
+        const long i_padded = kk + 2 * H * (jj + (long)N * ii);
+        double pixel_deltax = g->deltax[i_padded];
+        double pixel_x_HI = g->xH[i_real];
+        g->delta_T[i_real] = (float)(const_factor * pixel_x_HI * (1.0 + pixel_deltax));
+        if (p->Flag_IncludeSpinTemp) {
+          double pixel_Ts_factor = (1 - T_rad / g->TS_box[i_real]);
+          g->delta_T[i_real] *= pixel_Ts_factor;
+        }
+      }
+  for (long i = 0; i < (long)N * N * N; i++)
+    Ave_Tb += (double)g->delta_T[i];
+  Ave_Tb /= total_n_cells;
+  if (volume_ave_Tb)
+    *volume_ave_Tb = Ave_Tb;
+  return 0;
+}

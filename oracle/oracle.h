@@ -1,0 +1,68 @@
+/*
+ * oracle.h -- CPU restatement of the reference's reionization grid path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load liboracle.so, and only as the checker of (or the CPU timing beside)
+ * the HIP path.  The product (libmxr.so) never links, loads or falls back to it.
+ *
+ * Two restatements of the same pipeline:
+ *   *_f32 : faithful to the reference's types -- FP32 FFTs (FFTW's fftwf semantics), filter
+ *           arithmetic in float exactly as written in src/core/reionization.c:3373-3433,
+ *           clamps in float, the ionisation criterion in double.
+ *   *_f64 : every field, filter and FFT in double (the FP64 oracle the errors are quoted
+ *           against).
+ */
+#ifndef MXR_ORACLE_H
+#define MXR_ORACLE_H
+
+#include "../include/mxr.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* FFT (FFTW semantics, unnormalised).  data is interleaved (re, im). */
+int oracle_c2c1d_f32(int n, float* data, int sign);
+int oracle_c2c1d_f64(int n, double* data, int sign);
+int oracle_r2c3d_f32(int N, const float* in_padded, float* out_cplx);
+int oracle_r2c3d_f64(int N, const double* in_padded, double* out_cplx);
+int oracle_c2r3d_f32(int N, float* inout);
+int oracle_c2r3d_f64(int N, double* inout);
+
+/* filter() on a half spectrum (float complex / double complex). */
+int oracle_filter_f32(float* box, int local_ix_start, int slab_nx, int grid_dim, double BoxSize, float R,
+                      int filter_type);
+int oracle_filter_f64(double* box, int local_ix_start, int slab_nx, int grid_dim, double BoxSize, double R,
+                      int filter_type);
+
+/* Physics helpers. */
+double oracle_RtoM(const mxr_hii_params* p, double R);
+int oracle_hii_radii(const mxr_hii_params* p, double* radii, int capacity);
+double oracle_T_RECFAST(const char* tables_dir, float z);         /* < 0 on failure */
+double oracle_spline_natural(const double* x, const double* y, int n, double xq);
+double oracle_hubble_time(const mxr_hii_params* p);                /* internal units */
+double oracle_rho_crit(const mxr_hii_params* p);                   /* internal units */
+
+/* The pipeline.  Pointers are host arrays in the reference layouts (mxr.h).
+ * margin (optional, N^3 doubles): max over radii of (LHS/RHS - 1) of the criterion, the
+ * distance of each cell from the ionisation threshold. */
+int oracle_find_hii_bubbles_f32(const mxr_hii_params* p, const mxr_hii_grids* g, mxr_hii_globals* out,
+                                double* margin);
+int oracle_find_hii_bubbles_f64(const mxr_hii_params* p, const mxr_hii_grids* g, mxr_hii_globals* out,
+                                double* margin);
+
+/* Bounded timing sample: forward transforms plus the first n_radii radii of the f32
+ * restatement (n_radii <= 0: all).  Returns radii done; *seconds gets the wall time of the
+ * sample and *seconds_fwd the part spent before the radius loop. */
+int oracle_hii_sample_f32(const mxr_hii_params* p, const mxr_hii_grids* g, int n_radii, double* seconds,
+                          double* seconds_fwd);
+
+int oracle_brightness_temperature(const mxr_tb_params* p, const mxr_tb_grids* g, double* volume_ave_Tb);
+
+int oracle_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,161 @@
+"""ctypes binding of oracle/liboracle.so -- the CPU restatement used as the checker.
+
+Test infrastructure: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg only.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(REPO, "meraxes-devs_qyxnew_amd")
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
+
+import mxr  # noqa: E402
+
+ORACLE_DIR = os.path.join(REPO, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(ORACLE_SO):
+        build()
+    L = ctypes.CDLL(ORACLE_SO)
+    fp = ctypes.c_void_p
+    L.oracle_c2c1d_f32.argtypes = [ctypes.c_int, fp, ctypes.c_int]
+    L.oracle_c2c1d_f64.argtypes = [ctypes.c_int, fp, ctypes.c_int]
+    L.oracle_r2c3d_f32.argtypes = [ctypes.c_int, fp, fp]
+    L.oracle_r2c3d_f64.argtypes = [ctypes.c_int, fp, fp]
+    L.oracle_c2r3d_f32.argtypes = [ctypes.c_int, fp]
+    L.oracle_c2r3d_f64.argtypes = [ctypes.c_int, fp]
+    L.oracle_filter_f32.argtypes = [fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_float,
+                                    ctypes.c_int]
+    L.oracle_filter_f64.argtypes = [fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_int]
+    L.oracle_RtoM.argtypes = [ctypes.POINTER(mxr.HiiParams), ctypes.c_double]
+    L.oracle_RtoM.restype = ctypes.c_double
+    L.oracle_hii_radii.argtypes = [ctypes.POINTER(mxr.HiiParams), ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    L.oracle_T_RECFAST.argtypes = [ctypes.c_char_p, ctypes.c_float]
+    L.oracle_T_RECFAST.restype = ctypes.c_double
+    L.oracle_spline_natural.argtypes = [fp, fp, ctypes.c_int, ctypes.c_double]
+    L.oracle_spline_natural.restype = ctypes.c_double
+    L.oracle_hubble_time.argtypes = [ctypes.POINTER(mxr.HiiParams)]
+    L.oracle_hubble_time.restype = ctypes.c_double
+    L.oracle_rho_crit.argtypes = [ctypes.POINTER(mxr.HiiParams)]
+    L.oracle_rho_crit.restype = ctypes.c_double
+    for name in ("oracle_find_hii_bubbles_f32", "oracle_find_hii_bubbles_f64"):
+        getattr(L, name).argtypes = [ctypes.POINTER(mxr.HiiParams), ctypes.POINTER(mxr.HiiGrids),
+                                     ctypes.POINTER(mxr.HiiGlobals), fp]
+    L.oracle_hii_sample_f32.argtypes = [ctypes.POINTER(mxr.HiiParams), ctypes.POINTER(mxr.HiiGrids), ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.oracle_brightness_temperature.argtypes = [ctypes.POINTER(mxr.TbParams), ctypes.POINTER(mxr.TbGrids),
+                                                ctypes.POINTER(ctypes.c_double)]
+    L.oracle_num_threads.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data
+
+
+def c2c1d(x: np.ndarray, sign: int) -> np.ndarray:
+    dt = np.complex64 if x.dtype == np.complex64 else np.complex128
+    y = np.ascontiguousarray(x.astype(dt))
+    fn = lib().oracle_c2c1d_f32 if dt == np.complex64 else lib().oracle_c2c1d_f64
+    assert fn(len(y), _p(y), sign) == 0
+    return y
+
+
+def r2c3d(padded: np.ndarray, N: int) -> np.ndarray:
+    H = N // 2 + 1
+    if padded.dtype == np.float32:
+        out = np.zeros(N * N * H, np.complex64)
+        assert lib().oracle_r2c3d_f32(N, _p(np.ascontiguousarray(padded)), _p(out)) == 0
+    else:
+        out = np.zeros(N * N * H, np.complex128)
+        assert lib().oracle_r2c3d_f64(N, _p(np.ascontiguousarray(padded, np.float64)), _p(out)) == 0
+    return out
+
+
+def c2r3d(spec: np.ndarray, N: int) -> np.ndarray:
+    """Returns the padded real output (2*Nc reals)."""
+    y = np.ascontiguousarray(spec.copy())
+    if y.dtype == np.complex64:
+        assert lib().oracle_c2r3d_f32(N, _p(y)) == 0
+        return y.view(np.float32)
+    assert lib().oracle_c2r3d_f64(N, _p(y)) == 0
+    return y.view(np.float64)
+
+
+def filter_box(spec: np.ndarray, N: int, box: float, R: float, ftype: int) -> np.ndarray:
+    y = np.ascontiguousarray(spec.copy())
+    if y.dtype == np.complex64:
+        rc = lib().oracle_filter_f32(_p(y), 0, N, N, box, R, ftype)
+    else:
+        rc = lib().oracle_filter_f64(_p(y), 0, N, N, box, R, ftype)
+    if rc:
+        raise mxr.MxrError(rc, "oracle_filter")
+    return y
+
+
+def radii(params) -> list:
+    n = lib().oracle_hii_radii(ctypes.byref(params), None, 0)
+    buf = (ctypes.c_double * n)()
+    lib().oracle_hii_radii(ctypes.byref(params), buf, n)
+    return list(buf)
+
+
+def find_hii_bubbles(params, inputs: dict, state: "mxr.HiiState", precision="f32", margin=False):
+    N = params.ReionGridDim
+    arrays = dict(inputs)
+    arrays.update(state.arrays())
+    g = mxr.hii_grids(**arrays)
+    out = mxr.HiiGlobals()
+    m = np.zeros(N ** 3, np.float64) if margin else None
+    fn = lib().oracle_find_hii_bubbles_f32 if precision == "f32" else lib().oracle_find_hii_bubbles_f64
+    rc = fn(ctypes.byref(params), ctypes.byref(g), ctypes.byref(out), _p(m) if margin else None)
+    if rc:
+        raise mxr.MxrError(rc, "oracle_find_hii_bubbles")
+    return (out, m) if margin else out
+
+
+def hii_sample(params, inputs: dict, state, n_radii: int):
+    """Returns (radii done, total seconds, seconds of the forward transforms)."""
+    arrays = dict(inputs)
+    arrays.update(state.arrays())
+    g = mxr.hii_grids(**arrays)
+    secs, fwd = ctypes.c_double(), ctypes.c_double()
+    n = lib().oracle_hii_sample_f32(ctypes.byref(params), ctypes.byref(g), n_radii, ctypes.byref(secs),
+                                    ctypes.byref(fwd))
+    if n < 0:
+        raise mxr.MxrError(n, "oracle_hii_sample_f32")
+    return n, secs.value, fwd.value
+
+
+def brightness_temperature(params, xH, deltax, TS_box=None):
+    N = params.ReionGridDim
+    dT = np.zeros(N ** 3, np.float32)
+    g = mxr.TbGrids()
+    g.mem = 0
+    g.xH, g.deltax, g.TS_box, g.delta_T = _p(xH), _p(deltax), (_p(TS_box) if TS_box is not None else None), _p(dT)
+    ave = ctypes.c_double()
+    rc = lib().oracle_brightness_temperature(ctypes.byref(params), ctypes.byref(g), ctypes.byref(ave))
+    if rc:
+        raise mxr.MxrError(rc, "oracle_brightness_temperature")
+    return dT, ave.value

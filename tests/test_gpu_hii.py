@@ -1,0 +1,238 @@
+"""GPU parity of find_HII_bubbles (libmxr.so through its C ABI) against the oracle.
+
+All tests here need an MI355X and run on it; inputs are the seeded synthetic grids of
+synth.py, fed identically to the HIP path and to the CPU restatement.
+"""
+import numpy as np
+import pytest
+
+import mxr
+import oracle_lib as O
+from conftest import realisation
+from parity import check, compare
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(N):
+    st = mxr.HiiState.host(N)
+    return st
+
+
+def _run_gpu(ctx, params, inputs, state):
+    arrays = dict(inputs)
+    arrays.update(state.arrays())
+    return ctx.find_hii_bubbles(params, **arrays)
+
+
+def _as_dict(st):
+    return {k: np.asarray(v) for k, v in st.arrays().items()}
+
+
+@pytest.fixture(scope="module")
+def ctxs():
+    cache = {}
+
+    def get(N, path=mxr.MXR_PATH_FUSED):
+        if N not in cache:
+            cache[N] = mxr.Context(N)
+        cache[N].set_path(path)
+        return cache[N]
+
+    yield get
+    for c in cache.values():
+        c.close()
+
+
+@pytest.mark.parametrize("N,box,ftype", [(16, 20.0, 1), (32, 40.0, 1), (64, 67.8, 0), (64, 67.8, 1), (64, 67.8, 2),
+                                         (96, 100.0, 1), (128, 125.0, 1), (160, 150.0, 1)])
+def test_parity_vs_oracle(ctxs, N, box, ftype):
+    print(f"parity N={N} box={box} filter={ftype}", flush=True)
+    inp = realisation(N, box, ftype)
+    p = mxr.hii_params(N, box, filter_type=ftype)
+    sg, sr = _state(N), _state(N)
+    gg = _run_gpu(ctxs(N), p, inp, sg)
+    rg, margin = O.find_hii_bubbles(p, inp, sr, margin=True)
+    st = compare(N, _as_dict(sg), _as_dict(sr), gg, rg, margin)
+    print(st)
+    check(st)
+    assert gg.n_radii == rg.n_radii
+    assert abs(gg.mass_weighted_global_xH - rg.mass_weighted_global_xH) <= 1e-5 + st["mismatch"] * 4 / N ** 3
+    assert abs(gg.volume_weighted_global_weighted_sfr / rg.volume_weighted_global_weighted_sfr - 1) < 1e-12
+    assert abs(gg.volume_weighted_global_temp_kinetic_all_gas / rg.volume_weighted_global_temp_kinetic_all_gas
+               - 1) < 1e-4 + st["mismatch"] / N ** 3 * 10
+
+
+def test_parity_256(ctxs):
+    N, box = 256, 200.0
+    print("parity N=256", flush=True)
+    inp = realisation(N, box, 1)
+    p = mxr.hii_params(N, box, filter_type=1)
+    sg, sr = _state(N), _state(N)
+    gg = _run_gpu(ctxs(N), p, inp, sg)
+    rg, margin = O.find_hii_bubbles(p, inp, sr, margin=True)
+    st = compare(N, _as_dict(sg), _as_dict(sr), gg, rg, margin)
+    print(st)
+    check(st)
+
+
+def test_spin_temp_grid(ctxs):
+    """Flag_IncludeSpinTemp: x_e_box is a 4th filtered grid and Tk_box sets the temperatures."""
+    N, box = 64, 67.8
+    inp = realisation(N, box, 1)
+    r = np.random.default_rng(3)
+    H = N // 2 + 1
+    xe = np.zeros((N, N, 2 * H), np.float32)
+    xe[:, :, :N] = r.uniform(0.0, 0.3, (N, N, N)).astype(np.float32)
+    inp["x_e_box"] = xe.reshape(-1)
+    inp["Tk_box"] = r.uniform(5.0, 50.0, N ** 3).astype(np.float32)
+    p = mxr.hii_params(N, box, filter_type=1, spin_temp=True)
+    sg, sr = _state(N), _state(N)
+    gg = _run_gpu(ctxs(N), p, inp, sg)
+    rg, margin = O.find_hii_bubbles(p, inp, sr, margin=True)
+    st = compare(N, _as_dict(sg), _as_dict(sr), gg, rg, margin)
+    print(st)
+    check(st)
+
+
+@pytest.mark.parametrize("ftype", [0, 1, 2])
+def test_paths_bit_identical(ctxs, ftype):
+    """PLAIN, FUSED and FUSED_DENSE compute the same filtered fields: identical state grids."""
+    N, box = 64, 67.8
+    inp = realisation(N, box, ftype)
+    p = mxr.hii_params(N, box, filter_type=ftype)
+    outs = {}
+    for path in (mxr.MXR_PATH_FUSED, mxr.MXR_PATH_FUSED_DENSE, mxr.MXR_PATH_PLAIN):
+        st = _state(N)
+        g = _run_gpu(ctxs(N, path), p, inp, st)
+        outs[path] = (_as_dict(st), g)
+    ref, rg = outs[mxr.MXR_PATH_FUSED]
+    for path in (mxr.MXR_PATH_FUSED_DENSE, mxr.MXR_PATH_PLAIN):
+        o, g = outs[path]
+        for k in ref:
+            np.testing.assert_array_equal(o[k], ref[k], err_msg=f"path {path} field {k}")
+        assert abs(g.volume_weighted_global_xH - rg.volume_weighted_global_xH) < 1e-12
+    ctxs(N, mxr.MXR_PATH_FUSED)
+
+
+def test_pruned_equals_dense_512(ctxs):
+    """At the benchmark size the mode-skipping path equals the dense fused path bit for bit."""
+    import synth
+    import torch
+
+    N, box = 512, 400.0
+    print("512 pruned vs dense", flush=True)
+    inp = {k: v.contiguous() for k, v in synth.realisation(N, box, 42, device="cuda", stars_to_matter=0.05).items()}
+    p = mxr.hii_params(N, box, filter_type=1)
+    res = []
+    for path in (mxr.MXR_PATH_FUSED, mxr.MXR_PATH_FUSED_DENSE):
+        n = N ** 3
+        st = mxr.HiiState(torch.ones(n, device="cuda"), torch.full((n,), -1.0, device="cuda"),
+                          torch.zeros(n, device="cuda"), torch.zeros(n, device="cuda"), torch.zeros(n, device="cuda"))
+        g = _run_gpu(ctxs(N, path), p, inp, st)
+        torch.cuda.synchronize()
+        res.append((st, g))
+    (a, ga), (b, gb) = res
+    for k in ("xH", "z_at_ionization", "J_21_at_ionization", "temp_kinetic_all_gas"):
+        assert torch.equal(getattr(a, k), getattr(b, k)), k
+    xh = a.xH
+    assert float(xh.min()) >= 0.0 and float(xh.max()) <= 1.0
+    assert abs(ga.volume_weighted_global_xH - float(xh.double().mean())) < 1e-9
+    print("512 xH", ga.volume_weighted_global_xH, "ms", ga.device_ms, gb.device_ms)
+    ctxs(N, mxr.MXR_PATH_FUSED)
+
+
+def test_device_pointers_match_host(ctxs):
+    import torch
+
+    N, box = 64, 67.8
+    inp = realisation(N, box, 1)
+    p = mxr.hii_params(N, box, filter_type=1)
+    sh = _state(N)
+    gh = _run_gpu(ctxs(N), p, inp, sh)
+    dinp = {k: torch.from_numpy(v).cuda() for k, v in inp.items()}
+    n = N ** 3
+    sd = mxr.HiiState(torch.ones(n, device="cuda"), torch.full((n,), -1.0, device="cuda"),
+                      torch.zeros(n, device="cuda"), torch.zeros(n, device="cuda"), torch.zeros(n, device="cuda"))
+    gd = _run_gpu(ctxs(N), p, dinp, sd)
+    for k, v in sh.arrays().items():
+        np.testing.assert_array_equal(getattr(sd, k).cpu().numpy(), v, err_msg=k)
+    assert gd.volume_weighted_global_xH == gh.volume_weighted_global_xH
+
+
+def test_two_snapshots_z_in_persists(ctxs):
+    """z_at_ionization set at the first crossing and carried to the next call (z decreasing)."""
+    N, box = 64, 67.8
+    inp = realisation(N, box, 1)
+    sg, sr = _state(N), _state(N)
+    for z, zeta in ((9.0, 0.5), (8.0, 1.0)):
+        p = mxr.hii_params(N, box, filter_type=1, redshift=z, zeta=zeta)
+        gg = _run_gpu(ctxs(N), p, inp, sg)
+        rg, margin = O.find_hii_bubbles(p, inp, sr, margin=True)
+        st = compare(N, _as_dict(sg), _as_dict(sr), gg, rg, margin)
+        check(st)
+    z = np.asarray(sg.z_at_ionization)
+    assert set(np.unique(z)) <= {-1.0, 9.0, 8.0}
+    assert (z == 9.0).sum() > 0 and (z == 8.0).sum() > 0
+
+
+def test_monotone_in_efficiency(ctxs):
+    N, box = 64, 67.8
+    inp = realisation(N, box, 1)
+    prev = 1.1
+    for zeta in (0.5, 1.0, 2.0):
+        st = _state(N)
+        g = _run_gpu(ctxs(N), mxr.hii_params(N, box, zeta=zeta), inp, st)
+        assert g.volume_weighted_global_xH <= prev
+        prev = g.volume_weighted_global_xH
+        ion_now = np.asarray(st.xH) == 0
+        if zeta > 0.5:
+            assert np.all(ion_now[ion_prev])
+        ion_prev = ion_now
+
+
+def test_error_codes(ctxs):
+    N = 32
+    inp = realisation(N, 40.0, 1)
+    st = _state(N)
+    for over, code in ((dict(ReionFilterType=3), mxr.MXR_ERR_FILTER),
+                       (dict(Flag_IncludeRecombinations=1), mxr.MXR_ERR_UNSUPPORTED),
+                       (dict(ReionGridDim=64), mxr.MXR_ERR_INVALID_ARG)):
+        p = mxr.hii_params(N, 40.0, **over)
+        with pytest.raises(mxr.MxrError) as e:
+            _run_gpu(ctxs(N), p, inp, st)
+        assert e.value.code == code
+
+
+@pytest.mark.parametrize("ftype", [0, 1, 2])
+def test_filter_entry_point(ctxs, ftype):
+    """mxr_filter against the oracle's filter on a random half spectrum."""
+    N, box, R = 64, 67.8, 7.3
+    r = np.random.default_rng(ftype)
+    H = N // 2 + 1
+    spec = (r.standard_normal(N * N * H) + 1j * r.standard_normal(N * N * H)).astype(np.complex64)
+    ref = O.filter_box(spec, N, box, R, ftype)
+    got = spec.copy()
+    ctxs(N).filter(got, 0, N, N, box, R, ftype)
+    if ftype == 1:
+        np.testing.assert_array_equal(got, ref)
+    else:
+        np.testing.assert_allclose(got, ref, rtol=2e-6, atol=2e-7)
+
+
+def test_brightness_temperature(ctxs):
+    N = 64
+    r = np.random.default_rng(5)
+    H = N // 2 + 1
+    xH = r.uniform(0, 1, N ** 3).astype(np.float32)
+    dx = np.zeros((N, N, 2 * H), np.float32)
+    dx[:, :, :N] = r.lognormal(0, 0.5, (N, N, N)) - 1
+    dx = dx.reshape(-1)
+    TS = r.uniform(3, 300, N ** 3).astype(np.float32)
+    for spin in (False, True):
+        p = mxr.tb_params(N, 8.0, spin)
+        ref, ave_r = O.brightness_temperature(p, xH, dx, TS if spin else None)
+        got = np.zeros(N ** 3, np.float32)
+        ave = ctxs(N).brightness_temperature(p, xH, dx, got, TS if spin else None)
+        np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-6)
+        assert abs(ave / ave_r - 1) < 1e-9
