@@ -77,8 +77,10 @@ def main():
     ap.add_argument("--N", type=int, default=512)
     ap.add_argument("--box", type=float, default=400.0)
     ap.add_argument("--seed", type=int, default=2024)
+    # bisected once on this GPU path so that seed 2024 ends at a global xH of 0.500
+    ap.add_argument("--stars-to-matter", type=float, default=0.010438)
     ap.add_argument("--plain-steps", type=int, default=2)
-    ap.add_argument("--cpu-radii", type=int, default=2, help="radii in the CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu-radii", type=int, default=8, help="radii in the CPU-baseline sample (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -97,7 +99,7 @@ def main():
     log(f"bench: N={N} box={box} radii={n_radii} generating inputs")
     t0 = time.time()
     inputs = {k: v.contiguous() for k, v in
-              synth.realisation(N, box, args.seed, device="cuda", stars_to_matter=0.45).items()}
+              synth.realisation(N, box, args.seed, device="cuda", stars_to_matter=args.stars_to_matter).items()}
     torch.cuda.synchronize()
     log(f"bench: inputs ready in {time.time() - t0:.1f} s")
     st = make_state(torch, N ** 3)

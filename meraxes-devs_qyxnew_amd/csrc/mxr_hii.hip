@@ -81,8 +81,9 @@ __device__ __forceinline__ bool sharpk_keep(int nx, int ny, int nz, const FiltAr
   return !(s > 1.0f);
 }
 
-// multiplicative window of types 0 (real-space top-hat) and 2 (Gaussian)
-__device__ __forceinline__ float window_mul(int nx, int ny, int nz, const FiltArgs& fa)
+// multiplicative window of types 0 (real-space top-hat) and 2 (Gaussian); one out-of-line copy
+// serves every kernel, so the fused and plain paths evaluate identical code
+__device__ __noinline__ float window_mul(int nx, int ny, int nz, const FiltArgs& fa)
 {
   float kR = __fmul_rn(kmag_ref(nx, ny, nz, fa.N, fa.dk), fa.R);
   if (fa.ftype == 0) {
@@ -152,7 +153,8 @@ __global__ void __launch_bounds__(NT) k_col(CGridPtrs in, GridPtrs out, ColGeom 
           } else {
             const float w = window_mul(e, fj, fk, fa);
             x = src[base + (long)e * geo.es];
-            x = make_float2(x.x * w, x.y * w);
+            // rounded on its own (no fusion into the first butterfly), as filter() does
+            x = make_float2(__fmul_rn(x.x, w), __fmul_rn(x.y, w));
           }
         } else if constexpr (MODE == COL_YPRUNE) {
           if (!fa.apply || !fa.prune || sharpk_keep(0, e, fk, fa))
@@ -549,7 +551,7 @@ __global__ void k_filter(float2* __restrict__ box, int local_ix_start, int slab_
     } else {
       const float w = window_mul(nx, ny, nz, fa);
       const float2 x = box[idx];
-      box[idx] = make_float2(x.x * w, x.y * w);
+      box[idx] = make_float2(__fmul_rn(x.x, w), __fmul_rn(x.y, w));
     }
   }
 }
@@ -943,7 +945,7 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
     fa.dk = (float)(2.0 * M_PI / bs);
   }
   fa.ftype = p->ReionFilterType;
-  fa.prune = (c->path == MXR_PATH_FUSED) ? 1 : 0;
+  fa.prune = (c->path == MXR_PATH_FUSED && p->ReionFilterType == 1) ? 1 : 0;  // only sharp-k zeroes modes
 
   CRealPtrs rin{};
   GridPtrs spec{}, work{};

@@ -28,7 +28,7 @@ CALIB = {
     (96, 100.0, 1): (1096, 0.01253),
     (128, 125.0, 1): (1128, 0.19495),
     (160, 150.0, 1): (1160, 0.02558),
-    (256, 200.0, 1): (1256, 0.05),
+    (256, 200.0, 1): (1256, 0.020055),
 }
 
 _cache = {}

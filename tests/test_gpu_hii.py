@@ -55,7 +55,7 @@ def test_parity_vs_oracle(ctxs, N, box, ftype):
     rg, margin = O.find_hii_bubbles(p, inp, sr, margin=True)
     st = compare(N, _as_dict(sg), _as_dict(sr), gg, rg, margin)
     print(st)
-    check(st)
+    check(st, p999_tol=1e-4 if ftype == 1 else 5e-4)
     assert gg.n_radii == rg.n_radii
     assert abs(gg.mass_weighted_global_xH - rg.mass_weighted_global_xH) <= 1e-5 + st["mismatch"] * 4 / N ** 3
     assert abs(gg.volume_weighted_global_weighted_sfr / rg.volume_weighted_global_weighted_sfr - 1) < 1e-12
@@ -110,7 +110,8 @@ def test_paths_bit_identical(ctxs, ftype):
     for path in (mxr.MXR_PATH_FUSED_DENSE, mxr.MXR_PATH_PLAIN):
         o, g = outs[path]
         for k in ref:
-            np.testing.assert_array_equal(o[k], ref[k], err_msg=f"path {path} field {k}")
+            nd = int((o[k] != ref[k]).sum())
+            assert nd == 0, f"path {path} field {k}: {nd} cells differ"
         assert abs(g.volume_weighted_global_xH - rg.volume_weighted_global_xH) < 1e-12
     ctxs(N, mxr.MXR_PATH_FUSED)
 
@@ -236,3 +237,29 @@ def test_brightness_temperature(ctxs):
         ave = ctxs(N).brightness_temperature(p, xH, dx, got, TS if spin else None)
         np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-6)
         assert abs(ave / ave_r - 1) < 1e-9
+
+
+@pytest.mark.parametrize("ftype", [0, 1])
+def test_error_vs_fp64_oracle(ctxs, ftype):
+    """Anchor on the FP64 oracle: the GPU's per-cell error is no larger than the FP32
+    restatement's own error (both are FP32 FFTs of the same fields)."""
+    N, box = 64, 67.8
+    inp = realisation(N, box, ftype)
+    p = mxr.hii_params(N, box, filter_type=ftype)
+    sg, s32, s64 = _state(N), _state(N), _state(N)
+    gg = _run_gpu(ctxs(N), p, inp, sg)
+    g32 = O.find_hii_bubbles(p, inp, s32)
+    g64, margin = O.find_hii_bubbles(p, inp, s64, precision="f64", margin=True)
+    both = (np.asarray(sg.xH) > 0) & (s32.xH > 0) & (s64.xH > 0)
+    e_gpu = np.abs(np.asarray(sg.xH)[both] - s64.xH[both])
+    e_32 = np.abs(s32.xH[both] - s64.xH[both])
+    print("xH error vs FP64: gpu max", e_gpu.max(), "p99.9", np.quantile(e_gpu, 0.999), "| f32 restatement max",
+          e_32.max(), "p99.9", np.quantile(e_32, 0.999))
+    assert e_gpu.max() <= 2 * e_32.max() + 1e-6
+    assert np.quantile(e_gpu, 0.999) <= 2 * np.quantile(e_32, 0.999) + 1e-6
+    mg = (np.asarray(sg.xH) == 0) != (s64.xH == 0)
+    m32 = (s32.xH == 0) != (s64.xH == 0)
+    assert mg.sum() <= max(2, 2 * m32.sum())
+    assert np.all(np.abs(margin[mg]) < 1e-3)
+    assert abs(gg.volume_weighted_global_xH - g64.volume_weighted_global_xH) <= \
+        2 * abs(g32.volume_weighted_global_xH - g64.volume_weighted_global_xH) + 1e-6
