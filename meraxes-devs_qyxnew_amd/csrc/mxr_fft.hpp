@@ -195,11 +195,35 @@ template <int P>
 struct LdsRow {
   __device__ static int at(int c, int e) { return c * P + e; }
 };
+// Row-major with one pad slot after every 8 elements: the first radix-8 stage writes its
+// outputs 8 apart (j*8 + r), which on the unpadded layout lands 16 lanes on 4 bank groups;
+// with the pad consecutive lanes are 9 slots (18 banks) apart and the 16 lanes never collide.
+// P must be at least L + L/8 + 1.
+template <int P>
+struct LdsRowPad {
+  __device__ static int at(int c, int e) { return c * P + e + (e >> 3); }
+};
+
+// Synchronisation between stages.  SyncBlock: the transforms of a tile span several waves.
+// SyncWave: every transform lives inside one wavefront (its threads are consecutive lanes), so
+// LDS traffic between stages only has to be ordered within the wave: LDS instructions of a wave
+// execute in order, and the wave barrier keeps the compiler from moving LDS accesses across it.
+struct SyncBlock {
+  __device__ static void sync() { __syncthreads(); }
+};
+struct SyncWave {
+  __device__ static void sync()
+  {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+};
 
 // Middle and last stages.  `v` holds the outputs of stage s-1 in its natural slots; on return
 // it holds the outputs of the last stage: slot (q, r) <-> element j + r L/Rlast,
 // j = t + T q.  lds is reused between stages; tw is the LDS twiddle table of length LT.
-template <int L, int T, int S, int LT, class Idx, int s>
+template <int L, int T, int S, int LT, class Idx, int s, class Sync = SyncBlock>
 __device__ __forceinline__ void stages_from(float2* v, float2* lds, const float2* tw, int c, int t)
 {
   using F = Factor<L>;
@@ -220,7 +244,7 @@ __device__ __forceinline__ void stages_from(float2* v, float2* lds, const float2
           lds[Idx::at(c, base + r * Nsp)] = v[q * Rp + r];
       }
     }
-    __syncthreads();
+    Sync::sync();
     constexpr int R = F::radix(s);
     constexpr int B = L / R;
     constexpr int Q = cdiv(B, T);
@@ -237,19 +261,19 @@ __device__ __forceinline__ void stages_from(float2* v, float2* lds, const float2
           v[q * R + r] = ctw<S>(lds[Idx::at(c, j + r * B)], tw[k * r * step]);
       }
     }
-    __syncthreads();
+    Sync::sync();
 #pragma unroll
     for (int q = 0; q < Q; q++) {
       const int j = t + T * q;
       if (B % T == 0 || j < B)
         dft<R, S>(v + q * R);
     }
-    stages_from<L, T, S, LT, Idx, s + 1>(v, lds, tw, c, t);
+    stages_from<L, T, S, LT, Idx, s + 1, Sync>(v, lds, tw, c, t);
   }
 }
 
 // Full transform given stage-0 inputs in v: slot (q, r) = x[j + r L/R0], j = t + T q.
-template <int L, int T, int S, int LT, class Idx>
+template <int L, int T, int S, int LT, class Idx, class Sync = SyncBlock>
 __device__ __forceinline__ void tile_fft(float2* v, float2* lds, const float2* tw, int c, int t)
 {
   using F = Factor<L>;
@@ -262,7 +286,7 @@ __device__ __forceinline__ void tile_fft(float2* v, float2* lds, const float2* t
     if (B0 % T == 0 || j < B0)
       dft<R0, S>(v + q * R0);
   }
-  stages_from<L, T, S, LT, Idx, 1>(v, lds, tw, c, t);
+  stages_from<L, T, S, LT, Idx, 1, Sync>(v, lds, tw, c, t);
 }
 
 // Slot geometry of the first and last stage, for callers doing their own loads / stores.

@@ -61,7 +61,7 @@ __device__ __forceinline__ auto pick(const P& ps, int g)
 {
   return g == 0 ? ps.p[0] : g == 1 ? ps.p[1] : g == 2 ? ps.p[2] : ps.p[3];
 }
-enum { C2R_STORE = 0, C2R_HII = 1 };
+enum { C2R_STORE = 0, C2R_HII = 1, C2R_HII_SPIN = 2 };
 
 // ---- the reference's window arithmetic (float, unfused) --------------------------------
 __device__ __forceinline__ float kmag_ref(int nx, int ny, int nz, int N, float dk)
@@ -192,8 +192,9 @@ __global__ void __launch_bounds__(NT) k_col(CGridPtrs in, GridPtrs out, ColGeom 
 template <int N, int NT, int TR>
 __global__ void __launch_bounds__(NT) k_row_r2c(CRealPtrs in, GridPtrs out, long nrows, const float2* __restrict__ twg)
 {
-  constexpr int M = N / 2, H = N / 2 + 1, T = NT / TR, P = M + 1;
+  constexpr int M = N / 2, H = N / 2 + 1, T = NT / TR, P = M + M / 8 + 1;
   using SL = Slots<M, T>;
+  using LI = LdsRowPad<P>;
   __shared__ float2 lds[TR * P];
   __shared__ float2 tw[N];
   const int g = blockIdx.y;
@@ -217,22 +218,22 @@ __global__ void __launch_bounds__(NT) k_row_r2c(CRealPtrs in, GridPtrs out, long
     }
   }
   __syncthreads();  // twiddles visible (single-stage transforms use them below)
-  tile_fft<M, T, -1, N, LdsRow<P>>(v, lds, tw, c, t);
+  tile_fft<M, T, -1, N, LI, SyncWave>(v, lds, tw, c, t);
 #pragma unroll
   for (int q = 0; q < SL::QL; q++) {
     const int j = t + T * q;
     if (SL::BL % T == 0 || j < SL::BL)
 #pragma unroll
       for (int r = 0; r < SL::RL; r++)
-        lds[c * P + j + r * SL::BL] = v[q * SL::RL + r];
+        lds[LI::at(c, j + r * SL::BL)] = v[q * SL::RL + r];
   }
   __syncthreads();
   if (!valid)
     return;
   float2* __restrict__ dst = pick(out, g) + row * H;
   for (int k = t; k <= M; k += T) {
-    const float2 zk = lds[c * P + (k % M)];
-    const float2 zm0 = lds[c * P + ((M - k) % M)];
+    const float2 zk = lds[LI::at(c, k % M)];
+    const float2 zm0 = lds[LI::at(c, (M - k) % M)];
     const float2 zm = make_float2(zm0.x, -zm0.y);
     const float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
     const float2 d = make_float2(0.5f * (zk.x - zm.x), 0.5f * (zk.y - zm.y));
@@ -321,106 +322,123 @@ __device__ __forceinline__ void block_sum(double* acc, double* out)
   }
 }
 
-// One cell of the fused update.  df/sf/xf: filtered deltax/stars/x_e (already c2r'd).
-__device__ __forceinline__ void hii_cell(const HiiEpi& ea, long ir, long ip, float df, float sf, float wf, float xf,
-                                         double* acc)
+// Clamp (find_HII_bubbles.c:377-411) and criterion (:457-485) of one cell, in the reference's
+// types: returns whether the cell ionises at this radius and, when `need_lhs`, the exact
+// fl(fl(sf / dom) pref) that the last step's partial xH uses.  The quotient is only formed
+// when the division-free comparison sf pref vs nf dom is within 1e-12 of a tie; elsewhere
+// both orderings agree exactly.
+__device__ __forceinline__ bool hii_criterion(const HiiEpi& ea, float df, float& sf, float xf, bool need_lhs,
+                                              double& lhs, double& nf)
 {
 #pragma clang fp contract(off)
-  // aliasing clamp, find_HII_bubbles.c:377-411
   df = fmaxf(df, -1.0f + REL_TOL);
   sf = fmaxf(sf, 0.0f);
   if (sf < ABS_TOL)
     sf = 0.0f;
-  wf = fmaxf(wf, 0.0f);
-  if (wf < ABS_TOL)
-    wf = 0.0f;
-  (void)wf;  // weighted_sfr_filtered enters only the recombination branch
   if (ea.spin) {
     xf = fmaxf(xf, 0.0f);
     if (xf < ABS_TOL)
       xf = 0.0f;
     xf = fminf(xf, 0.999f);
   }
-  // criterion, :457-485 (double)
   const double density_over_mean = 1.0 + (double)df;
+  nf = ea.spin ? 1.0 - (double)xf : 1.0;
+  const double a = (double)sf * ea.f_coll_prefactor;
+  const double b = nf * density_over_mean;
+  if (!need_lhs && a > b * (1.0 + 1e-12))
+    return true;
+  if (!need_lhs && a < b * (1.0 - 1e-12))
+    return false;
   const double f_coll_stars = (double)sf / density_over_mean;
-  const double neutral_fraction = ea.spin ? 1.0 - (double)xf : 1.0;
-  const double lhs = (0.0 + f_coll_stars) * ea.f_coll_prefactor;
-  const bool ionise = lhs > neutral_fraction * (1. + 0.0);
+  lhs = (0.0 + f_coll_stars) * ea.f_coll_prefactor;
+  return lhs > nf * (1. + 0.0);
+}
 
-  const bool was = ea.first ? false : (ea.ion[ir] != 0);
-  float xh = was ? 0.0f : 1.0f;
-  bool partial = false;
-  float T_HI = 0.0f;
-  if (!was) {
-    if (ionise) {
-      xh = 0.0f;
-      ea.ion[ir] = 1;
-    } else {
-      if (ea.first)
-        ea.ion[ir] = 0;
-      if (ea.last) {  // :516-528
-        T_HI = ea.spin ? ea.Tk_box[ir] : thermal_T(ea.TK, ea.cT_ad, ea.deltax[ip]);
-        partial = true;
-        xh = (float)(neutral_fraction - lhs);
-        if (xh < 0.0f)
-          xh = 0.0f;
-        else if (xh > 1.0f)
-          xh = 1.0f;
+// The fused update of two adjacent cells ir, ir+1 (ip, ip+1 in the padded layout).
+// Per radius (:486-543): a cell not yet ionised in this call that passes the criterion is
+// flagged; on its first crossing (z_at_ionization < 0) z_at_ionization and, for UVBFlag 1,
+// J_21_at_ionization are set.  On the last (unfiltered) radius every cell also gets its final
+// xH (0, or the clamped partial 1 - f_coll) and the not-ionised ones their partial-ionisation
+// temperature T_HI (ComputePartiallyIoinizedTemperature(T_HI, xH = 1) = T_HI); the post-loop
+// temperatures and the FP64 sums follow in k_postloop.
+__device__ __forceinline__ void hii_pair(const HiiEpi& ea, int ir, int ip, float2 df, float2 sf, float2 xf)
+{
+#pragma clang fp contract(off)
+  uchar2 was = make_uchar2(0, 0);
+  if (!ea.first)
+    was = *reinterpret_cast<const uchar2*>(ea.ion + ir);
+  const float dfs[2] = {df.x, df.y};
+  float sfs[2] = {sf.x, sf.y};
+  const float xfs[2] = {xf.x, xf.y};
+  const bool w[2] = {was.x != 0, was.y != 0};
+  bool ion[2], cross[2];
+  float xh[2];
+#pragma unroll
+  for (int e = 0; e < 2; e++) {
+    double lhs = 0.0, nf = 1.0;
+    ion[e] = w[e];
+    xh[e] = w[e] ? 0.0f : 1.0f;
+    cross[e] = false;
+    if (!w[e]) {
+      const bool i = hii_criterion(ea, dfs[e], sfs[e], xfs[e], ea.last != 0, lhs, nf);
+      ion[e] = i;
+      if (i) {
+        xh[e] = 0.0f;
+      } else if (ea.last) {
+        float v = (float)(nf - lhs);
+        xh[e] = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
       }
-    }
-    if (xh < REL_TOL) {  // :531-543 (z_in read only on a new crossing)
-      if (ea.z_in[ir] < 0) {
-        ea.z_in[ir] = ea.z;
-        if (ea.uvb == 1)
-          ea.J21[ir] = __fmul_rn(sf, ea.J21c);
-      }
+      cross[e] = xh[e] < REL_TOL;
     }
   }
-  if (!ea.last)
-    return;
-  // post-loop, :577-611
-  const float dx_in = ea.deltax[ip];
-  const float zin = ea.z_in[ir];
-  float temp;
-  if (zin > 0 && xh < REL_TOL)
-    temp = fully_ionized_T(zin, ea.z, dx_in);
-  else if (partial)
-    temp = T_HI;  // ComputePartiallyIoinizedTemperature(T_HI, xH = 1) = T_HI
-  else
-    temp = ea.temp[ir];
-  const float floor_T = ea.spin ? ea.Tk_box[ir] : thermal_T(ea.TK, ea.cT_ad, dx_in);
-  if (temp < floor_T)
-    temp = floor_T;
-  ea.xH[ir] = xh;
-  ea.temp[ir] = temp;
-  ea.r_bubble[ir] = 0.0f;
-  const double dom = 1.0 + (double)dx_in;
-  acc[0] += (double)xh;
-  acc[1] += (double)xh * dom;
-  acc[2] += dom;
-  acc[3] += (double)ea.sfr[ip];
-  acc[4] += (double)temp;
-  acc[5] += (double)temp * dom;
+  if (ea.first || ion[0] != w[0] || ion[1] != w[1])
+    *reinterpret_cast<uchar2*>(ea.ion + ir) = make_uchar2(ion[0] ? 1 : 0, ion[1] ? 1 : 0);
+  if (cross[0] || cross[1]) {  // a new crossing in this call: z_in read only here
+    float2 z = *reinterpret_cast<const float2*>(ea.z_in + ir);
+    const float zs[2] = {z.x, z.y};
+#pragma unroll
+    for (int e = 0; e < 2; e++)
+      if (cross[e] && zs[e] < 0) {
+        ea.z_in[ir + e] = ea.z;
+        if (ea.uvb == 1)
+          ea.J21[ir + e] = __fmul_rn(sfs[e], ea.J21c);
+      }
+  }
+  if (ea.last) {
+    *reinterpret_cast<float2*>(ea.xH + ir) = make_float2(xh[0], xh[1]);
+    *reinterpret_cast<float2*>(ea.r_bubble + ir) = make_float2(0.0f, 0.0f);
+    if (!ion[0] || !ion[1]) {
+      float2 T = *reinterpret_cast<const float2*>(ea.temp + ir);
+      float Ts[2] = {T.x, T.y};
+#pragma unroll
+      for (int e = 0; e < 2; e++)
+        if (!ion[e])
+          Ts[e] = ea.spin ? ea.Tk_box[ir + e] : thermal_T(ea.TK, ea.cT_ad, ea.deltax[ip + e]);
+      *reinterpret_cast<float2*>(ea.temp + ir) = make_float2(Ts[0], Ts[1]);
+    }
+  }
 }
 
 // One grid's rows through the inverse z-pass: load the half-spectrum rows into LDS (only the
-// k the window keeps when pruning), pre-twiddle, M-point inverse FFT.  out receives the
-// last-stage slots: slot (q, r) = y[2n] + i y[2n+1], n = t + T q + r BL.
+// kc leading k the window keeps -- a prefix, since |k| grows with k_z), pre-twiddle, M-point inverse FFT.  out receives the
+// last-stage slots: slot (q, r) = y[2n] + i y[2n+1], n = t + T q + r BL.  A row's T threads
+// are consecutive lanes of one wavefront, so the stages synchronise per wave, not per block.
 template <int N, int T, int TR>
 __device__ __forceinline__ void row_c2r_grid(const float2* __restrict__ base, bool valid, long row, float2* lds,
-                                             const float2* tw, const FiltArgs& fa, int c, int t, float2* out)
+                                             const float2* tw, int kc, int c, int t, float2* out)
 {
-  constexpr int M = N / 2, H = N / 2 + 1, P = M + 1;
+  constexpr int M = N / 2, H = N / 2 + 1, P = M + M / 8 + 1;
+  static_assert(64 % T == 0, "a row must not straddle two wavefronts");
   using SL = Slots<M, T>;
+  using LI = LdsRowPad<P>;
   const float2* __restrict__ src = base + (valid ? row : 0) * H;
   for (int k = t; k < H; k += T) {
     float2 x = make_float2(0.f, 0.f);
-    if (valid && (!fa.apply || !fa.prune || sharpk_keep(0, 0, k, fa)))
+    if (valid && k < kc)
       x = src[k];
-    lds[c * P + k] = x;
+    lds[LI::at(c, k)] = x;
   }
-  __syncthreads();
+  SyncWave::sync();
   float2 v[SL::NREG];
 #pragma unroll
   for (int q = 0; q < SL::Q0; q++) {
@@ -430,8 +448,8 @@ __device__ __forceinline__ void row_c2r_grid(const float2* __restrict__ base, bo
       const int n = j + r * SL::B0;
       float2 z = make_float2(0.f, 0.f);
       if (SL::B0 % T == 0 || j < SL::B0) {
-        const float2 xn = lds[c * P + n];
-        const float2 xm0 = lds[c * P + M - n];
+        const float2 xn = lds[LI::at(c, n)];
+        const float2 xm0 = lds[LI::at(c, M - n)];
         const float2 xm = make_float2(xm0.x, -xm0.y);
         const float2 s = cadd(xn, xm);
         const float2 d = csub(xn, xm);
@@ -442,24 +460,28 @@ __device__ __forceinline__ void row_c2r_grid(const float2* __restrict__ base, bo
       v[q * SL::R0 + r] = z;
     }
   }
-  __syncthreads();
-  tile_fft<M, T, +1, N, LdsRow<P>>(v, lds, tw, c, t);
+  SyncWave::sync();
+  tile_fft<M, T, +1, N, LI, SyncWave>(v, lds, tw, c, t);
 #pragma unroll
   for (int s = 0; s < SL::QL * SL::RL; s++)
     out[s] = v[s];
-  __syncthreads();
+  SyncWave::sync();
 }
 
 // ---- z-pass inverse: half-spectrum rows -> real rows, + store or fused update -----------
 // Z[n] = (X[n] + conj X[M-n]) + i exp(+2 pi i n / N) (X[n] - conj X[M-n]), n < M;
 // an M-point inverse FFT of Z gives y[2n] + i y[2n+1] (unnormalised c2r).
+// C2R_HII keeps the filtered deltax and stars (and x_e) in registers for the update;
+// weighted_sfr is transformed and clamped like the reference does (find_HII_bubbles.c:382-385)
+// but, with recombinations off, nothing reads it afterwards.
 template <int N, int NT, int TR, int EPI>
 __global__ void __launch_bounds__(NT) k_row_c2r(CGridPtrs in, int ngrid, long nrows, const float2* __restrict__ twg,
                                                 FiltArgs fa, HiiEpi ea)
 {
-  constexpr int M = N / 2, H = N / 2 + 1, T = NT / TR, P = M + 1;
+  constexpr int M = N / 2, H = N / 2 + 1, T = NT / TR, P = M + M / 8 + 1;
   using SL = Slots<M, T>;
-  constexpr int NG = (EPI == C2R_HII) ? 4 : 1;
+  constexpr int NS = SL::QL * SL::RL;
+  constexpr int NK = (EPI == C2R_STORE) ? 1 : (EPI == C2R_HII ? 2 : 3);  // grids kept in registers
   __shared__ float2 lds[TR * P];
   __shared__ float2 tw[N];
   const int c = threadIdx.x / T, t = threadIdx.x % T;
@@ -467,19 +489,35 @@ __global__ void __launch_bounds__(NT) k_row_c2r(CGridPtrs in, int ngrid, long nr
   const bool valid = row < nrows;
   for (int m = threadIdx.x; m < N; m += NT)
     tw[m] = twg[m];
-  float2 res[NG][SL::QL * SL::RL];
-  const int g0 = (EPI == C2R_HII) ? 0 : blockIdx.y;
-  row_c2r_grid<N, T, TR>(pick(in, g0), valid, row, lds, tw, fa, c, t, res[0]);
-  if constexpr (NG == 4) {
-    row_c2r_grid<N, T, TR>(in.p[1], valid, row, lds, tw, fa, c, t, res[1]);
-    row_c2r_grid<N, T, TR>(in.p[2], valid, row, lds, tw, fa, c, t, res[2]);
-    if (ngrid > 3)  // uniform across the block
-      row_c2r_grid<N, T, TR>(in.p[3], valid, row, lds, tw, fa, c, t, res[3]);
-    else
-#pragma unroll
-      for (int s = 0; s < SL::QL * SL::RL; s++)
-        res[3][s] = make_float2(0.f, 0.f);
+  // number of k_z the window keeps on the k_x = k_y = 0 line (all of them when not pruning)
+  int kc = H;
+  if (fa.apply && fa.prune) {
+    kc = 0;
+    for (int k0 = 0; k0 < H; k0 += NT) {
+      const int k = k0 + (int)threadIdx.x;
+      kc += __syncthreads_count(k < H && sharpk_keep(0, 0, k, fa));
+    }
   }
+  __syncthreads();
+  // grid order keeps few fields live: weighted_sfr first (transformed, clamped, dropped), then
+  // deltax (and x_e) kept, stars last and consumed straight from the transform's registers
+  float2 res[NK][NS];
+  const int g0 = (EPI == C2R_STORE) ? blockIdx.y : 0;
+  if constexpr (EPI != C2R_STORE) {
+    float2 wsfr[NS];
+    row_c2r_grid<N, T, TR>(in.p[2], valid, row, lds, tw, kc, c, t, wsfr);  // weighted_sfr
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      float wx = fmaxf(wsfr[s].x, 0.0f), wy = fmaxf(wsfr[s].y, 0.0f);
+      wx = wx < ABS_TOL ? 0.0f : wx;
+      wy = wy < ABS_TOL ? 0.0f : wy;
+      asm volatile("" ::"v"(wx), "v"(wy));  // the clamped field is computed, then dropped
+    }
+    if constexpr (EPI == C2R_HII_SPIN)
+      row_c2r_grid<N, T, TR>(in.p[3], valid, row, lds, tw, kc, c, t, res[2]);  // x_e
+    row_c2r_grid<N, T, TR>(in.p[1], valid, row, lds, tw, kc, c, t, res[1]);  // stars
+  }
+  row_c2r_grid<N, T, TR>(pick(in, g0), valid, row, lds, tw, kc, c, t, res[0]);  // deltax / the grid
 
   if constexpr (EPI == C2R_STORE) {
     if (!valid)
@@ -496,27 +534,22 @@ __global__ void __launch_bounds__(NT) k_row_c2r(CGridPtrs in, int ngrid, long nr
         }
     }
   } else {
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    if (valid) {
+    if (!valid)
+      return;
 #pragma unroll
-      for (int q = 0; q < SL::QL; q++) {
-        const int j = t + T * q;
-        if (SL::BL % T == 0 || j < SL::BL)
+    for (int q = 0; q < SL::QL; q++) {
+      const int j = t + T * q;
+      if (SL::BL % T == 0 || j < SL::BL)
 #pragma unroll
-          for (int r = 0; r < SL::RL; r++) {
-            const int n = j + r * SL::BL;
-            const int s = q * SL::RL + r;
-            const float2 d = res[0][s], st = res[1][s], w = res[2][s];
-            const float2 xe = ea.spin ? res[3][s] : make_float2(0.f, 0.f);
-            const long ir = row * N + 2 * n;
-            const long ip = row * 2 * H + 2 * n;
-            hii_cell(ea, ir, ip, d.x, st.x, w.x, xe.x, acc);
-            hii_cell(ea, ir + 1, ip + 1, d.y, st.y, w.y, xe.y, acc);
-          }
-      }
+        for (int r = 0; r < SL::RL; r++) {
+          const int n = j + r * SL::BL;
+          const int s = q * SL::RL + r;
+          const float2 xe = (EPI == C2R_HII_SPIN) ? res[NK - 1][s] : make_float2(0.f, 0.f);
+          const int ir = (int)row * N + 2 * n;  // N^3 < 2^31 for N <= 1024
+          const int ip = (int)row * 2 * H + 2 * n;
+          hii_pair(ea, ir, ip, res[0][s], res[1][s], xe);
+        }
     }
-    if (ea.last)
-      block_sum<NT, 6>(acc, ea.partials + (long)blockIdx.x * 6);
   }
 }
 
@@ -765,14 +798,13 @@ struct Hii {
   static int c2r(Ctx* c, CGridPtrs in, int ngrid, FiltArgs fa, const HiiEpi& ea)
   {
     const long nrows = (long)N * N;
-    dim3 grid((unsigned)((nrows + C::TR - 1) / C::TR), (unsigned)(EPI == C2R_HII ? 1 : ngrid));
+    dim3 grid((unsigned)((nrows + C::TR - 1) / C::TR), (unsigned)(EPI == C2R_STORE ? ngrid : 1));
     hipLaunchKernelGGL((k_row_c2r<N, C::ROW_NT, C::TR, EPI>), grid, dim3(C::ROW_NT), 0, c->stream, in, ngrid, nrows,
                        c->tw, fa, ea);
     MXR_HIP(hipGetLastError());
     return MXR_OK;
   }
 
-  static long c2r_blocks() { return ((long)N * N + C::TR - 1) / C::TR; }
 };
 
 static unsigned ew_blocks(long n)
@@ -1017,13 +1049,17 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
     if (!plain) {
       // kept-mode counts of this radius (all modes when not pruning)
       KeptModes km = (fa.apply && fa.prune && fa.ftype == 1) ? kept_modes(N, fa.dk, fa.R) : KeptModes::all(N);
-      const double state_b = last ? (double)nreal * (1 + 8 + 4 + 12 + (spin ? 4 : 0)) : (double)nreal * 1.0;
+      // flags read (+ written on the first radius); on the last radius xH, r_bubble and temp
+      // written and the deltax / temp (or Tk_box) needed for T_HI read
+      const double state_b = last ? (double)nreal * (1 + 4 + 4 + 4 + 4 + 4) : (double)nreal * 1.0;
       RC(T(K_INV_X, G * 8.0 * ((double)km.k3 + (double)N * km.c2),
            [&] { return K::template col<+1, COL_FILTER>(c, cspec, work, ngrid, K::xgeo(), fa, 1.0); }));
       RC(T(K_INV_Y, G * 8.0 * ((double)N * km.c2 + (double)N * N * km.k1),
            [&] { return K::template col<+1, COL_YPRUNE>(c, cwork, work, ngrid, K::ygeo(), fa, 1.0); }));
-      RC(T(K_INV_Z, G * 8.0 * (double)N * N * km.k1 + state_b,
-           [&] { return K::template c2r<C2R_HII>(c, cwork, ngrid, fa, ea); }));
+      RC(T(K_INV_Z, G * 8.0 * (double)N * N * km.k1 + state_b, [&] {
+        return spin ? K::template c2r<C2R_HII_SPIN>(c, cwork, ngrid, fa, ea)
+                    : K::template c2r<C2R_HII>(c, cwork, ngrid, fa, ea);
+      }));
     } else {
       for (int i = 0; i < ngrid; i++) {
         RC(T(K_COPY, 2 * cplx, [&] {
@@ -1062,10 +1098,10 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
   }
 
   // ---- reductions (:549-664)
-  int nblk = plain ? (int)ew_blocks(nreal) : (int)K::c2r_blocks();
-  RC(T(K_POST, plain ? (double)nreal * (4 + 4 + 4 + 4 + 4 + 4) : 0.0, [&] {
-    if (plain)
-      hipLaunchKernelGGL((k_postloop<256>), dim3(nblk), dim3(256), 0, c->stream, N, ea);
+  // post-loop temperatures and FP64 sums: xH, deltax, z_in, temp, weighted_sfr read, temp written
+  const int nblk = (int)ew_blocks(nreal);
+  RC(T(K_POST, (double)nreal * (4 + 4 + 4 + 4 + 4 + 4 + (spin ? 4 : 0)), [&] {
+    hipLaunchKernelGGL((k_postloop<256>), dim3(nblk), dim3(256), 0, c->stream, N, ea);
     hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, c->stream, c->partials, nblk, 6, c->sums);
     MXR_HIP(hipGetLastError());
     return MXR_OK;
