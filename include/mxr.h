@@ -190,6 +190,120 @@ typedef struct mxr_tb_grids {
 int mxr_brightness_temperature(mxr_ctx* ctx, const mxr_tb_params* params, const mxr_tb_grids* grids,
                                double* volume_ave_Tb);
 
+/* ---- spin temperature ----------------------------------------------------------------------- */
+#define MXR_TS_MAX_STEPS 64  /* TsNumFilterSteps upper bound                                  */
+#define MXR_TS_MAX_OPS 16    /* source-grid operations before one shell's forward FFT          */
+#define MXR_X_INT_NXHII 14   /* x_e bins of the secondary-electron tables                     */
+/* source-grid operations (ComputeTs.c:366-478, load_reion_sfr_grids reionization.c:2750-2797) */
+#define MXR_TS_OP_CLEAR 0    /* grid = 0                                                       */
+#define MXR_TS_OP_SCALE 1    /* grid = (float)((double)grid * weight)                          */
+#define MXR_TS_OP_LOAD 2     /* grid = history[hist] * (float)weight                           */
+#define MXR_TS_OP_ADD 3      /* grid += history[hist] * (float)weight                          */
+
+typedef struct mxr_ts_params {
+  int ReionGridDim;
+  double BoxSize;                 /* Mpc/h                                                 */
+  double Hubble_h;
+  double OmegaM;
+  double OmegaLambda;
+  double OmegaK;
+  double OmegaR;
+  double wLambda;
+  double BaryonFrac;
+  double Y_He;
+  int ReionRtoMFilterType;        /* RtoM of the R_ct = 0 collapse fraction                */
+  int TsNumFilterSteps;           /* shells (<= MXR_TS_MAX_STEPS)                          */
+  int TsHeatingFilterType;        /* window of the shells (reionization.c:3373-3433)       */
+  double ReionMaxHeatingRedshift; /* homogeneous branch at z >= this                       */
+  double ReionEfficiency;
+  double ReionNionPhotPerBary;    /* Pop II Lyman-band photons per baryon                  */
+  double NuXrayThreshold;         /* eV                                                    */
+  double NuXraySoftCut;           /* eV                                                    */
+  double NuXrayMax;               /* eV                                                    */
+  double SpecIndexXrayGal;
+  double UnitLength_in_cm;
+  double UnitMass_in_g;
+  double UnitVelocity_in_cm_per_s;
+  int snapshot;                   /* index of the current snapshot in ZZ                   */
+  const double* ZZ;               /* host array: redshift of snapshots 0..snapshot         */
+  const char* TablesForXHeatingDir; /* recfast_LCDM.dat, kappa_*_table.dat,
+                                       stellar_spectra.dat, x_int_tables/                  */
+} mxr_ts_params;
+
+typedef struct mxr_ts_grids {
+  int mem;                        /* MXR_MEM_HOST or MXR_MEM_DEVICE (grids below)          */
+  const float* deltax;            /* in  padded real                                        */
+  const float* stars;             /* in  padded real (collapse fraction of the shell R_0)   */
+  const float* sfr_histories;     /* in  n_histories padded grids, [s] = s snapshots back;
+                                     [0] is the current sfr grid (reionization.c:2106-2141) */
+  const float* xray_luminosity_histories; /* in, same layout (1e40 erg/s units)             */
+  int n_histories;
+  float* x_e_box_prev;            /* in/out padded real                                     */
+  float* x_e_box;                 /* out padded real (x_e_box_prev after the step)          */
+  float* Tk_box;                  /* in/out real                                            */
+  float* TS_box;                  /* out real                                               */
+  double* stored_fcoll;           /* host, in/out: collapse fraction per snapshot (the entry
+                                     [snapshot] is written, [snapshot - 1] read)            */
+} mxr_ts_grids;
+
+/* Grid-independent per-shell tables and the history plan ComputeTs builds on the host
+ * (ComputeTs.c:339-478, 749-1132).  Filled by mxr_compute_ts when asked.                  */
+typedef struct mxr_ts_tables {
+  int n;                          /* shells                                                */
+  int homogeneous;                /* 1: the ReionMaxHeatingRedshift branch ran             */
+  int NO_LIGHT;
+  double zp, dzp, dt_dzp, growth_factor_zp, dgrowth_factor_dzp, const_zp_prefactor_GAL;
+  double collapse_fraction, x_e_ave, filling_factor_of_HI_zp;
+  double R[MXR_TS_MAX_STEPS];         /* Mpc                                               */
+  double zpp_edge[MXR_TS_MAX_STEPS];
+  double zpp[MXR_TS_MAX_STEPS];
+  double nu_tau_one[MXR_TS_MAX_STEPS];
+  double sum_lyn[MXR_TS_MAX_STEPS];
+  double freq_int_heat[MXR_X_INT_NXHII][MXR_TS_MAX_STEPS];
+  double freq_int_ion[MXR_X_INT_NXHII][MXR_TS_MAX_STEPS];
+  double freq_int_lya[MXR_X_INT_NXHII][MXR_TS_MAX_STEPS];
+  float x_int_XHII[MXR_X_INT_NXHII];
+  int n_ops[MXR_TS_MAX_STEPS];        /* source operations before each shell's FFT         */
+  int op_kind[MXR_TS_MAX_STEPS][MXR_TS_MAX_OPS];
+  int op_hist[MXR_TS_MAX_STEPS][MXR_TS_MAX_OPS];
+  double op_weight[MXR_TS_MAX_STEPS][MXR_TS_MAX_OPS];
+  int hist_max;                   /* largest history index read (-1: none)                 */
+} mxr_ts_tables;
+
+typedef struct mxr_ts_globals {
+  double volume_ave_J_alpha;      /* the four below are not written in the homogeneous     */
+  double volume_ave_xalpha;       /* branch (the reference leaves them unchanged there)    */
+  double volume_ave_Xheat;
+  double volume_ave_Xion;
+  double volume_ave_TS;
+  double volume_ave_TK;
+  double volume_ave_xe;
+  double device_ms;
+} mxr_ts_globals;
+
+/* One spin-temperature step: the homogeneous RECFAST branch above ReionMaxHeatingRedshift,
+ * otherwise TsNumFilterSteps shells of history-weighted SFR and X-ray luminosity grids, each
+ * r2c -> window -> c2r on the GPU and kept in HBM, the host's per-shell frequency tables, and
+ * the per-cell forward-Euler update of x_e and T_K with the spin temperature.  `tables` may be
+ * NULL; otherwise it receives the tables used.  Replaces _ComputeTs
+ * (src/core/ComputeTs.c:85-1554; entry ComputeTs, src/core/ComputeTs.h:13) for
+ * Flag_IncludeAGNXray = 0, no mini-haloes, no Lyman-Werner feedback.                       */
+int mxr_compute_ts(mxr_ctx* ctx, const mxr_ts_params* params, const mxr_ts_grids* grids,
+                   mxr_ts_globals* globals, mxr_ts_tables* tables);
+
+/* The host-side, grid-independent part of one spin-temperature step on its own: the shell
+ * geometry and history plan (ComputeTs.c:339-478) and, given the R_ct = 0 averages the grid
+ * pass produces (collapse_fraction, x_e_ave) and stored_fcoll (entry [snapshot] must already
+ * hold collapse_fraction), the frequency tables, sum_lyn and prefactors (:749-1132).  Needs
+ * no GPU; mxr_compute_ts runs the same code.                                              */
+int mxr_ts_build_tables(const mxr_ts_params* params, double collapse_fraction, double x_e_ave,
+                        const double* stored_fcoll, mxr_ts_tables* tables);
+
+/* Number of history snapshots the shells of snapshots 1..last_snapshot reach back to
+ * (params->ZZ must hold last_snapshot + 1 redshifts).  Replaces set_sfr_history
+ * (src/core/ComputeTs.c:32-83), which sizes NstoreSnapshots_Heating.                       */
+int mxr_ts_history_depth(const mxr_ts_params* params, int last_snapshot);
+
 /* ---- per-kernel timing (hipEvents on the context's stream) ------------------------------ */
 /* One row per kernel class of the path: launches, summed device time and the algorithmic
  * HBM bytes those launches must move (inputs read + outputs written, modes the sharp-k window

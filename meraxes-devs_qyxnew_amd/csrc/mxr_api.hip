@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "mxr_internal.hpp"
+#include "mxr_ts.hpp"
 
 namespace mxr {
 
@@ -146,7 +147,9 @@ int check_hii_params(const mxr_hii_params* p, int N)
 
 static const char* const kclass_names[K_NCLASS] = {
   "fwd_r2c_z", "fwd_fft_y", "fwd_fft_x_scale", "inv_fft_x_filter", "inv_fft_y", "inv_c2r_z_update",
-  "plain_copy", "plain_filter", "plain_clamp", "plain_criterion", "plain_scale_fill", "postloop_sums"};
+  "plain_copy", "plain_filter", "plain_clamp", "plain_criterion", "plain_scale_fill", "postloop_sums",
+  "ts_source", "ts_fwd_fft", "ts_inv_fft_x_filter", "ts_inv_fft_y", "ts_inv_c2r_z_store", "ts_reduce_r0",
+  "ts_evolve"};
 
 const char* kclass_name(int cls) { return (cls >= 0 && cls < K_NCLASS) ? kclass_names[cls] : "?"; }
 
@@ -290,6 +293,15 @@ void mxr_destroy(mxr_ctx* h)
     hipFree(c.out_stage[g]);
   }
   hipFree(c.ion);
+  for (int g = 0; g < 2; g++)
+    hipFree(c.ts_src[g]);
+  hipFree(c.ts_store);
+  hipFree(c.ts_hist);
+  hipFree(c.ts_dev);
+  if (c.ts_dev_host)
+    hipHostFree(c.ts_dev_host);
+  if (c.ev_red)
+    hipEventDestroy(c.ev_red);
   hipFree(c.partials);
   hipFree(c.sums);
   if (c.sums_host)
@@ -425,6 +437,83 @@ int mxr_filter(mxr_ctx* h, void* box, int mem, int local_ix_start, int slab_nx, 
   }
   hipFree(d);
   return rc;
+}
+
+int mxr_compute_ts(mxr_ctx* h, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_globals* globals,
+                   mxr_ts_tables* tables)
+{
+  if (!h || !p || !g || !globals) {
+    set_error("NULL context, params, grids or globals");
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (p->ReionGridDim != h->c.N) {
+    set_error("ReionGridDim %d does not match the context's %d", p->ReionGridDim, h->c.N);
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (p->TsNumFilterSteps < 1 || p->TsNumFilterSteps > MXR_TS_MAX_STEPS || !p->ZZ || p->snapshot < 0 ||
+      !(p->BoxSize > 0) || !(p->Hubble_h > 0) || !(p->OmegaM > 0) || !(p->UnitLength_in_cm > 0) ||
+      !(p->UnitMass_in_g > 0) || !(p->UnitVelocity_in_cm_per_s > 0) || !(p->NuXrayThreshold > 0) ||
+      p->ReionRtoMFilterType < 0 || p->ReionRtoMFilterType > 1) {
+    set_error("bad spin-temperature parameters (TsNumFilterSteps 1..%d, ZZ, snapshot, units, RtoM type)",
+              MXR_TS_MAX_STEPS);
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (p->TsHeatingFilterType < 0 || p->TsHeatingFilterType > 2) {
+    set_error("TsHeatingFilterType type %d is undefined", p->TsHeatingFilterType);
+    return MXR_ERR_FILTER;
+  }
+  if (!g->deltax || !g->stars || !g->x_e_box_prev || !g->x_e_box || !g->Tk_box || !g->TS_box ||
+      !g->stored_fcoll || (g->mem != MXR_MEM_HOST && g->mem != MXR_MEM_DEVICE)) {
+    set_error("missing grid pointer or bad memory kind");
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (hipSetDevice(h->c.device) != hipSuccess) {
+    set_error("hipSetDevice(%d) failed", h->c.device);
+    return MXR_ERR_HIP;
+  }
+  return run_compute_ts(&h->c, p, g, globals, tables);
+}
+
+int mxr_ts_build_tables(const mxr_ts_params* p, double collapse_fraction, double x_e_ave,
+                        const double* stored_fcoll, mxr_ts_tables* tables)
+{
+  if (!p || !p->ZZ || !stored_fcoll || !tables || p->snapshot < 0 || p->TsNumFilterSteps < 1 ||
+      p->TsNumFilterSteps > MXR_TS_MAX_STEPS || !(p->BoxSize > 0) || !(p->Hubble_h > 0) || !(p->OmegaM > 0) ||
+      p->ReionGridDim <= 0 || !(p->NuXrayThreshold > 0)) {
+    set_error("bad parameters for the spin-temperature tables");
+    return MXR_ERR_INVALID_ARG;
+  }
+  TsPlan pl;
+  int rc = ts_plan(p, &pl);
+  if (rc)
+    return rc;
+  if (pl.homogeneous) {
+    std::memset(tables, 0, sizeof(*tables));
+    tables->homogeneous = 1;
+    tables->zp = pl.zp;
+    tables->dzp = pl.dzp;
+    tables->dt_dzp = pl.dt_dzp;
+    tables->hist_max = -1;
+    return MXR_OK;
+  }
+  TsDev* dev = new (std::nothrow) TsDev;
+  if (!dev) {
+    set_error("out of host memory");
+    return MXR_ERR_OOM;
+  }
+  rc = ts_tables(p, pl, collapse_fraction, x_e_ave, stored_fcoll, dev, tables);
+  delete dev;
+  return rc;
+}
+
+int mxr_ts_history_depth(const mxr_ts_params* p, int last_snapshot)
+{
+  if (!p || !p->ZZ || last_snapshot < 0 || p->TsNumFilterSteps < 1 || p->TsNumFilterSteps > MXR_TS_MAX_STEPS ||
+      !(p->BoxSize > 0) || !(p->Hubble_h > 0) || !(p->OmegaM > 0) || p->ReionGridDim <= 0) {
+    set_error("bad parameters for the history depth");
+    return MXR_ERR_INVALID_ARG;
+  }
+  return ts_history_depth(p, last_snapshot);
 }
 
 int mxr_profile_enable(mxr_ctx* h, int on)

@@ -51,6 +51,13 @@ enum KClass {
   K_CRIT,         // plain: criterion / state kernel
   K_SCALE,        // plain: /N^3, xH and r_bubble fills
   K_POST,         // post-loop temperatures and FP64 sums
+  K_TS_SRC,       // spin temperature: history-weighted source grids
+  K_TS_FWD,       // spin temperature: forward r2c + y + x (/N^3) of a changed source
+  K_TS_INV_X,     // spin temperature: inverse x-pass (+ window)
+  K_TS_INV_Y,     // spin temperature: inverse y-pass
+  K_TS_INV_Z,     // spin temperature: inverse z-pass c2r storing the smoothed shell
+  K_TS_RED,       // spin temperature: R_ct = 0 collapse fraction, <x_e>, <L_X>
+  K_TS_EVOLVE,    // spin temperature: per-cell x_e / T_K / T_S update (or homogeneous branch)
   K_NCLASS
 };
 
@@ -89,6 +96,15 @@ struct Ctx {
   size_t partials_cap = 0;
   double* sums = nullptr;          // final FP64 sums (device)
   double* sums_host = nullptr;     // pinned copy
+  // spin temperature
+  float* ts_src[2] = {};           // current sfr / xray_luminosity source grids (padded real)
+  float* ts_store = nullptr;       // smoothed shells: [grid][shell][cell] FP32, real layout
+  size_t ts_store_cap = 0;         // floats
+  float* ts_hist = nullptr;        // staged host histories (sfr then xray), padded grids
+  size_t ts_hist_cap = 0;          // floats
+  void* ts_dev = nullptr;          // device copy of the per-call table block
+  void* ts_dev_host = nullptr;     // pinned host copy
+  hipEvent_t ev_red = nullptr;     // R_ct = 0 reductions ready
 };
 
 inline long n_complex(int N) { return (long)N * N * (N / 2 + 1); }
@@ -106,6 +122,9 @@ bool supported_dim(int N);
 int run_find_hii_bubbles(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_hii_globals* out);
 int run_filter(Ctx* c, float2* box, int local_ix_start, int slab_nx, int grid_dim, float dk, float R, int ftype);
 int run_brightness_temperature(Ctx* c, const mxr_tb_params* p, const mxr_tb_grids* g, double* ave);
+int launch_final_sum(Ctx* c, const double* partials, int nblk, int nv, double* out);  // fixed-order FP64 sum
+int run_compute_ts(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_globals* out,
+                   mxr_ts_tables* tables);
 int ensure_buffers(Ctx* c, int ngrid);
 int prof_begin(Ctx* c);                              // returns the event index (or -1)
 int prof_end(Ctx* c, int ev, int cls, double bytes); // records the launch

@@ -1,0 +1,428 @@
+// mxr_passes.hpp -- the FFT passes shared by the find_HII_bubbles and spin-temperature
+// kernels: grid pointer packs, the reference's k-space window arithmetic, the column pass
+// (x and y), the forward row pass (r2c), one grid's inverse row pass (c2r) into registers,
+// FP64 block sums and the per-dimension launch configuration.
+#pragma once
+
+#include <cmath>
+
+#include "mxr_fft.hpp"
+#include "mxr_internal.hpp"
+
+namespace mxr {
+
+struct GridPtrs {
+  float2* p[4];
+};
+struct CGridPtrs {
+  const float2* p[4];
+};
+struct CRealPtrs {
+  const float* p[4];
+};
+
+// column c of a column pass starts at (c / cpp) * ps + (c % cpp); its elements are es apart
+struct ColGeom {
+  long cpp, ps, es, ncol;
+};
+
+// Window of one radius.  apply = 0 on the last (unfiltered) step.
+struct FiltArgs {
+  int N;
+  float dk;   // (float)(2 pi / (float)BoxSize)
+  float R;    // (float)R
+  int ftype;
+  int apply;
+  int prune;  // skip the modes the sharp-k window zeroes
+};
+
+// COL_SCALE divides by `div` in double (find_HII_bubbles.c:235-253), COL_FSCALE by (float)div
+// in float (ComputeTs.c:485-487)
+enum { COL_PLAIN = 0, COL_SCALE = 1, COL_FILTER = 2, COL_YPRUNE = 3, COL_FSCALE = 4 };
+
+// grid pointer selection without dynamic indexing of a by-value kernel argument
+template <class P>
+__device__ __forceinline__ auto pick(const P& ps, int g)
+{
+  return g == 0 ? ps.p[0] : g == 1 ? ps.p[1] : g == 2 ? ps.p[2] : ps.p[3];
+}
+// ---- the reference's window arithmetic (float, unfused) --------------------------------
+__device__ __forceinline__ float kmag_ref(int nx, int ny, int nz, int N, float dk)
+{
+  const int mid = N / 2;
+  const float kx = __fmul_rn((float)(nx > mid ? nx - N : nx), dk);
+  const float ky = __fmul_rn((float)(ny > mid ? ny - N : ny), dk);
+  const float kz = __fmul_rn((float)nz, dk);
+  return __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz, kz)));
+}
+
+// sharp-k (type 1): kR *= 0.413566994 (double product, float result); zero if kR > 1
+__device__ __forceinline__ bool sharpk_keep(int nx, int ny, int nz, const FiltArgs& fa)
+{
+  const float kR = __fmul_rn(kmag_ref(nx, ny, nz, fa.N, fa.dk), fa.R);
+  const float s = (float)((double)kR * 0.413566994);
+  return !(s > 1.0f);
+}
+
+// multiplicative window of types 0 (real-space top-hat) and 2 (Gaussian); one out-of-line copy
+// serves every kernel, so the fused and plain paths evaluate identical code
+static __device__ __noinline__ float window_mul(int nx, int ny, int nz, const FiltArgs& fa)
+{
+  float kR = __fmul_rn(kmag_ref(nx, ny, nz, fa.N, fa.dk), fa.R);
+  if (fa.ftype == 0) {
+    if ((double)kR > 1e-4) {
+      const float a = __fdiv_rn(sinf(kR), powf(kR, 3.0f));
+      const float b = __fdiv_rn(cosf(kR), powf(kR, 2.0f));
+      return (float)(3.0 * (double)__fsub_rn(a, b));
+    }
+    return 1.0f;
+  }
+  kR = (float)((double)kR * 0.643);
+  return powf(2.71828182845904523536f, (float)((double)__fmul_rn(-kR, kR) / 2.0));
+}
+
+// ---- column pass (x: columns = (j,k) flattened, elements i; y: columns = (i,k), elements j)
+template <int L, int NT, int TC, int S, int MODE>
+__global__ void __launch_bounds__(NT) k_col(CGridPtrs in, GridPtrs out, ColGeom geo, const float2* __restrict__ twg,
+                                            FiltArgs fa, double div)
+{
+  constexpr int T = NT / TC;
+  using SL = Slots<L, T>;
+  __shared__ float2 lds[L * TC];
+  __shared__ float2 tw[L];
+  const int g = blockIdx.y;
+  const int c = threadIdx.x % TC;
+  const int t = threadIdx.x / TC;
+  const long col = (long)blockIdx.x * TC + c;
+  const bool valid = col < geo.ncol;
+  const long base = valid ? (col / geo.cpp) * geo.ps + (col % geo.cpp) : 0;
+  for (int m = threadIdx.x; m < L; m += NT)
+    tw[m] = twg[m];
+
+  bool colnz = valid;
+  int fj = 0, fk = 0;
+  if constexpr (MODE == COL_FILTER) {
+    const int H = fa.N / 2 + 1;
+    fj = (int)(col / H);
+    fk = (int)(col % H);
+    if (valid && fa.apply && fa.prune && fa.ftype == 1)
+      colnz = sharpk_keep(0, fj, fk, fa);
+  } else if constexpr (MODE == COL_YPRUNE) {
+    fk = (int)(col % geo.cpp);
+    if (valid && fa.apply && fa.prune)
+      colnz = sharpk_keep(0, 0, fk, fa);
+  }
+  if constexpr (MODE == COL_FILTER || MODE == COL_YPRUNE) {
+    if (!__syncthreads_or(colnz))
+      return;  // every column of the tile is zero for this radius
+  }
+
+  const float2* __restrict__ src = pick(in, g);
+  float2 v[SL::NREG];
+#pragma unroll
+  for (int q = 0; q < SL::Q0; q++) {
+    const int j = t + T * q;
+#pragma unroll
+    for (int r = 0; r < SL::R0; r++) {
+      const int e = j + r * SL::B0;
+      float2 x = make_float2(0.f, 0.f);
+      if ((SL::B0 % T == 0 || j < SL::B0) && colnz) {
+        if constexpr (MODE == COL_FILTER) {
+          if (!fa.apply) {
+            x = src[base + (long)e * geo.es];
+          } else if (fa.ftype == 1) {
+            if (sharpk_keep(e, fj, fk, fa))
+              x = src[base + (long)e * geo.es];
+          } else {
+            const float w = window_mul(e, fj, fk, fa);
+            x = src[base + (long)e * geo.es];
+            // rounded on its own (no fusion into the first butterfly), as filter() does
+            x = make_float2(__fmul_rn(x.x, w), __fmul_rn(x.y, w));
+          }
+        } else if constexpr (MODE == COL_YPRUNE) {
+          if (!fa.apply || !fa.prune || sharpk_keep(0, e, fk, fa))
+            x = src[base + (long)e * geo.es];
+        } else {
+          x = src[base + (long)e * geo.es];
+        }
+      }
+      v[q * SL::R0 + r] = x;
+    }
+  }
+  tile_fft<L, T, S, L, LdsCol<TC>>(v, lds, tw, c, t);
+  if (!colnz)
+    return;
+  float2* __restrict__ dst = pick(out, g);
+#pragma unroll
+  for (int q = 0; q < SL::QL; q++) {
+    const int j = t + T * q;
+    if (SL::BL % T == 0 || j < SL::BL) {
+#pragma unroll
+      for (int r = 0; r < SL::RL; r++) {
+        const int e = j + r * SL::BL;
+        float2 y = v[q * SL::RL + r];
+        if constexpr (MODE == COL_SCALE)
+          y = make_float2((float)((double)y.x / div), (float)((double)y.y / div));
+        if constexpr (MODE == COL_FSCALE)
+          y = make_float2(__fdiv_rn(y.x, (float)div), __fdiv_rn(y.y, (float)div));
+        dst[base + (long)e * geo.es] = y;
+      }
+    }
+  }
+}
+
+// ---- z-pass forward: padded real rows -> half-spectrum rows ------------------------------
+// x[2n] + i x[2n+1] packed into an N/2-point complex FFT Z; then
+// X[k] = (Z[k] + conj Z[M-k]) / 2 + exp(-2 pi i k / N) (Z[k] - conj Z[M-k]) / (2i), k = 0..M.
+template <int N, int NT, int TR>
+__global__ void __launch_bounds__(NT) k_row_r2c(CRealPtrs in, GridPtrs out, long nrows, const float2* __restrict__ twg)
+{
+  constexpr int M = N / 2, H = N / 2 + 1, T = NT / TR, P = M + M / 8 + 1;
+  using SL = Slots<M, T>;
+  using LI = LdsRowPad<P>;
+  __shared__ float2 lds[TR * P];
+  __shared__ float2 tw[N];
+  const int g = blockIdx.y;
+  const int c = threadIdx.x / T, t = threadIdx.x % T;
+  const long row = (long)blockIdx.x * TR + c;
+  const bool valid = row < nrows;
+  for (int m = threadIdx.x; m < N; m += NT)
+    tw[m] = twg[m];
+  const float* __restrict__ src = pick(in, g) + (valid ? row : 0) * 2 * H;
+  float2 v[SL::NREG];
+#pragma unroll
+  for (int q = 0; q < SL::Q0; q++) {
+    const int j = t + T * q;
+#pragma unroll
+    for (int r = 0; r < SL::R0; r++) {
+      const int n = j + r * SL::B0;
+      float2 x = make_float2(0.f, 0.f);
+      if (valid && (SL::B0 % T == 0 || j < SL::B0))
+        x = *reinterpret_cast<const float2*>(src + 2 * n);
+      v[q * SL::R0 + r] = x;
+    }
+  }
+  __syncthreads();  // twiddles visible (single-stage transforms use them below)
+  tile_fft<M, T, -1, N, LI, SyncWave>(v, lds, tw, c, t);
+#pragma unroll
+  for (int q = 0; q < SL::QL; q++) {
+    const int j = t + T * q;
+    if (SL::BL % T == 0 || j < SL::BL)
+#pragma unroll
+      for (int r = 0; r < SL::RL; r++)
+        lds[LI::at(c, j + r * SL::BL)] = v[q * SL::RL + r];
+  }
+  __syncthreads();
+  if (!valid)
+    return;
+  float2* __restrict__ dst = pick(out, g) + row * H;
+  for (int k = t; k <= M; k += T) {
+    const float2 zk = lds[LI::at(c, k % M)];
+    const float2 zm0 = lds[LI::at(c, (M - k) % M)];
+    const float2 zm = make_float2(zm0.x, -zm0.y);
+    const float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
+    const float2 d = make_float2(0.5f * (zk.x - zm.x), 0.5f * (zk.y - zm.y));
+    const float2 o = make_float2(d.y, -d.x);  // d / i
+    dst[k] = cadd(e, cmul(tw[k], o));
+  }
+}
+
+// wave-wide then block-wide FP64 sum of NV values; thread 0 writes them to out
+template <int NT, int NV>
+__device__ __forceinline__ void block_sum(double* acc, double* out)
+{
+  __shared__ double red[NT / 64][NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    double x = acc[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+      x += __shfl_down(x, o, 64);
+    acc[q] = x;
+  }
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NV; q++)
+      red[w][q] = acc[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      double s = 0.0;
+      for (int ww = 0; ww < NT / 64; ww++)
+        s += red[ww][q];
+      out[q] = s;
+    }
+  }
+}
+
+// One grid's rows through the inverse z-pass: load the half-spectrum rows into LDS (only the
+// kc leading k the window keeps -- a prefix, since |k| grows with k_z), pre-twiddle, M-point inverse FFT.  out receives the
+// last-stage slots: slot (q, r) = y[2n] + i y[2n+1], n = t + T q + r BL.  A row's T threads
+// are consecutive lanes of one wavefront, so the stages synchronise per wave, not per block.
+template <int N, int T, int TR>
+__device__ __forceinline__ void row_c2r_grid(const float2* __restrict__ base, bool valid, long row, float2* lds,
+                                             const float2* tw, int kc, int c, int t, float2* out)
+{
+  constexpr int M = N / 2, H = N / 2 + 1, P = M + M / 8 + 1;
+  static_assert(64 % T == 0, "a row must not straddle two wavefronts");
+  using SL = Slots<M, T>;
+  using LI = LdsRowPad<P>;
+  const float2* __restrict__ src = base + (valid ? row : 0) * H;
+  for (int k = t; k < H; k += T) {
+    float2 x = make_float2(0.f, 0.f);
+    if (valid && k < kc)
+      x = src[k];
+    lds[LI::at(c, k)] = x;
+  }
+  SyncWave::sync();
+  float2 v[SL::NREG];
+#pragma unroll
+  for (int q = 0; q < SL::Q0; q++) {
+    const int j = t + T * q;
+#pragma unroll
+    for (int r = 0; r < SL::R0; r++) {
+      const int n = j + r * SL::B0;
+      float2 z = make_float2(0.f, 0.f);
+      if (SL::B0 % T == 0 || j < SL::B0) {
+        const float2 xn = lds[LI::at(c, n)];
+        const float2 xm0 = lds[LI::at(c, M - n)];
+        const float2 xm = make_float2(xm0.x, -xm0.y);
+        const float2 s = cadd(xn, xm);
+        const float2 d = csub(xn, xm);
+        const float2 w = make_float2(tw[n].x, -tw[n].y);  // exp(+2 pi i n / N)
+        const float2 u = cmul(w, d);
+        z = make_float2(s.x - u.y, s.y + u.x);  // s + i u
+      }
+      v[q * SL::R0 + r] = z;
+    }
+  }
+  SyncWave::sync();
+  tile_fft<M, T, +1, N, LI, SyncWave>(v, lds, tw, c, t);
+#pragma unroll
+  for (int s = 0; s < SL::QL * SL::RL; s++)
+    out[s] = v[s];
+  SyncWave::sync();
+}
+
+// ---- launch configuration per grid dimension ------------------------------------------
+constexpr int pow2_floor(int x)
+{
+  int p = 1;
+  while (p * 2 <= x)
+    p *= 2;
+  return p;
+}
+
+template <int N>
+struct Cfg {
+  static constexpr int M = N / 2;
+  static constexpr int H = N / 2 + 1;
+  static constexpr int TC = 16;
+  static constexpr int COL_T = (N / 8 >= 16) ? 16 : (N / 8 < 1 ? 1 : N / 8);
+  static constexpr int COL_NT = TC * COL_T;
+  static constexpr int ROW_T = pow2_floor((M / 8 >= 16) ? 16 : (M / 8 < 1 ? 1 : M / 8));
+  static constexpr int ROW_NT = 256;
+  static constexpr int TR = ROW_NT / ROW_T;
+};
+
+static inline unsigned ew_blocks(long n)
+{
+  long b = (n + 255) / 256;
+  if (b > 8192)
+    b = 8192;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+#define RC(x)                                                                                                        \
+  do {                                                                                                               \
+    int rc_ = (x);                                                                                                   \
+    if (rc_)                                                                                                         \
+      return rc_;                                                                                                    \
+  } while (0)
+
+// Modes the sharp-k window keeps at radius R (host restatement of sharpk_keep, used only to
+// count algorithmic bytes): k1 = kept k_z on the k_x = k_y = 0 line, c2 = kept (k_y, k_z)
+// columns at k_x = 0, k3 = kept modes overall.
+struct KeptModes {
+  long k1, c2, k3;
+  static KeptModes all(int N)
+  {
+    const long H = N / 2 + 1;
+    return KeptModes{H, (long)N * H, (long)N * N * H};
+  }
+};
+
+static inline bool keep_host(int nx, int ny, int nz, int N, float dk, float R)
+{
+  const int mid = N / 2;
+  volatile float kx = (float)(nx > mid ? nx - N : nx) * dk;
+  volatile float ky = (float)(ny > mid ? ny - N : ny) * dk;
+  volatile float kz = (float)nz * dk;
+  volatile float x2 = kx * kx, y2 = ky * ky, z2 = kz * kz;
+  volatile float s = x2 + y2;
+  s = s + z2;
+  volatile float kR = std::sqrt((float)s) * R;
+  return !((float)((double)kR * 0.413566994) > 1.0f);
+}
+
+static inline KeptModes kept_modes(int N, float dk, float R)
+{
+  const int H = N / 2 + 1;
+  KeptModes km{0, 0, 0};
+  for (int k = 0; k < H; k++) {
+    if (!keep_host(0, 0, k, N, dk, R))
+      break;
+    km.k1++;
+    for (int j = 0; j < N; j++) {
+      if (!keep_host(0, j, k, N, dk, R))
+        continue;
+      km.c2++;
+      // kept |n_x| form a prefix of 0..N/2 (|k| grows with |n_x|)
+      int lo = 0, hi = N / 2;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (keep_host(mid, j, k, N, dk, R))
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      km.k3 += 1 + 2L * lo - ((N % 2 == 0 && lo == N / 2) ? 1 : 0);
+    }
+  }
+  return km;
+}
+
+// grid dimensions the library instantiates
+#define MXR_DIMS(X) X(16) X(32) X(64) X(96) X(128) X(160) X(256) X(512)
+
+// launchers of the shared passes for one grid dimension
+template <int N>
+struct Passes {
+  using C = Cfg<N>;
+
+  static ColGeom xgeo() { return ColGeom{(long)N * C::H, 0, (long)N * C::H, (long)N * C::H}; }
+  static ColGeom ygeo() { return ColGeom{(long)C::H, (long)N * C::H, (long)C::H, (long)N * C::H}; }
+
+  template <int S, int MODE>
+  static int col(Ctx* c, CGridPtrs in, GridPtrs out, int ngrid, ColGeom geo, FiltArgs fa, double div)
+  {
+    dim3 grid((unsigned)((geo.ncol + C::TC - 1) / C::TC), (unsigned)ngrid);
+    hipLaunchKernelGGL((k_col<N, C::COL_NT, C::TC, S, MODE>), grid, dim3(C::COL_NT), 0, c->stream, in, out, geo, c->tw,
+                       fa, div);
+    MXR_HIP(hipGetLastError());
+    return MXR_OK;
+  }
+
+  static int r2c(Ctx* c, CRealPtrs in, GridPtrs out, int ngrid)
+  {
+    const long nrows = (long)N * N;
+    dim3 grid((unsigned)((nrows + C::TR - 1) / C::TR), (unsigned)ngrid);
+    hipLaunchKernelGGL((k_row_r2c<N, C::ROW_NT, C::TR>), grid, dim3(C::ROW_NT), 0, c->stream, in, out, nrows, c->tw);
+    MXR_HIP(hipGetLastError());
+    return MXR_OK;
+  }
+};
+
+}  // namespace mxr

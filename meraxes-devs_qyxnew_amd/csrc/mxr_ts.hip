@@ -1,0 +1,663 @@
+// mxr_ts.hip -- the spin-temperature step (_ComputeTs, src/core/ComputeTs.c:85-1554) on one
+// gfx950 GPU.
+//
+// Per call, inhomogeneous branch (z < ReionMaxHeatingRedshift):
+//   for each of the F shells R_ct (radius R_ct in Mpc, ComputeTs.c:339-746):
+//     source   : when the history plan changes the SFR / X-ray source grids before this shell,
+//                one element-wise launch applies the plan's float operations to both grids
+//                (ComputeTs.c:366-478, load_reion_sfr_grids reionization.c:2750-2797);
+//     forward  : only then r2c + y + x (/ (float)N^3); an unchanged source keeps its spectrum,
+//                which the reference recomputes bit-identically every shell (:480-507);
+//     inverse  : x-pass with the window on load (TsHeatingFilterType, R_ct > 0), y-pass, and a
+//                z-pass c2r whose epilogue clamps the SFR at 0 and stores both smoothed fields
+//                as FP32 [shell][cell] in HBM (:528-743) -- the reference's FP64 SMOOTHED_*
+//                arrays hold exactly (double)float of these values times unit constants, so
+//                FP32 storage loses nothing and halves the traffic;
+//     R_ct = 0 : FP64 sums of the capped collapse fraction, x_e and the X-ray emissivity
+//                (:585-683), read back while the remaining shells run, so the host builds the
+//                frequency tables (mxr_ts_host.hip) concurrently with the GPU's FFTs;
+//   evolve     : one pass per cell: table interpolation at x_e, the shell sums of evolveInt,
+//                the forward-Euler x_e / T_K update and the T_S fixed point, with FP64 sums of
+//                the volume averages (:1171-1505; XRayHeatingFunctions.c:1308-1561, 1639-2008).
+// Homogeneous branch (:252-331): x_e = xion_RECFAST, T_K = T_RECFAST (1 + cT delta), T_S.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mxr_passes.hpp"
+#include "mxr_ts.hpp"
+
+namespace mxr {
+
+// ---- source grids ---------------------------------------------------------------------------
+struct TsOpsArgs {
+  int nops;
+  int init;  // start from history[0] (the current snapshot's grid) instead of the source buffer
+  TsOp ops[TS_MAX_OPS];
+};
+
+__global__ void k_ts_source(TsOpsArgs oa, const float* __restrict__ hs, const float* __restrict__ hx, long hstride,
+                            float* __restrict__ ss, float* __restrict__ sx, long n)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    float v[2] = {oa.init ? hs[i] : ss[i], oa.init ? hx[i] : sx[i]};
+    for (int k = 0; k < oa.nops; k++) {
+      const TsOp o = oa.ops[k];
+#pragma unroll
+      for (int g = 0; g < 2; g++) {
+        const float* __restrict__ h = g == 0 ? hs : hx;
+        if (o.kind == TS_OP_CLEAR)
+          v[g] = 0.0f;
+        else if (o.kind == TS_OP_SCALE)
+          v[g] = (float)((double)v[g] * o.wd);
+        else if (o.kind == TS_OP_LOAD)
+          v[g] = __fmul_rn(h[(long)o.hist * hstride + i], o.wf);
+        else
+          v[g] = __fadd_rn(v[g], __fmul_rn(h[(long)o.hist * hstride + i], o.wf));
+      }
+    }
+    ss[i] = v[0];
+    sx[i] = v[1];
+  }
+}
+
+// ---- inverse z-pass storing one smoothed shell ------------------------------------------------
+struct TsStore {
+  float* out[2];  // sfr (clamped at 0), xray (as is); real layout
+};
+
+template <int N, int NT, int TR>
+__global__ void __launch_bounds__(NT) k_row_c2r_ts(CGridPtrs in, long nrows, const float2* __restrict__ twg,
+                                                   FiltArgs fa, TsStore st)
+{
+  constexpr int M = N / 2, H = N / 2 + 1, T = NT / TR, P = M + M / 8 + 1;
+  using SL = Slots<M, T>;
+  constexpr int NS = SL::QL * SL::RL;
+  __shared__ float2 lds[TR * P];
+  __shared__ float2 tw[N];
+  const int c = threadIdx.x / T, t = threadIdx.x % T;
+  const long row = (long)blockIdx.x * TR + c;
+  const bool valid = row < nrows;
+  for (int m = threadIdx.x; m < N; m += NT)
+    tw[m] = twg[m];
+  int kc = H;
+  if (fa.apply && fa.prune) {
+    kc = 0;
+    for (int k0 = 0; k0 < H; k0 += NT) {
+      const int k = k0 + (int)threadIdx.x;
+      kc += __syncthreads_count(k < H && sharpk_keep(0, 0, k, fa));
+    }
+  }
+  __syncthreads();
+  const int g = blockIdx.y;
+  float2 res[NS];
+  row_c2r_grid<N, T, TR>(pick(in, g), valid, row, lds, tw, kc, c, t, res);
+  if (!valid)
+    return;
+  float* __restrict__ dst = (g == 0 ? st.out[0] : st.out[1]) + row * N;
+#pragma unroll
+  for (int q = 0; q < SL::QL; q++) {
+    const int j = t + T * q;
+    if (SL::BL % T == 0 || j < SL::BL)
+#pragma unroll
+      for (int r = 0; r < SL::RL; r++) {
+        const int n = j + r * SL::BL;
+        float2 y = res[q * SL::RL + r];
+        if (g == 0)
+          y = make_float2(fmaxf(y.x, 0.0f), fmaxf(y.y, 0.0f));  // ComputeTs.c:592, 703
+        *reinterpret_cast<float2*>(dst + 2 * n) = y;
+      }
+  }
+}
+
+// ---- R_ct = 0 reductions (ComputeTs.c:585-683) ------------------------------------------------
+struct TsRed0 {
+  int N;
+  const float* stars;   // padded
+  const float* deltax;  // padded
+  const float* xe;      // padded (x_e_box_prev before the update)
+  const float* xray0;   // real: smoothed X-ray field of shell 0
+  double K1, R03, pv, ULm3;
+  double* partials;
+};
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_ts_red0(TsRed0 a)
+{
+#pragma clang fp contract(off)
+  const int N = a.N, H = N / 2 + 1;
+  const long n = (long)N * N * N;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (long ir = (long)blockIdx.x * NT + threadIdx.x; ir < n; ir += (long)gridDim.x * NT) {
+    const long ip = (ir / N) * 2 * H + ir % N;
+    const double dom = 1.0 + (double)a.deltax[ip];
+    double cf = (double)a.stars[ip] / (a.K1 * dom);
+    cf = cf * (4.0 / 3.0);
+    cf = cf * M_PI;
+    cf = cf * a.R03;
+    cf = cf / a.pv;
+    if (cf > 1.0)
+      cf = 1.0;
+    acc[0] += cf;
+    acc[1] += (double)a.xe[ip];
+    acc[2] += (((double)a.xray0[ir] / a.pv) * XRAY_LUMINOSITY_UNIT) * a.ULm3;
+  }
+  block_sum<NT, 3>(acc, a.partials + (long)blockIdx.x * 3);
+}
+
+// ---- microphysics (XRayHeatingFunctions.c), FP64, in the reference's evaluation order ---------
+// natural cubic spline s of the device block (GSL's cspline form), knots kx[s][0..n-1]
+__device__ __forceinline__ double spline_eval(const TsDev* __restrict__ T, int s, double xq)
+{
+#pragma clang fp contract(off)
+  const int n = T->kn[s];
+  int ilo = 0, ihi = n - 1;
+  while (ihi > ilo + 1) {
+    const int i = (ihi + ilo) / 2;
+    if (T->kx[s][i] > xq)
+      ihi = i;
+    else
+      ilo = i;
+  }
+  const double dx = T->kx[s][ilo + 1] - T->kx[s][ilo];
+  const double dy = T->ky[s][ilo + 1] - T->ky[s][ilo];
+  const double delx = xq - T->kx[s][ilo];
+  const double ci = T->kc[s][ilo], ci1 = T->kc[s][ilo + 1];
+  const double b = (dy / dx) - dx * (ci1 + 2.0 * ci) / 3.0;
+  const double d = (ci1 - ci) / (3.0 * dx);
+  return T->ky[s][ilo] + delx * (b + delx * (ci + delx * d));
+}
+
+// kappa_10 (:1728-1822): log-log spline, 1 K floor, T^0.381 above 10^4 K
+__device__ __noinline__ double kappa_10(const TsDev* __restrict__ T, double TK)
+{
+#pragma clang fp contract(off)
+  double ans;
+  const double lT = log(TK);
+  if (lT < T->kx[0][0])
+    ans = T->ky[0][0];
+  else if (lT > T->kx[0][KAP_N - 1])
+    ans = log(T->k10_E1 * pow(TK / T->k10_E2, 0.381));
+  else
+    ans = spline_eval(T, 0, lT);
+  return exp(ans);
+}
+
+// kappa_10_elec / kappa_10_pH (:1827-1966): log-log spline, floor below, linear extrapolation above
+__device__ __noinline__ double kappa_tab(const TsDev* __restrict__ T, int s, double Tk)
+{
+#pragma clang fp contract(off)
+  const int n = T->kn[s];
+  const double lT = log(Tk);
+  double ans;
+  if (lT < T->kx[s][0])
+    ans = T->ky[s][0];
+  else if (lT > T->kx[s][n - 1])
+    ans = T->ky[s][n - 1] +
+          ((T->ky[s][n - 1] - T->ky[s][n - 2]) / (T->kx[s][n - 1] - T->kx[s][n - 2]) * (lT - T->kx[s][n - 1]));
+  else
+    ans = spline_eval(T, s, lT);
+  return exp(ans);
+}
+
+// get_Ts (:1639-1684) with xcoll (:1686-1726), xalpha_tilde, taugp, Salpha_tilde, Tc_eff
+// (:1973-2008); z is the (float) redshift of the call, so Trad and (1+z)^3 come from the block
+__device__ __noinline__ float get_Ts(const TsDev* __restrict__ T, float delta, float TKf, float xef, float Jalpha,
+                                     float* curr_xalpha)
+{
+#pragma clang fp contract(off)
+  const double TK = TKf, xe = xef, dl = delta;
+  const double Trad = T->Trad;
+  const double opz3 = T->opz3;
+  const double nH = (((1.0 - xe) * T->No) * opz3) * (1.0 + dl);
+  const double x_HI = (((T21 / Trad) * nH) * kappa_10(T, TK)) / A10_HYPERFINE;
+  const double ne = ((xe * T->N_b0) * opz3) * (1.0 + dl);
+  const double x_e = (((T21 / Trad) * ne) * kappa_tab(T, 1, TK)) / A10_HYPERFINE;
+  const double np = ((xe * T->No) * opz3) * (1.0 + dl);
+  const double x_p = (((T21 / Trad) * np) * kappa_tab(T, 2, TK)) / A10_HYPERFINE;
+  const double xc = (x_HI + x_e) + x_p;
+  double TS, TSinv;
+  if (Jalpha > 1.0e-20) {
+    const double Ja = Jalpha;
+    const double tgp = ((((1.342881e-7 / T->hub_z) * T->No) * opz3) * (1.0 + dl)) * (1.0 - xe);
+    const double xi = pow(((1.0e-7 * tgp) / TK) / TK, 1.0 / 3.0);
+    const double den = ((1.0 + 2.98394 * xi) + (1.53583 * xi) * xi) + ((3.85289 * xi) * xi) * xi;
+    const double opz = 1.0 + (double)T->zpf;
+    double xa = 0.0, TSold = 0.0;
+    TS = Trad;
+    // the reference iterates until converged; the cap only ends a cell that never converges
+    for (int it = 0; it < TS_MAX_ITER && fabs(TS - TSold) / TS > 1.0e-3; it++) {
+      TSold = TS;
+      double S = ((1.0 - 0.0631789 / TK) + (0.115995 / TK) / TK) - (0.401403 / TS) / TK;
+      S += ((0.336463 / TS) / TK) / TK;
+      S /= den;
+      xa = ((1.66e11 / opz) * S) * Ja;
+      double Tce = 1.0 / TK + (0.405535 / TK) * (1.0 / TS - 1.0 / TK);
+      Tce = 1.0 / Tce;
+      TSinv = ((1.0 / Trad + xa / Tce) + xc / TK) / ((1.0 + xa) + xc);
+      TS = 1.0 / TSinv;
+    }
+    *curr_xalpha = (float)xa;
+  } else {
+    TSinv = (1.0 / Trad + xc / TK) / (1.0 + xc);
+    TS = 1.0 / TSinv;
+    *curr_xalpha = 0.0f;
+  }
+  if (TS < 0.)
+    TS = fabs(TS);
+  return (float)TS;
+}
+
+// alpha_A (:1576-1584): case-A recombination coefficient, Abel et al. 1997
+__device__ __forceinline__ double alpha_A(double Tk)
+{
+#pragma clang fp contract(off)
+  const double L = log(Tk / (double)1.1604505e4);
+  const double e = -28.6130338 - 0.72411256 * L - 2.02604473e-2 * pow(L, 2) - 2.38086188e-3 * pow(L, 3) -
+                   3.21260521e-4 * pow(L, 4) - 1.42150291e-5 * pow(L, 5) + 4.98910892e-6 * pow(L, 6) +
+                   5.75561414e-7 * pow(L, 7) - 1.85676704e-8 * pow(L, 8) - 3.07113524e-9 * pow(L, 9);
+  return exp(e);
+}
+
+// ---- per-cell evolve (ComputeTs.c:1171-1407, evolveInt :1308-1561) ------------------------------
+struct TsEvolve {
+  int N;
+  const float* deltax;  // padded
+  float* xe;            // padded, in/out (x_e_box_prev)
+  float* Tk;            // real, in/out
+  float* TS;            // real, out
+  const float* sm;      // [2][F][N^3]: smoothed SFR then X-ray shells
+  double* partials;
+};
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_ts_evolve(const TsDev* __restrict__ T, TsEvolve a)
+{
+#pragma clang fp contract(off)
+  const int N = a.N, H = N / 2 + 1, F = T->F;
+  const long n = (long)N * N * N;
+  const float* __restrict__ sm_sfr = a.sm;
+  const float* __restrict__ sm_xr = a.sm + (long)F * n;
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (long ir = (long)blockIdx.x * NT + threadIdx.x; ir < n; ir += (long)gridDim.x * NT) {
+    const long ip = (ir / N) * 2 * H + ir % N;
+    const float dx = a.deltax[ip];
+    const float xe0 = a.xe[ip];
+    float tk = a.Tk[ir];
+    // table interpolation at the cell's x_e (:1190-1287)
+    double xcall = xe0;
+    const double xtop = (double)T->xint[XNH - 1] * 0.999;
+    if (xcall > xtop)
+      xcall = xtop;
+    else if (xcall < (double)T->xint[0])
+      xcall = 1.001 * (double)T->xint[0];
+    const float xcf = (float)xcall;
+    int m = XNH - 1;
+    while (xcf < T->xint[m])
+      m--;
+    const double dxl = xcall - (double)T->xint[m];
+    double dheat = 0, dion = 0, dlya = 0, dstar = 0;
+    if (!T->no_light) {
+      for (int r = 0; r < F; r++) {
+        const float sf = sm_sfr[(long)r * n + ir];
+        const float xr = sm_xr[(long)r * n + ir];
+        const double SFR = ((((double)sf / T->pv) * T->UMUT) * T->ULm3) / SOLAR_MASS;
+        const double XR = (((double)xr / T->pv) * XRAY_LUMINOSITY_UNIT) * T->ULm3;
+        const double fh = T->slope[0][m][r] * dxl + T->base[0][m][r];
+        const double fi = T->slope[1][m][r] * dxl + T->base[1][m][r];
+        const double fl = T->slope[2][m][r] * dxl + T->base[2][m][r];
+        const double tt = T->dtd[r] * (XR * T->pw[r]);
+        dheat += tt * fh;
+        dion += tt * fi;
+        dlya += tt * fl;
+        dstar += ((((SFR * T->P2) * T->opz[r]) * T->lyn[r]) * T->dtz[r]) * T->dzz[r];
+      }
+    }
+    const double x_e = xe0, Td = tk;
+    const double zpd = (double)T->zpf;
+    const double n_b = (T->N_b0 * T->nb3) * (double)__fadd_rn(1.0f, dx);
+    dheat *= T->prefac_gal;
+    dion *= T->prefac_gal;
+    dlya *= T->prefac_gal * n_b;
+    dstar *= T->conv;
+    const double sink = ((((alpha_A(Td) * CLUMPING_FACTOR) * x_e) * x_e) * T->f_H) * n_b;
+    const double dxe_dzp = T->dt_dzp * (((dion + 0.0) + 0.0) - sink);
+    double dadia = 3 / (1.0 + zpd);
+    if (fabs((double)dx) > FRACT_FLOAT_ERR)
+      dadia += T->dgrowth / (1.0 / (double)dx + T->growth);
+    dadia *= (2.0 / 3.0) * Td;
+    const double dspec = ((-dxe_dzp) * Td) / (1 + x_e);
+    double dcomp = ((((-1.51e-4) * (x_e / ((1.0 + x_e) + T->f_He))) / T->hub_ratio) / T->h) * T->Trad4;
+    dcomp = dcomp / (1.0 + zpd);
+    dcomp *= T->Trad - Td;
+    const double dxheat_dzp = ((((dheat * T->dt_dzp) * 2.0) / 3.0) / BOLTZMANN) / (1.0 + x_e);
+    const double d1 = ((dxheat_dzp + dcomp) + dspec) + dadia;
+    const double d2 = ((dlya + 0.0) + 0.0) + dstar;
+    const double d4 = T->dt_dzp * ((dion + 0.0) + 0.0);
+    // Euler step and floors (:1345-1377)
+    float xe1 = (float)((double)xe0 + dxe_dzp * T->dzp);
+    if (xe1 > 1)
+      xe1 = (float)(1 - FRACT_FLOAT_ERR);
+    else if (xe1 < 0)
+      xe1 = 0;
+    if (tk < MAX_TK)
+      tk = (float)((double)tk + d1 * T->dzp);
+    if (tk < MIN_TK)
+      tk = (float)(TCMB * (1 + T->zp));
+    float xa;
+    const float ts = get_Ts(T, dx, tk, xe1, (float)d2, &xa);
+    a.xe[ip] = xe1;
+    a.Tk[ir] = tk;
+    a.TS[ir] = ts;
+    acc[0] += d2;
+    acc[1] += (double)xa;
+    acc[2] += dxheat_dzp;
+    acc[3] += d4;
+    acc[4] += (double)ts;
+    acc[5] += (double)tk;
+    acc[6] += (double)xe1;
+  }
+  block_sum<NT, 7>(acc, a.partials + (long)blockIdx.x * 7);
+}
+
+// homogeneous branch (ComputeTs.c:252-287) with the final averages
+template <int NT>
+__global__ void __launch_bounds__(NT) k_ts_homog(const TsDev* __restrict__ T, TsEvolve a)
+{
+#pragma clang fp contract(off)
+  const int N = a.N, H = N / 2 + 1;
+  const long n = (long)N * N * N;
+  double acc[3] = {0, 0, 0};
+  for (long ir = (long)blockIdx.x * NT + threadIdx.x; ir < n; ir += (long)gridDim.x * NT) {
+    const long ip = (ir / N) * 2 * H + ir % N;
+    const float dx = a.deltax[ip];
+    const float xe = T->xe_h;
+    const float tk = (float)((double)T->TK_h * (1.0 + (double)__fmul_rn(T->cT_ad, dx)));
+    float xa;
+    const float ts = get_Ts(T, dx, tk, xe, 0.0f, &xa);
+    a.xe[ip] = xe;
+    a.Tk[ir] = tk;
+    a.TS[ir] = ts;
+    acc[0] += (double)ts;
+    acc[1] += (double)tk;
+    acc[2] += (double)xe;
+  }
+  block_sum<NT, 3>(acc, a.partials + (long)blockIdx.x * 3);
+}
+
+// ---- driver -----------------------------------------------------------------------------------
+static int ts_ensure(Ctx* c, int F)
+{
+  const long npad = 2 * n_complex(c->N), nreal = n_real(c->N);
+  RC(ensure_buffers(c, F > 0 ? 2 : 0));
+  for (int g = 0; g < 2 && F > 0; g++)
+    if (!c->ts_src[g])
+      MXR_HIP(hipMalloc(&c->ts_src[g], sizeof(float) * npad));
+  const size_t need = (size_t)2 * F * nreal;
+  if (F > 0 && c->ts_store_cap < need) {
+    (void)hipFree(c->ts_store);
+    c->ts_store = nullptr;
+    c->ts_store_cap = 0;
+    MXR_HIP(hipMalloc(&c->ts_store, sizeof(float) * need));
+    c->ts_store_cap = need;
+  }
+  if (!c->ts_dev)
+    MXR_HIP(hipMalloc(&c->ts_dev, sizeof(TsDev)));
+  if (!c->ts_dev_host)
+    MXR_HIP(hipHostMalloc(&c->ts_dev_host, sizeof(TsDev), hipHostMallocDefault));
+  if (!c->ev_red)
+    MXR_HIP(hipEventCreateWithFlags(&c->ev_red, hipEventDisableTiming));
+  return MXR_OK;
+}
+
+template <int N>
+static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_globals* out, mxr_ts_tables* tables)
+{
+  using K = Passes<N>;
+  constexpr int NT = 256;
+  const long Nc = n_complex(N), nreal = n_real(N), npad = 2 * Nc;
+  const double total_n_cells = std::pow((double)N, 3);
+
+  TsPlan pl;
+  RC(ts_plan(p, &pl));
+  const int F = p->TsNumFilterSteps;
+  if (!pl.homogeneous) {
+    if (!g->sfr_histories || !g->xray_luminosity_histories || g->n_histories <= pl.hist_max ||
+        g->n_histories < 1) {
+      set_error("ComputeTs: the history plan reads history %d but %d histories were given", pl.hist_max,
+                g->n_histories);
+      return MXR_ERR_INVALID_ARG;
+    }
+  }
+  RC(ts_ensure(c, pl.homogeneous ? 0 : F));
+  TsDev* hd = reinterpret_cast<TsDev*>(c->ts_dev_host);
+  const TsDev* dd = reinterpret_cast<const TsDev*>(c->ts_dev);
+
+  auto T = [&](int cls, double bytes, auto&& fn) -> int {
+    const int ev = prof_begin(c);
+    if (ev < -1)
+      return ev;
+    const int rc = fn();
+    if (rc)
+      return rc;
+    return prof_end(c, ev, cls, bytes);
+  };
+
+  // ---- inputs on the device
+  const float* dx = g->deltax;
+  const float* stars = g->stars;
+  float* xe = g->x_e_box_prev;
+  float* Tk = g->Tk_box;
+  float* TS = g->TS_box;
+  float* xe_out = g->x_e_box;
+  const float* hs = g->sfr_histories;
+  const float* hx = g->xray_luminosity_histories;
+  const int nstage = pl.homogeneous ? 0 : pl.hist_max + 1 > 1 ? pl.hist_max + 1 : 1;
+  if (g->mem == MXR_MEM_HOST) {
+    auto stage = [&](int slot, const float* host, long nfl, bool in) -> int {
+      if (!c->in_stage[slot])
+        MXR_HIP(hipMalloc(&c->in_stage[slot], sizeof(float) * npad));
+      if (in)
+        MXR_HIP(hipMemcpyAsync(c->in_stage[slot], host, sizeof(float) * nfl, hipMemcpyHostToDevice, c->stream));
+      return MXR_OK;
+    };
+    RC(stage(0, dx, npad, true));
+    dx = c->in_stage[0];
+    if (!pl.homogeneous) {
+      RC(stage(1, stars, npad, true));
+      stars = c->in_stage[1];
+    }
+    RC(stage(2, xe, npad, true));
+    xe = c->in_stage[2];
+    RC(stage(3, Tk, nreal, !pl.homogeneous));
+    Tk = c->in_stage[3];
+    RC(stage(4, nullptr, 0, false));
+    TS = c->in_stage[4];
+    if (!c->out_stage[0])
+      MXR_HIP(hipMalloc(&c->out_stage[0], sizeof(float) * npad));
+    xe_out = c->out_stage[0];
+    if (!pl.homogeneous) {
+      const size_t need = (size_t)2 * nstage * npad;
+      if (c->ts_hist_cap < need) {
+        (void)hipFree(c->ts_hist);
+        c->ts_hist = nullptr;
+        c->ts_hist_cap = 0;
+        MXR_HIP(hipMalloc(&c->ts_hist, sizeof(float) * need));
+        c->ts_hist_cap = need;
+      }
+      MXR_HIP(hipMemcpyAsync(c->ts_hist, hs, sizeof(float) * nstage * npad, hipMemcpyHostToDevice, c->stream));
+      MXR_HIP(hipMemcpyAsync(c->ts_hist + (size_t)nstage * npad, hx, sizeof(float) * nstage * npad,
+                             hipMemcpyHostToDevice, c->stream));
+      hs = c->ts_hist;
+      hx = c->ts_hist + (size_t)nstage * npad;
+    }
+  }
+
+  const int nblk = (int)ew_blocks(nreal);
+  double* sums = c->sums_host;
+  MXR_HIP(hipEventRecord(c->ev0, c->stream));
+  if (pl.homogeneous) {
+    RC(ts_homogeneous(p, hd));
+    MXR_HIP(hipMemcpyAsync(c->ts_dev, hd, sizeof(TsDev), hipMemcpyHostToDevice, c->stream));
+    TsEvolve ea{N, dx, xe, Tk, TS, nullptr, c->partials};
+    RC(T(K_TS_EVOLVE, (double)nreal * 16, [&] {
+      hipLaunchKernelGGL((k_ts_homog<NT>), dim3(nblk), dim3(NT), 0, c->stream, dd, ea);
+      MXR_HIP(hipGetLastError());
+      return launch_final_sum(c, c->partials, nblk, 3, c->sums);
+    }));
+    MXR_HIP(hipMemcpyAsync(sums, c->sums, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+    if (tables) {
+      std::memset(tables, 0, sizeof(*tables));
+      tables->homogeneous = 1;
+      tables->zp = pl.zp;
+      tables->dzp = pl.dzp;
+      tables->dt_dzp = pl.dt_dzp;
+      tables->hist_max = -1;
+    }
+  } else {
+    FiltArgs fa{};
+    fa.N = N;
+    {
+      const float bs = (float)p->BoxSize;  // filter() uses run_globals.params.BoxSize (Mpc/h)
+      fa.dk = (float)(2.0 * M_PI / bs);
+    }
+    fa.ftype = p->TsHeatingFilterType;
+    fa.prune = p->TsHeatingFilterType == 1 ? 1 : 0;
+    CRealPtrs rin{};
+    GridPtrs spec{}, work{};
+    CGridPtrs cspec{}, cwork{};
+    for (int i = 0; i < 2; i++) {
+      spec.p[i] = c->spec[i];
+      work.p[i] = c->work[i];
+      cspec.p[i] = c->spec[i];
+      cwork.p[i] = c->work[i];
+    }
+    FiltArgs none{};
+    none.N = N;
+    const double cplx = 8.0 * (double)Nc;
+    bool src_valid = false;
+    for (int r = 0; r < F; r++) {
+      if (r == 0 || pl.nops[r] > 0) {
+        if (pl.nops[r] > 0) {
+          TsOpsArgs oa{};
+          oa.nops = pl.nops[r];
+          oa.init = src_valid ? 0 : 1;
+          int nread = oa.init ? 1 : 1;
+          for (int k = 0; k < pl.nops[r]; k++) {
+            oa.ops[k] = pl.ops[r][k];
+            nread += (pl.ops[r][k].kind >= TS_OP_LOAD) ? 1 : 0;
+          }
+          RC(T(K_TS_SRC, 2.0 * 4.0 * npad * (nread + 1), [&] {
+            hipLaunchKernelGGL(k_ts_source, dim3(ew_blocks(npad)), dim3(256), 0, c->stream, oa, hs, hx, npad,
+                               c->ts_src[0], c->ts_src[1], npad);
+            MXR_HIP(hipGetLastError());
+            return MXR_OK;
+          }));
+          src_valid = true;
+        }
+        rin.p[0] = src_valid ? c->ts_src[0] : hs;
+        rin.p[1] = src_valid ? c->ts_src[1] : hx;
+        RC(T(K_TS_FWD, 2.0 * 3 * 2 * cplx, [&] {
+          RC(K::r2c(c, rin, spec, 2));
+          RC((K::template col<-1, COL_PLAIN>(c, cspec, spec, 2, K::ygeo(), none, 1.0)));
+          return K::template col<-1, COL_FSCALE>(c, cspec, spec, 2, K::xgeo(), none, total_n_cells);
+        }));
+      }
+      fa.R = (float)pl.R[r];
+      fa.apply = r > 0 ? 1 : 0;
+      const KeptModes km = (fa.apply && fa.prune) ? kept_modes(N, fa.dk, fa.R) : KeptModes::all(N);
+      TsStore st{{c->ts_store + (size_t)r * nreal, c->ts_store + ((size_t)F + r) * nreal}};
+      RC(T(K_TS_INV_X, 2 * 8.0 * ((double)km.k3 + (double)N * km.c2),
+           [&] { return K::template col<+1, COL_FILTER>(c, cspec, work, 2, K::xgeo(), fa, 1.0); }));
+      RC(T(K_TS_INV_Y, 2 * 8.0 * ((double)N * km.c2 + (double)N * N * km.k1),
+           [&] { return K::template col<+1, COL_YPRUNE>(c, cwork, work, 2, K::ygeo(), fa, 1.0); }));
+      RC(T(K_TS_INV_Z, 2 * (8.0 * (double)N * N * km.k1 + 4.0 * nreal), [&] {
+        const long nrows = (long)N * N;
+        dim3 grid((unsigned)((nrows + Cfg<N>::TR - 1) / Cfg<N>::TR), 2u);
+        hipLaunchKernelGGL((k_row_c2r_ts<N, Cfg<N>::ROW_NT, Cfg<N>::TR>), grid, dim3(Cfg<N>::ROW_NT), 0, c->stream,
+                           cwork, nrows, c->tw, fa, st);
+        MXR_HIP(hipGetLastError());
+        return MXR_OK;
+      }));
+      if (r == 0) {
+        TsRed0 ra{};
+        ra.N = N;
+        ra.stars = stars;
+        ra.deltax = dx;
+        ra.xe = xe;
+        ra.xray0 = c->ts_store + (size_t)F * nreal;
+        const double h = p->Hubble_h;
+        const double box_size = p->BoxSize / h;
+        ra.K1 = (ts_rtom(p, pl.R[0]) * h) * h;
+        ra.R03 = std::pow(pl.R[0], 3.0);
+        ra.pv = std::pow(box_size / (double)N, 3);
+        ra.ULm3 = std::pow(p->UnitLength_in_cm, -3.);
+        ra.partials = c->partials;
+        RC(T(K_TS_RED, (double)nreal * 16, [&] {
+          hipLaunchKernelGGL((k_ts_red0<NT>), dim3(nblk), dim3(NT), 0, c->stream, ra);
+          MXR_HIP(hipGetLastError());
+          return launch_final_sum(c, c->partials, nblk, 3, c->sums);
+        }));
+        MXR_HIP(hipMemcpyAsync(sums, c->sums, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+        MXR_HIP(hipEventRecord(c->ev_red, c->stream));
+      }
+    }
+    // the host tables overlap the remaining shells on the GPU
+    MXR_HIP(hipEventSynchronize(c->ev_red));
+    const double collapse_fraction = sums[0] / total_n_cells;
+    const double x_e_ave = sums[1] / total_n_cells;
+    g->stored_fcoll[p->snapshot] = collapse_fraction;
+    RC(ts_tables(p, pl, collapse_fraction, x_e_ave, g->stored_fcoll, hd, tables));
+    MXR_HIP(hipMemcpyAsync(c->ts_dev, hd, sizeof(TsDev), hipMemcpyHostToDevice, c->stream));
+    TsEvolve ea{N, dx, xe, Tk, TS, c->ts_store, c->partials};
+    RC(T(K_TS_EVOLVE, (double)nreal * (4 + 8 + 8 + 4 + (hd->no_light ? 0 : 8.0 * F)), [&] {
+      hipLaunchKernelGGL((k_ts_evolve<NT>), dim3(nblk), dim3(NT), 0, c->stream, dd, ea);
+      MXR_HIP(hipGetLastError());
+      return launch_final_sum(c, c->partials, nblk, 7, c->sums);
+    }));
+    MXR_HIP(hipMemcpyAsync(sums, c->sums, sizeof(double) * 7, hipMemcpyDeviceToHost, c->stream));
+  }
+  // x_e_box = x_e_box_prev (ComputeTs.c:1459), the whole padded array
+  MXR_HIP(hipMemcpyAsync(xe_out, xe, sizeof(float) * npad, hipMemcpyDeviceToDevice, c->stream));
+  MXR_HIP(hipEventRecord(c->ev1, c->stream));
+  if (g->mem == MXR_MEM_HOST) {
+    MXR_HIP(hipMemcpyAsync(g->x_e_box_prev, xe, sizeof(float) * npad, hipMemcpyDeviceToHost, c->stream));
+    MXR_HIP(hipMemcpyAsync(g->x_e_box, xe_out, sizeof(float) * npad, hipMemcpyDeviceToHost, c->stream));
+    MXR_HIP(hipMemcpyAsync(g->Tk_box, Tk, sizeof(float) * nreal, hipMemcpyDeviceToHost, c->stream));
+    MXR_HIP(hipMemcpyAsync(g->TS_box, TS, sizeof(float) * nreal, hipMemcpyDeviceToHost, c->stream));
+  }
+  MXR_HIP(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  MXR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  RC(prof_resolve(c));
+  if (pl.homogeneous) {
+    out->volume_ave_TS = sums[0] / total_n_cells;
+    out->volume_ave_TK = sums[1] / total_n_cells;
+    out->volume_ave_xe = sums[2] / total_n_cells;
+  } else {
+    out->volume_ave_J_alpha = sums[0] / total_n_cells;
+    out->volume_ave_xalpha = sums[1] / total_n_cells;
+    out->volume_ave_Xheat = sums[2] / total_n_cells;
+    out->volume_ave_Xion = sums[3] / total_n_cells;
+    out->volume_ave_TS = sums[4] / total_n_cells;
+    out->volume_ave_TK = sums[5] / total_n_cells;
+    out->volume_ave_xe = sums[6] / total_n_cells;
+  }
+  out->device_ms = ms;
+  return MXR_OK;
+}
+
+int run_compute_ts(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_globals* out,
+                   mxr_ts_tables* tables)
+{
+  switch (c->N) {
+#define X(n)                                                                                                         \
+  case n:                                                                                                            \
+    return ts_run<n>(c, p, g, out, tables);
+    MXR_DIMS(X)
+#undef X
+    default: set_error("unsupported grid dimension %d", c->N); return MXR_ERR_INVALID_ARG;
+  }
+}
+
+}  // namespace mxr

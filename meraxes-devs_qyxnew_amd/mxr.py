@@ -142,6 +142,123 @@ class TbGrids(ctypes.Structure):
     ]
 
 
+MXR_TS_MAX_STEPS = 64
+MXR_TS_MAX_OPS = 16
+MXR_X_INT_NXHII = 14
+TS_OP_NAMES = {0: "clear", 1: "scale", 2: "load", 3: "add"}
+
+
+class TsParams(ctypes.Structure):
+    _fields_ = [
+        ("ReionGridDim", ctypes.c_int),
+        ("BoxSize", ctypes.c_double),
+        ("Hubble_h", ctypes.c_double),
+        ("OmegaM", ctypes.c_double),
+        ("OmegaLambda", ctypes.c_double),
+        ("OmegaK", ctypes.c_double),
+        ("OmegaR", ctypes.c_double),
+        ("wLambda", ctypes.c_double),
+        ("BaryonFrac", ctypes.c_double),
+        ("Y_He", ctypes.c_double),
+        ("ReionRtoMFilterType", ctypes.c_int),
+        ("TsNumFilterSteps", ctypes.c_int),
+        ("TsHeatingFilterType", ctypes.c_int),
+        ("ReionMaxHeatingRedshift", ctypes.c_double),
+        ("ReionEfficiency", ctypes.c_double),
+        ("ReionNionPhotPerBary", ctypes.c_double),
+        ("NuXrayThreshold", ctypes.c_double),
+        ("NuXraySoftCut", ctypes.c_double),
+        ("NuXrayMax", ctypes.c_double),
+        ("SpecIndexXrayGal", ctypes.c_double),
+        ("UnitLength_in_cm", ctypes.c_double),
+        ("UnitMass_in_g", ctypes.c_double),
+        ("UnitVelocity_in_cm_per_s", ctypes.c_double),
+        ("snapshot", ctypes.c_int),
+        ("ZZ", ctypes.POINTER(ctypes.c_double)),
+        ("TablesForXHeatingDir", ctypes.c_char_p),
+    ]
+
+
+class TsGrids(ctypes.Structure):
+    _fields_ = [
+        ("mem", ctypes.c_int),
+        ("deltax", ctypes.c_void_p),
+        ("stars", ctypes.c_void_p),
+        ("sfr_histories", ctypes.c_void_p),
+        ("xray_luminosity_histories", ctypes.c_void_p),
+        ("n_histories", ctypes.c_int),
+        ("x_e_box_prev", ctypes.c_void_p),
+        ("x_e_box", ctypes.c_void_p),
+        ("Tk_box", ctypes.c_void_p),
+        ("TS_box", ctypes.c_void_p),
+        ("stored_fcoll", ctypes.c_void_p),
+    ]
+
+
+_S, _K, _X = MXR_TS_MAX_STEPS, MXR_TS_MAX_OPS, MXR_X_INT_NXHII
+
+
+class TsTables(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int),
+        ("homogeneous", ctypes.c_int),
+        ("NO_LIGHT", ctypes.c_int),
+        ("zp", ctypes.c_double),
+        ("dzp", ctypes.c_double),
+        ("dt_dzp", ctypes.c_double),
+        ("growth_factor_zp", ctypes.c_double),
+        ("dgrowth_factor_dzp", ctypes.c_double),
+        ("const_zp_prefactor_GAL", ctypes.c_double),
+        ("collapse_fraction", ctypes.c_double),
+        ("x_e_ave", ctypes.c_double),
+        ("filling_factor_of_HI_zp", ctypes.c_double),
+        ("R", ctypes.c_double * _S),
+        ("zpp_edge", ctypes.c_double * _S),
+        ("zpp", ctypes.c_double * _S),
+        ("nu_tau_one", ctypes.c_double * _S),
+        ("sum_lyn", ctypes.c_double * _S),
+        ("freq_int_heat", (ctypes.c_double * _S) * _X),
+        ("freq_int_ion", (ctypes.c_double * _S) * _X),
+        ("freq_int_lya", (ctypes.c_double * _S) * _X),
+        ("x_int_XHII", ctypes.c_float * _X),
+        ("n_ops", ctypes.c_int * _S),
+        ("op_kind", (ctypes.c_int * _K) * _S),
+        ("op_hist", (ctypes.c_int * _K) * _S),
+        ("op_weight", (ctypes.c_double * _K) * _S),
+        ("hist_max", ctypes.c_int),
+    ]
+
+    def arr(self, name):
+        """numpy view of a field, trimmed to the n shells."""
+        a = np.ctypeslib.as_array(getattr(self, name))
+        if name in ("R", "zpp_edge", "zpp", "nu_tau_one", "sum_lyn"):
+            return a[: self.n].copy()
+        if name.startswith("freq_int"):
+            return a[:, : self.n].copy()
+        return a.copy()
+
+    def ops(self):
+        """[[(kind_name, hist, weight), ...] per shell]"""
+        return [[(TS_OP_NAMES[self.op_kind[r][k]], self.op_hist[r][k], self.op_weight[r][k])
+                 for k in range(self.n_ops[r])] for r in range(self.n)]
+
+
+class TsGlobals(ctypes.Structure):
+    _fields_ = [
+        ("volume_ave_J_alpha", ctypes.c_double),
+        ("volume_ave_xalpha", ctypes.c_double),
+        ("volume_ave_Xheat", ctypes.c_double),
+        ("volume_ave_Xion", ctypes.c_double),
+        ("volume_ave_TS", ctypes.c_double),
+        ("volume_ave_TK", ctypes.c_double),
+        ("volume_ave_xe", ctypes.c_double),
+        ("device_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 # ---------------------------------------------------------------------------------------
 # Parameter sets.  Cosmology and units: Genesis L10_N512
 # (input/params/simulations/Genesis_L10_N512.par:17-31); reionisation parameters from
@@ -195,6 +312,38 @@ def tb_params(N: int, redshift: float = 8.0, spin_temp: bool = False) -> TbParam
     return p
 
 
+# Spin-temperature defaults: input/params/defaults.par:51-68, 245, 274-282; OmegaR 0 and
+# wLambda -1 as in input/params/simulations/Genesis_*.par.
+TS_DEFAULTS = dict(OmegaR=0.0, wLambda=-1.0, Y_He=0.24, ReionRtoMFilterType=0, TsNumFilterSteps=40,
+                   TsHeatingFilterType=1, ReionMaxHeatingRedshift=20.0, ReionEfficiency=1.0,
+                   ReionNionPhotPerBary=4000.0, NuXrayThreshold=500.0, NuXraySoftCut=2000.0, NuXrayMax=10000.0,
+                   SpecIndexXrayGal=1.0)
+
+
+def snapshot_redshifts(z_hi: float = 20.0, z_lo: float = 8.0, n: int = 41) -> np.ndarray:
+    """Snapshot redshifts linear in z, z_hi first (snapshot 0) down to z_lo."""
+    return np.linspace(z_hi, z_lo, n).astype(np.float64)
+
+
+def ts_params(N: int, box: float, ZZ: np.ndarray, snapshot: int, tables_dir: str | None = None, **over) -> TsParams:
+    p = TsParams()
+    p.ReionGridDim = N
+    p.BoxSize = box
+    for k in ("Hubble_h", "OmegaM", "OmegaLambda", "OmegaK", "BaryonFrac", "UnitLength_in_cm", "UnitMass_in_g",
+              "UnitVelocity_in_cm_per_s"):
+        setattr(p, k, GENESIS[k])
+    for k, v in TS_DEFAULTS.items():
+        setattr(p, k, v)
+    zz = np.ascontiguousarray(ZZ, dtype=np.float64)
+    p._zz = zz  # keeps the array alive with the struct
+    p.ZZ = zz.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    p.snapshot = snapshot
+    p.TablesForXHeatingDir = (tables_dir or TABLES_DIR).encode()
+    for k, v in over.items():
+        setattr(p, k, v)
+    return p
+
+
 def padded_len(N: int) -> int:
     return N * N * 2 * (N // 2 + 1)
 
@@ -241,6 +390,14 @@ def lib():
     L.mxr_brightness_temperature.argtypes = [vp, ctypes.POINTER(TbParams), ctypes.POINTER(TbGrids),
                                              ctypes.POINTER(ctypes.c_double)]
     L.mxr_brightness_temperature.restype = ctypes.c_int
+    L.mxr_compute_ts.argtypes = [vp, ctypes.POINTER(TsParams), ctypes.POINTER(TsGrids), ctypes.POINTER(TsGlobals),
+                                 ctypes.POINTER(TsTables)]
+    L.mxr_compute_ts.restype = ctypes.c_int
+    L.mxr_ts_build_tables.argtypes = [ctypes.POINTER(TsParams), ctypes.c_double, ctypes.c_double,
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(TsTables)]
+    L.mxr_ts_build_tables.restype = ctypes.c_int
+    L.mxr_ts_history_depth.argtypes = [ctypes.POINTER(TsParams), ctypes.c_int]
+    L.mxr_ts_history_depth.restype = ctypes.c_int
     L.mxr_profile_enable.argtypes = [vp, ctypes.c_int]
     L.mxr_profile_enable.restype = ctypes.c_int
     L.mxr_profile_read.argtypes = [vp, ctypes.POINTER(KernelStat), ctypes.c_int]
@@ -297,6 +454,23 @@ def radii(params: HiiParams):
     return list(buf)
 
 
+def ts_history_depth(params: TsParams, last_snapshot: int) -> int:
+    n = lib().mxr_ts_history_depth(ctypes.byref(params), last_snapshot)
+    if n < 0:
+        check(n, "mxr_ts_history_depth")
+    return n
+
+
+def ts_build_tables(params: TsParams, collapse_fraction: float, x_e_ave: float, stored_fcoll: np.ndarray) -> TsTables:
+    """Host-only tables of one spin-temperature step (no GPU needed)."""
+    sf = np.ascontiguousarray(stored_fcoll, dtype=np.float64)
+    t = TsTables()
+    check(lib().mxr_ts_build_tables(ctypes.byref(params), collapse_fraction, x_e_ave,
+                                    sf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(t)),
+          "mxr_ts_build_tables")
+    return t
+
+
 class Context:
     """One mxr_ctx: a device, a grid dimension, persistent buffers and tuned kernels."""
 
@@ -336,6 +510,27 @@ class Context:
         check(lib().mxr_brightness_temperature(self._h, ctypes.byref(params), ctypes.byref(g), ctypes.byref(ave)),
               "mxr_brightness_temperature")
         return ave.value
+
+    def compute_ts(self, params: TsParams, deltax, stars, sfr_histories, xray_luminosity_histories, n_histories: int,
+                   x_e_box_prev, x_e_box, Tk_box, TS_box, stored_fcoll: np.ndarray, want_tables: bool = True):
+        """One ComputeTs step.  Returns (TsGlobals, TsTables or None); stored_fcoll (host float64, one
+        entry per snapshot) is updated in place."""
+        g = TsGrids()
+        g.mem = _mem_of([deltax, stars, sfr_histories, xray_luminosity_histories, x_e_box_prev, x_e_box, Tk_box,
+                         TS_box])
+        if g.mem == MXR_MEM_DEVICE:
+            _sync_torch()
+        g.deltax, g.stars = _ptr(deltax), _ptr(stars)
+        g.sfr_histories, g.xray_luminosity_histories = _ptr(sfr_histories), _ptr(xray_luminosity_histories)
+        g.n_histories = n_histories
+        g.x_e_box_prev, g.x_e_box, g.Tk_box, g.TS_box = _ptr(x_e_box_prev), _ptr(x_e_box), _ptr(Tk_box), _ptr(TS_box)
+        assert isinstance(stored_fcoll, np.ndarray) and stored_fcoll.dtype == np.float64
+        g.stored_fcoll = stored_fcoll.ctypes.data
+        out = TsGlobals()
+        tab = TsTables() if want_tables else None
+        check(lib().mxr_compute_ts(self._h, ctypes.byref(params), ctypes.byref(g), ctypes.byref(out),
+                                   ctypes.byref(tab) if tab is not None else None), "mxr_compute_ts")
+        return out, tab
 
     def profile(self, on: bool = True):
         check(lib().mxr_profile_enable(self._h, int(on)), "mxr_profile_enable")

@@ -127,3 +127,45 @@ def history_sfr(N: int, box: float, seed: int, redshifts, device="cpu", **kw):
         r = realisation(N, box, seed + i, device=device, **kw)
         out.append(r["weighted_sfr"] * float((1 + z_last) / (1 + z)))
     return out
+
+
+SOLAR_MASS = 1.989e33
+SEC_PER_YEAR = 3.155e7
+LXRAY_GAL = 3.16e40  # erg/s per Msun/yr (input/params/defaults.par:51)
+
+
+def ts_inputs(N: int, box: float, seed: int, ZZ, snapshot: int, n_hist: int, device="cpu",
+              stars_to_matter: float = 1.0, params=None):
+    """Inputs of one spin-temperature step at `snapshot`: deltax and stars of that snapshot and
+    n_hist padded SFR / X-ray luminosity history grids, [s] = s snapshots back.
+
+    Snapshot i's SFR grid is the realisation with seed + i scaled by (1 + z_last) / (1 + z_i)
+    (z_last = ZZ[-1]); histories before snapshot 0 are zero.  X-ray luminosity follows
+    reionization.c:1982-2022 with Flag_XrayMetallicity = 0: LXrayGal x SFR[Msun/yr] / 1e40."""
+    from mxr import GENESIS  # noqa: E402  (sibling module)
+
+    p = dict(GENESIS)
+    if params:
+        p.update(params)
+    UT = p["UnitLength_in_cm"] / p["UnitVelocity_in_cm_per_s"]
+    to_msun_yr = p["UnitMass_in_g"] / SOLAR_MASS * SEC_PER_YEAR / UT
+    H = N // 2 + 1
+    npad = N * N * 2 * H
+    cur = realisation(N, box, seed + snapshot, device=device, stars_to_matter=stars_to_matter, params=params)
+    sfr_h = torch.zeros(n_hist * npad, dtype=torch.float32, device=device)
+    xr_h = torch.zeros(n_hist * npad, dtype=torch.float32, device=device)
+    z_last = float(ZZ[-1])
+    for s in range(n_hist):
+        i = snapshot - s
+        if i < 0:
+            break
+        if s == 0:
+            sfr = cur["weighted_sfr"]
+        else:
+            sfr = realisation(N, box, seed + i, device=device, stars_to_matter=stars_to_matter,
+                              params=params)["weighted_sfr"]
+        sfr = (sfr.double() * ((1.0 + z_last) / (1.0 + float(ZZ[i])))).float()
+        sfr_h[s * npad:(s + 1) * npad] = sfr
+        xr_h[s * npad:(s + 1) * npad] = (sfr.double() * (to_msun_yr * LXRAY_GAL / 1e40)).float()
+        del sfr
+    return dict(deltax=cur["deltax"], stars=cur["stars"], sfr_histories=sfr_h, xray_luminosity_histories=xr_h)

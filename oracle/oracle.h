@@ -59,6 +59,13 @@ int oracle_hii_sample_f32(const mxr_hii_params* p, const mxr_hii_grids* g, int n
 
 int oracle_brightness_temperature(const mxr_tb_params* p, const mxr_tb_grids* g, double* volume_ave_Tb);
 
+/* One spin-temperature step on host grids, with the grid-independent tables `tb` the library
+ * reported for the same call (mxr_ts_tables).  red0 (optional, 3 doubles) receives the R_ct = 0
+ * averages: collapse fraction, x_e, X-ray emissivity; smoothed (optional, 2 F N^3 floats) the
+ * smoothed SFR (clamped) then X-ray shells, real layout. */
+int oracle_ts_step_f32(const mxr_ts_params* p, const mxr_ts_tables* tb, const mxr_ts_grids* g,
+                       mxr_ts_globals* out, double* red0, float* smoothed);
+
 int oracle_num_threads(void);
 
 #ifdef __cplusplus

@@ -66,6 +66,9 @@ def lib():
     L.oracle_brightness_temperature.argtypes = [ctypes.POINTER(mxr.TbParams), ctypes.POINTER(mxr.TbGrids),
                                                 ctypes.POINTER(ctypes.c_double)]
     L.oracle_num_threads.restype = ctypes.c_int
+    L.oracle_ts_step_f32.argtypes = [ctypes.POINTER(mxr.TsParams), ctypes.POINTER(mxr.TsTables),
+                                     ctypes.POINTER(mxr.TsGrids), ctypes.POINTER(mxr.TsGlobals), fp, fp]
+    L.oracle_ts_step_f32.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -159,3 +162,35 @@ def brightness_temperature(params, xH, deltax, TS_box=None):
     if rc:
         raise mxr.MxrError(rc, "oracle_brightness_temperature")
     return dT, ave.value
+
+
+def ts_step(params, tables, deltax, stars, sfr_histories, xray_histories, n_hist, x_e_box_prev, Tk_box,
+            want_smoothed=False):
+    """One oracle spin-temperature step on copies of the state.  Returns a dict with the new
+    x_e_box_prev, x_e_box, Tk_box, TS_box, the globals, red0 (collapse fraction, <x_e>, <L_X>)
+    and (optionally) the smoothed shells [2, F, N^3]."""
+    N = params.ReionGridDim
+    npad, nreal = mxr.padded_len(N), N ** 3
+    xe = np.array(x_e_box_prev, np.float32, copy=True)
+    Tk = np.array(Tk_box, np.float32, copy=True)
+    xe_out = np.zeros(npad, np.float32)
+    TS = np.zeros(nreal, np.float32)
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (deltax, stars, sfr_histories, xray_histories)]
+    g = mxr.TsGrids()
+    g.mem = mxr.MXR_MEM_HOST
+    g.deltax, g.stars, g.sfr_histories, g.xray_luminosity_histories = (_p(a) for a in arrs)
+    g.n_histories = n_hist
+    g.x_e_box_prev, g.x_e_box, g.Tk_box, g.TS_box = _p(xe), _p(xe_out), _p(Tk), _p(TS)
+    sf = np.zeros(1, np.float64)
+    g.stored_fcoll = _p(sf)
+    out = mxr.TsGlobals()
+    red0 = np.zeros(3, np.float64)
+    sm = np.zeros(2 * max(tables.n, 1) * nreal, np.float32) if want_smoothed else None
+    rc = lib().oracle_ts_step_f32(ctypes.byref(params), ctypes.byref(tables), ctypes.byref(g), ctypes.byref(out),
+                                  _p(red0), _p(sm) if sm is not None else None)
+    if rc != 0:
+        raise RuntimeError(f"oracle_ts_step_f32 failed: {rc}")
+    res = dict(x_e_box_prev=xe, x_e_box=xe_out, Tk_box=Tk, TS_box=TS, globals=out.as_dict(), red0=red0)
+    if sm is not None:
+        res["smoothed"] = sm.reshape(2, tables.n, nreal)
+    return res
