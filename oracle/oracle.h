@@ -65,6 +65,10 @@ int oracle_brightness_temperature(const mxr_tb_params* p, const mxr_tb_grids* g,
  * smoothed SFR (clamped) then X-ray shells, real layout. */
 int oracle_ts_step_f32(const mxr_ts_params* p, const mxr_ts_tables* tb, const mxr_ts_grids* g,
                        mxr_ts_globals* out, double* red0, float* smoothed);
+/* The same with the shells smoothed in FP64 (double source grids, FFTs, filter, clamp): the
+ * FP64 anchor the FP32 paths' errors are quoted against. */
+int oracle_ts_step_f64(const mxr_ts_params* p, const mxr_ts_tables* tb, const mxr_ts_grids* g,
+                       mxr_ts_globals* out, double* red0, float* smoothed);
 
 int oracle_num_threads(void);
 

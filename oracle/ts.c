@@ -276,18 +276,79 @@ static void evolve_int(const ts_env* e, const mxr_ts_tables* tb, const mxr_ts_pa
 
 static float cT_approx(float z) { return 0.58 - 0.006 * (z - 10.0); }
 
-static void load_grid(float* dst, const float* hist, float w, int new_load, long npad)
+/* The shells of one call (ComputeTs.c:366-746) in FP32 (fftwf, float filter and clamp, as
+ * the reference) or FP64 (double source grids, FFTs, filter and clamp).  sm receives the
+ * smoothed SFR then X-ray shells [2][F][N^3]; red0 the R_ct = 0 averages. */
+#define SMOOTH_BODY(REAL, R2C, C2R, FILTER, RCAST, FMAX)                                                              \
+  do {                                                                                                                 \
+    REAL* src[2] = {malloc(sizeof(REAL) * npad), malloc(sizeof(REAL) * npad)};                                        \
+    REAL* spec[2] = {malloc(sizeof(REAL) * 2 * nc), malloc(sizeof(REAL) * 2 * nc)};                                   \
+    REAL* work = malloc(sizeof(REAL) * 2 * nc);                                                                        \
+    if (!src[0] || !src[1] || !spec[0] || !spec[1] || !work) {                                                         \
+      rc = -3;                                                                                                         \
+    } else {                                                                                                           \
+      for (int k = 0; k < 2; k++)                                                                                      \
+        for (long i = 0; i < npad; i++)                                                                                \
+          src[k][i] = hist[k][i];                                                                                      \
+      for (int r = 0; r < F; r++) {                                                                                    \
+        for (int q = 0; q < tb->n_ops[r]; q++) {                                                                       \
+          const int kind = tb->op_kind[r][q], hi = tb->op_hist[r][q];                                                  \
+          const double w = tb->op_weight[r][q];                                                                        \
+          for (int k = 0; k < 2; k++) {                                                                                \
+            const float* h = hist[k] + (long)hi * npad;                                                                \
+            for (long i = 0; i < npad; i++) {                                                                          \
+              if (kind == MXR_TS_OP_CLEAR)                                                                             \
+                src[k][i] = 0;                                                                                         \
+              else if (kind == MXR_TS_OP_SCALE)                                                                        \
+                src[k][i] *= w;                                                                                        \
+              else if (kind == MXR_TS_OP_LOAD)                                                                         \
+                src[k][i] = h[i] * (RCAST)w;                                                                           \
+              else                                                                                                     \
+                src[k][i] += h[i] * (RCAST)w;                                                                          \
+            }                                                                                                          \
+          }                                                                                                            \
+        }                                                                                                              \
+        for (int k = 0; k < 2; k++) {                                                                                  \
+          if (r == 0 || tb->n_ops[r] > 0) {                                                                            \
+            R2C(N, src[k], spec[k]);                                                                                   \
+            for (long i = 0; i < 2 * nc; i++)                                                                          \
+              spec[k][i] /= (RCAST)total_n_cells;                                                                      \
+          }                                                                                                            \
+          memcpy(work, spec[k], sizeof(REAL) * 2 * nc);                                                                \
+          if (r > 0)                                                                                                   \
+            FILTER(work, 0, N, N, p->BoxSize, (RCAST)tb->R[r], p->TsHeatingFilterType);                                \
+          C2R(N, work);                                                                                                \
+          double* dst = sm + ((size_t)k * F + r) * nreal;                                                              \
+          for (long ir = 0; ir < nreal; ir++) {                                                                        \
+            long ip = (ir / N) * 2 * H + ir % N;                                                                       \
+            dst[ir] = k == 0 ? FMAX(work[ip], 0.0) : work[ip];                                                         \
+          }                                                                                                            \
+        }                                                                                                              \
+      }                                                                                                                \
+    }                                                                                                                  \
+    free(src[0]);                                                                                                      \
+    free(src[1]);                                                                                                      \
+    free(spec[0]);                                                                                                     \
+    free(spec[1]);                                                                                                     \
+    free(work);                                                                                                        \
+  } while (0)
+
+static int smooth(const mxr_ts_params* p, const mxr_ts_tables* tb, const mxr_ts_grids* g, double* sm, int f64)
 {
-  for (long i = 0; i < npad; i++) {
-    if (new_load)
-      dst[i] = hist[i] * w;
-    else
-      dst[i] += hist[i] * w;
-  }
+  const int N = p->ReionGridDim, H = N / 2 + 1, F = tb->n;
+  const long nreal = (long)N * N * N, nc = (long)N * N * H, npad = 2 * nc;
+  const double total_n_cells = pow((double)N, 3);
+  const float* hist[2] = {g->sfr_histories, g->xray_luminosity_histories};
+  int rc = 0;
+  if (f64)
+    SMOOTH_BODY(double, oracle_r2c3d_f64, oracle_c2r3d_f64, oracle_filter_f64, double, fmax);
+  else
+    SMOOTH_BODY(float, oracle_r2c3d_f32, oracle_c2r3d_f32, oracle_filter_f32, float, fmaxf);
+  return rc;
 }
 
-int oracle_ts_step_f32(const mxr_ts_params* p, const mxr_ts_tables* tb, const mxr_ts_grids* g,
-                       mxr_ts_globals* out, double* red0, float* smoothed)
+static int ts_step(const mxr_ts_params* p, const mxr_ts_tables* tb, const mxr_ts_grids* g, mxr_ts_globals* out,
+                   double* red0, float* smoothed, int f64)
 {
   if (!p || !tb || !g || !out || g->mem != MXR_MEM_HOST)
     return -1;
@@ -307,7 +368,6 @@ int oracle_ts_step_f32(const mxr_ts_params* p, const mxr_ts_tables* tb, const mx
   float* Tk = g->Tk_box;
   float* TS = g->TS_box;
   double Ave_Ts = 0, Ave_Tk = 0, Ave_xe = 0;
-  int rc = 0;
 
   if (tb->homogeneous) {
     float zf = (float)zp;
@@ -323,82 +383,44 @@ int oracle_ts_step_f32(const mxr_ts_params* p, const mxr_ts_tables* tb, const mx
     }
   } else {
     const int F = tb->n;
-    float* src[2] = {malloc(sizeof(float) * npad), malloc(sizeof(float) * npad)};
-    float* spec[2] = {malloc(sizeof(float) * 2 * nc), malloc(sizeof(float) * 2 * nc)};
-    float* work = malloc(sizeof(float) * 2 * nc);
-    float* sm = malloc(sizeof(float) * 2 * (size_t)F * nreal);
-    if (!src[0] || !src[1] || !spec[0] || !spec[1] || !work || !sm) {
-      rc = -3;
-      goto done_inh;
+    double* sm = malloc(sizeof(double) * 2 * (size_t)F * nreal);
+    if (!sm || smooth(p, tb, g, sm, f64)) {
+      free(sm);
+      free(e);
+      return -3;
     }
-    const float* hist[2] = {g->sfr_histories, g->xray_luminosity_histories};
-    for (int k = 0; k < 2; k++)
-      memcpy(src[k], hist[k], sizeof(float) * npad);
-    double collapse_fraction = 0, x_e_ave = 0, hmxb = 0;
-    for (int r = 0; r < F; r++) {
-      for (int q = 0; q < tb->n_ops[r]; q++) {
-        const int kind = tb->op_kind[r][q], hi = tb->op_hist[r][q];
-        const double w = tb->op_weight[r][q];
-        for (int k = 0; k < 2; k++) {
-          if (kind == MXR_TS_OP_CLEAR)
-            memset(src[k], 0, sizeof(float) * npad);
-          else if (kind == MXR_TS_OP_SCALE)
-            for (long i = 0; i < npad; i++)
-              src[k][i] *= w;
-          else
-            load_grid(src[k], hist[k] + (long)hi * npad, (float)w, kind == MXR_TS_OP_LOAD, npad);
-        }
+    {
+      double collapse_fraction = 0, x_e_ave = 0, hmxb = 0;
+      mxr_hii_params hp;
+      memset(&hp, 0, sizeof(hp));
+      hp.OmegaM = p->OmegaM;
+      hp.ReionRtoMFilterType = p->ReionRtoMFilterType;
+      hp.UnitLength_in_cm = p->UnitLength_in_cm;
+      hp.UnitMass_in_g = p->UnitMass_in_g;
+      hp.UnitVelocity_in_cm_per_s = p->UnitVelocity_in_cm_per_s;
+      const double R = tb->R[0];
+      const double M = oracle_RtoM(&hp, R);
+      for (long ir = 0; ir < nreal; ir++) {
+        long ip = (ir / N) * 2 * H + ir % N;
+        double dom = 1.0 + g->deltax[ip];
+        double cf =
+          g->stars[ip] / (M * p->Hubble_h * p->Hubble_h * dom) * (4.0 / 3.0) * M_PI * pow(R, 3.0) / pixel_volume;
+        if (cf > 1.0)
+          cf = 1.0;
+        collapse_fraction += cf;
+        x_e_ave += xe[ip];
+        hmxb += (sm[((size_t)F + 0) * nreal + ir] / pixel_volume) * XRAY_LUMINOSITY_UNIT *
+                pow(p->UnitLength_in_cm, -3.);
       }
-      for (int k = 0; k < 2; k++) {
-        if (r == 0 || tb->n_ops[r] > 0) {
-          oracle_r2c3d_f32(N, src[k], spec[k]);
-          for (long i = 0; i < 2 * nc; i++)
-            spec[k][i] /= (float)total_n_cells;
-        }
-        memcpy(work, spec[k], sizeof(float) * 2 * nc);
-        if (r > 0)
-          oracle_filter_f32(work, 0, N, N, p->BoxSize, (float)tb->R[r], p->TsHeatingFilterType);
-        oracle_c2r3d_f32(N, work);
-        float* dst = sm + ((size_t)k * F + r) * nreal;
-        for (long ir = 0; ir < nreal; ir++) {
-          long ip = (ir / N) * 2 * H + ir % N;
-          dst[ir] = k == 0 ? fmaxf(work[ip], 0.0) : work[ip];
-        }
-      }
-      if (r == 0) {
-        mxr_hii_params hp;
-        memset(&hp, 0, sizeof(hp));
-        hp.OmegaM = p->OmegaM;
-        hp.ReionRtoMFilterType = p->ReionRtoMFilterType;
-        hp.UnitLength_in_cm = p->UnitLength_in_cm;
-        hp.UnitMass_in_g = p->UnitMass_in_g;
-        hp.UnitVelocity_in_cm_per_s = p->UnitVelocity_in_cm_per_s;
-        const double R = tb->R[0];
-        const double M = oracle_RtoM(&hp, R);
-        for (long ir = 0; ir < nreal; ir++) {
-          long ip = (ir / N) * 2 * H + ir % N;
-          double dom = 1.0 + g->deltax[ip];
-          double cf = g->stars[ip] / (M * p->Hubble_h * p->Hubble_h * dom) * (4.0 / 3.0) * M_PI * pow(R, 3.0) /
-                      pixel_volume;
-          if (cf > 1.0)
-            cf = 1.0;
-          collapse_fraction += cf;
-          x_e_ave += xe[ip];
-          hmxb += (sm[((size_t)F + 0) * nreal + ir] / pixel_volume) * XRAY_LUMINOSITY_UNIT *
-                  pow(p->UnitLength_in_cm, -3.);
-        }
-        collapse_fraction /= total_n_cells;
-        x_e_ave /= total_n_cells;
-        hmxb /= total_n_cells;
-        if (red0) {
-          red0[0] = collapse_fraction;
-          red0[1] = x_e_ave;
-          red0[2] = hmxb;
-        }
+      if (red0) {
+        red0[0] = collapse_fraction / total_n_cells;
+        red0[1] = x_e_ave / total_n_cells;
+        red0[2] = hmxb / total_n_cells;
       }
     }
     if (smoothed)
-      memcpy(smoothed, sm, sizeof(float) * 2 * (size_t)F * nreal);
+      for (size_t i = 0; i < 2 * (size_t)F * nreal; i++)
+        smoothed[i] = (float)sm[i];
     double J_alpha_ave = 0, xalpha_ave = 0, Xheat_ave = 0, Xion_ave = 0;
     double SFR[MXR_TS_MAX_STEPS], XR[MXR_TS_MAX_STEPS], fh[MXR_TS_MAX_STEPS], fi[MXR_TS_MAX_STEPS],
       fl[MXR_TS_MAX_STEPS];
@@ -451,17 +473,7 @@ int oracle_ts_step_f32(const mxr_ts_params* p, const mxr_ts_tables* tb, const mx
     out->volume_ave_xalpha = xalpha_ave / total_n_cells;
     out->volume_ave_Xheat = Xheat_ave / total_n_cells;
     out->volume_ave_Xion = Xion_ave / total_n_cells;
-  done_inh:
-    free(src[0]);
-    free(src[1]);
-    free(spec[0]);
-    free(spec[1]);
-    free(work);
     free(sm);
-    if (rc) {
-      free(e);
-      return rc;
-    }
   }
   memcpy(g->x_e_box, xe, sizeof(float) * npad);
   for (long ir = 0; ir < nreal; ir++) {
@@ -475,4 +487,16 @@ int oracle_ts_step_f32(const mxr_ts_params* p, const mxr_ts_tables* tb, const mx
   out->volume_ave_xe = Ave_xe / total_n_cells;
   free(e);
   return 0;
+}
+
+int oracle_ts_step_f32(const mxr_ts_params* p, const mxr_ts_tables* tb, const mxr_ts_grids* g,
+                       mxr_ts_globals* out, double* red0, float* smoothed)
+{
+  return ts_step(p, tb, g, out, red0, smoothed, 0);
+}
+
+int oracle_ts_step_f64(const mxr_ts_params* p, const mxr_ts_tables* tb, const mxr_ts_grids* g,
+                       mxr_ts_globals* out, double* red0, float* smoothed)
+{
+  return ts_step(p, tb, g, out, red0, smoothed, 1);
 }

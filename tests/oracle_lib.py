@@ -66,9 +66,10 @@ def lib():
     L.oracle_brightness_temperature.argtypes = [ctypes.POINTER(mxr.TbParams), ctypes.POINTER(mxr.TbGrids),
                                                 ctypes.POINTER(ctypes.c_double)]
     L.oracle_num_threads.restype = ctypes.c_int
-    L.oracle_ts_step_f32.argtypes = [ctypes.POINTER(mxr.TsParams), ctypes.POINTER(mxr.TsTables),
+    for name in ("oracle_ts_step_f32", "oracle_ts_step_f64"):
+        getattr(L, name).argtypes = [ctypes.POINTER(mxr.TsParams), ctypes.POINTER(mxr.TsTables),
                                      ctypes.POINTER(mxr.TsGrids), ctypes.POINTER(mxr.TsGlobals), fp, fp]
-    L.oracle_ts_step_f32.restype = ctypes.c_int
+        getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
 
@@ -165,7 +166,7 @@ def brightness_temperature(params, xH, deltax, TS_box=None):
 
 
 def ts_step(params, tables, deltax, stars, sfr_histories, xray_histories, n_hist, x_e_box_prev, Tk_box,
-            want_smoothed=False):
+            want_smoothed=False, f64=False):
     """One oracle spin-temperature step on copies of the state.  Returns a dict with the new
     x_e_box_prev, x_e_box, Tk_box, TS_box, the globals, red0 (collapse fraction, <x_e>, <L_X>)
     and (optionally) the smoothed shells [2, F, N^3]."""
@@ -186,10 +187,11 @@ def ts_step(params, tables, deltax, stars, sfr_histories, xray_histories, n_hist
     out = mxr.TsGlobals()
     red0 = np.zeros(3, np.float64)
     sm = np.zeros(2 * max(tables.n, 1) * nreal, np.float32) if want_smoothed else None
-    rc = lib().oracle_ts_step_f32(ctypes.byref(params), ctypes.byref(tables), ctypes.byref(g), ctypes.byref(out),
+    fn = lib().oracle_ts_step_f64 if f64 else lib().oracle_ts_step_f32
+    rc = fn(ctypes.byref(params), ctypes.byref(tables), ctypes.byref(g), ctypes.byref(out),
                                   _p(red0), _p(sm) if sm is not None else None)
     if rc != 0:
-        raise RuntimeError(f"oracle_ts_step_f32 failed: {rc}")
+        raise RuntimeError(f"oracle ts step failed: {rc}")
     res = dict(x_e_box_prev=xe, x_e_box=xe_out, Tk_box=Tk, TS_box=TS, globals=out.as_dict(), red0=red0)
     if sm is not None:
         res["smoothed"] = sm.reshape(2, tables.n, nreal)

@@ -2,8 +2,11 @@
 
 Each step feeds the HIP path and oracle/ts.c the same inputs and state; the oracle uses the
 grid-independent tables the library reports for that call (checked on the CPU in
-test_ts_tables.py).  Bound: T_S, T_K and x_e within 1e-4 relative per cell, globals within
-1e-5, the R_ct = 0 averages within 1e-9.
+test_ts_tables.py).  Bounds: T_S, T_K and x_e within 1e-4 relative per cell at the 99.9th
+percentile against the FP32 restatement; the largest per-cell error against the FP64-smoothed
+oracle no larger than twice the FP32 restatement's own (near-empty cells of sparse source
+fields carry FP32 FFT rounding noise, ~1e-4 relative in T_S, in both); globals within 1e-5; the
+R_ct = 0 averages within 1e-9.
 """
 import numpy as np
 import pytest
@@ -46,16 +49,22 @@ def _step(ctx, p, d, depth, xe, Tk, sf):
     return dict(x_e_box_prev=xe_g, x_e_box=xe_box, Tk_box=Tk_g, TS_box=TS, globals=g.as_dict()), tab
 
 
-def _compare(N, gpu, ref, tab, tag):
-    stats = {}
-    for k in ("TS_box", "Tk_box"):
-        e = rel_err(gpu[k], ref[k])
-        stats[k] = (float(np.percentile(e, 99.9)), float(e.max()))
-    e = rel_err(_cells(N, gpu["x_e_box_prev"]), _cells(N, ref["x_e_box_prev"]), floor=1e-6)
-    stats["x_e"] = (float(np.percentile(e, 99.9)), float(e.max()))
-    print(tag, {k: "p99.9 %.2e max %.2e" % v for k, v in stats.items()}, flush=True)
-    for k, (p999, mx) in stats.items():
-        assert mx <= TOL, (k, p999, mx)
+def _compare(N, gpu, ref, tab, tag, ref64=None):
+    fields = {"TS_box": (lambda r: r["TS_box"], 0.0), "Tk_box": (lambda r: r["Tk_box"], 0.0),
+              "x_e": (lambda r: _cells(N, r["x_e_box_prev"]), 1e-6)}
+    for k, (get, floor) in fields.items():
+        e = rel_err(get(gpu), get(ref), floor)
+        p999, mx = float(np.percentile(e, 99.9)), float(e.max())
+        line = f"{tag} {k}: vs FP32 p99.9 {p999:.2e} max {mx:.2e}"
+        assert p999 <= TOL, (k, p999)
+        if ref64 is not None:
+            eg = rel_err(get(gpu), get(ref64), floor).max()
+            eo = rel_err(get(ref), get(ref64), floor).max()
+            line += f" | vs FP64 max: gpu {eg:.2e} fp32-oracle {eo:.2e}"
+            assert eg <= 2 * eo + 2e-6, (k, eg, eo)
+        else:
+            assert mx <= TOL, (k, p999, mx)
+        print(line, flush=True)
     np.testing.assert_array_equal(gpu["x_e_box"], gpu["x_e_box_prev"])
     gg, rg = gpu["globals"], ref["globals"]
     keys = ["volume_ave_TS", "volume_ave_TK", "volume_ave_xe"]
@@ -67,7 +76,7 @@ def _compare(N, gpu, ref, tab, tag):
         assert gg[k] == pytest.approx(rg[k], rel=1e-5, abs=1e-30), k
 
 
-def _case(ctx, N, box, snap, scale=1.0, seed=7, **over):
+def _case(ctx, N, box, snap, scale=1.0, seed=7, anchor=True, **over):
     p = params(N, box, snap, **over)
     depth = max(1, mxr.ts_history_depth(p, 40))
     d = inputs(N, box, snap, depth, scale=scale)
@@ -75,15 +84,16 @@ def _case(ctx, N, box, snap, scale=1.0, seed=7, **over):
     sf = np.zeros(len(ZZ))
     sf[:snap] = 1e-3 * np.arange(snap) / max(snap, 1)
     gpu, tab = _step(ctx, p, d, depth, xe, Tk, sf)
-    ref = ol.ts_step(p, tab, d["deltax"], d["stars"], d["sfr_histories"], d["xray_luminosity_histories"], depth,
-                     xe, Tk)
-    return gpu, ref, tab
+    args = (p, tab, d["deltax"], d["stars"], d["sfr_histories"], d["xray_luminosity_histories"], depth, xe, Tk)
+    ref = ol.ts_step(*args)
+    ref64 = ol.ts_step(*args, f64=True) if (anchor and not tab.homogeneous) else None
+    return gpu, ref, tab, ref64
 
 
 def test_homogeneous_branch(ctxs):
     for N, box in [(16, 20.0), (64, 67.8)]:
         print(f"ts homogeneous N={N}", flush=True)
-        gpu, ref, tab = _case(ctxs(N), N, box, 0)
+        gpu, ref, tab, _ = _case(ctxs(N), N, box, 0)
         assert tab.homogeneous == 1
         _compare(N, gpu, ref, tab, f"homog N={N}")
 
@@ -92,22 +102,22 @@ def test_homogeneous_branch(ctxs):
                                               (64, 67.8, 25, 1e-2), (64, 67.8, 40, 1.0)])
 def test_parity_step(ctxs, N, box, snap, scale):
     print(f"ts step N={N} snap={snap} scale={scale}", flush=True)
-    gpu, ref, tab = _case(ctxs(N), N, box, snap, scale)
+    gpu, ref, tab, ref64 = _case(ctxs(N), N, box, snap, scale)
     assert tab.homogeneous == 0 and tab.NO_LIGHT == 0
-    _compare(N, gpu, ref, tab, f"step N={N} snap={snap} scale={scale}")
+    _compare(N, gpu, ref, tab, f"step N={N} snap={snap} scale={scale}", ref64)
 
 
 @pytest.mark.parametrize("ftype", [0, 2])
 def test_parity_filter_types(ctxs, ftype):
     print(f"ts filter type {ftype}", flush=True)
-    gpu, ref, tab = _case(ctxs(32), 32, 40.0, 20, 1e-2, TsHeatingFilterType=ftype)
-    _compare(32, gpu, ref, tab, f"filter {ftype}")
+    gpu, ref, tab, ref64 = _case(ctxs(32), 32, 40.0, 20, 1e-2, TsHeatingFilterType=ftype)
+    _compare(32, gpu, ref, tab, f"filter {ftype}", ref64)
 
 
 def test_parity_non_pow2(ctxs):
     print("ts N=96", flush=True)
-    gpu, ref, tab = _case(ctxs(96), 96, 100.0, 30, 1e-2)
-    _compare(96, gpu, ref, tab, "N=96")
+    gpu, ref, tab, ref64 = _case(ctxs(96), 96, 100.0, 30, 1e-2)
+    _compare(96, gpu, ref, tab, "N=96", ref64)
 
 
 def test_sequence(ctxs):
@@ -126,9 +136,10 @@ def test_sequence(ctxs):
         if xe is None:
             xe, Tk = state(N, d["deltax"])
         gpu, tab = _step(ctx, p, d, depth, xe, Tk, sf)
-        ref = ol.ts_step(p, tab, d["deltax"], d["stars"], d["sfr_histories"], d["xray_luminosity_histories"], depth,
-                         xe, Tk)
-        _compare(N, gpu, ref, tab, f"seq snap={snap}")
+        args = (p, tab, d["deltax"], d["stars"], d["sfr_histories"], d["xray_luminosity_histories"], depth, xe, Tk)
+        ref = ol.ts_step(*args)
+        ref64 = None if tab.homogeneous else ol.ts_step(*args, f64=True)
+        _compare(N, gpu, ref, tab, f"seq snap={snap}", ref64)
         if snap > 0:
             assert sf[snap] == tab.collapse_fraction
         xe, Tk = gpu["x_e_box_prev"], gpu["Tk_box"]
