@@ -11,7 +11,10 @@ Prints ONE JSON line.  Besides the contract fields it carries
                  library, algorithmic bytes per launch from the kept-mode geometry),
   plain        : the same workload through the one-kernel-per-step PLAIN path,
   cpu_baseline : the oracle's FP32 restatement timed on this host on a bounded sample,
-  parity       : xH of the fused vs plain path (bit-identical fields are expected).
+  parity       : xH of the fused vs plain path (bit-identical fields are expected),
+  spin_temperature : one ComputeTs step at 256^3 with 40 shells (z = 8, 41 snapshots z = 20..8,
+                 per-snapshot SFR / X-ray histories), cell-shell updates/s, with its per-kernel
+                 breakdown and the oracle's CPU time on a bounded sample.
 """
 from __future__ import annotations
 
@@ -69,6 +72,98 @@ def cpu_baseline(N, box, inputs_host, n_radii_sample, n_radii_total):
     }
 
 
+def ts_leg(args, torch):
+    """One spin-temperature step per timed call on device-resident inputs."""
+    import numpy as np
+
+    import mxr
+    import synth
+
+    N, box, snap = args.ts_N, args.ts_box, 40
+    ZZ = mxr.snapshot_redshifts()
+    p = mxr.ts_params(N, box, ZZ, snap)
+    F = p.TsNumFilterSteps
+    depth = mxr.ts_history_depth(p, len(ZZ) - 1)
+    log(f"bench: spin temperature N={N} box={box} shells={F} histories={depth}")
+    d = synth.ts_inputs(N, box, args.seed, ZZ, snap, depth, device="cuda", stars_to_matter=args.ts_stars_to_matter)
+    npad = mxr.padded_len(N)
+    xe = torch.zeros(npad, device="cuda")
+    xe_box = torch.zeros(npad, device="cuda")
+    Tk = torch.zeros(N ** 3, device="cuda")
+    TS = torch.zeros(N ** 3, device="cuda")
+    sf = np.zeros(len(ZZ))
+    ctx = mxr.Context(N)
+    # initial state: the homogeneous (RECFAST) branch at the same redshift
+    ph = mxr.ts_params(N, box, ZZ, snap, ReionMaxHeatingRedshift=0.0)
+    ctx.compute_ts(ph, d["deltax"], d["stars"], d["sfr_histories"], d["xray_luminosity_histories"], depth, xe, xe_box,
+                   Tk, TS, sf, want_tables=False)
+
+    def call():
+        return ctx.compute_ts(p, d["deltax"], d["stars"], d["sfr_histories"], d["xray_luminosity_histories"], depth,
+                              xe, xe_box, Tk, TS, sf, want_tables=False)[0]
+
+    for _ in range(args.warmup):
+        call()
+    torch.cuda.synchronize()
+    dev = []
+    t0 = time.perf_counter()
+    for _ in range(args.ts_steps):
+        g = call()
+        dev.append(g.device_ms)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.ts_steps
+    ctx.profile(True)
+    call()
+    ks = ctx.kernel_stats()
+    ctx.profile(False)
+    kernels = {k: dict(launches=v["launches"], total_ms=round(v["total_ms"], 3),
+                       GBps=round(v["alg_bytes"] / max(v["total_ms"], 1e-9) / 1e6, 1)) for k, v in ks.items()}
+    out = {"metric": f"cell-shell updates/s (ComputeTs, {N}^3, {F} shells)", "value": N ** 3 * F / (ms / 1e3),
+           "unit": "cell-shell updates/s", "ms_per_step": round(ms, 3),
+           "device_ms": round(sum(dev) / len(dev), 3), "steps": args.ts_steps,
+           "config": {"workload": f"ComputeTs N={N} box={box} cMpc/h, {F} shells, TsHeatingFilterType=1, "
+                                  f"snapshot 40 (z=8) of 41, {depth} history grids each of SFR and L_X",
+                      "grid_dim": N, "shells": F},
+           "kernels": kernels, "volume_ave_TS": g.volume_ave_TS, "volume_ave_TK": g.volume_ave_TK,
+           "volume_ave_xe": g.volume_ave_xe}
+    log("bench: ts " + json.dumps({k: v for k, v in out.items() if k != "kernels"}))
+    log("bench: ts kernels " + json.dumps(kernels))
+    ctx.close()
+    del d
+    torch.cuda.empty_cache()
+    if args.ts_cpu_N > 0:
+        out["cpu_baseline"] = ts_cpu_baseline(args, N, F)
+    return out
+
+
+def ts_cpu_baseline(args, N_full, F):
+    """The oracle's FP32 spin-temperature step on a smaller grid of the same kind, scaled to
+    cell-shell updates/s (its cost is linear in cells x shells up to the FFTs' log N)."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import mxr
+    import oracle_lib as O
+    import synth
+
+    N, box, snap = args.ts_cpu_N, args.ts_box * args.ts_cpu_N / N_full, 40
+    ZZ = mxr.snapshot_redshifts()
+    p = mxr.ts_params(N, box, ZZ, snap)
+    depth = mxr.ts_history_depth(p, len(ZZ) - 1)
+    d = {k: v.numpy() for k, v in synth.ts_inputs(N, box, args.seed, ZZ, snap, depth,
+                                                   stars_to_matter=args.ts_stars_to_matter).items()}
+    xe = np.zeros(mxr.padded_len(N), np.float32)
+    xe.reshape(N, N, -1)[:, :, :N] = 2e-4
+    Tk = np.full(N ** 3, 30.0, np.float32)
+    sf = np.zeros(len(ZZ))
+    t = mxr.ts_build_tables(p, 1e-3, 2e-4, sf)
+    t0 = time.perf_counter()
+    O.ts_step(p, t, d["deltax"], d["stars"], d["sfr_histories"], d["xray_luminosity_histories"], depth, xe, Tk)
+    secs = time.perf_counter() - t0
+    return {"value": N ** 3 * F / secs, "unit": "cell-shell updates/s", "cores": int(O.lib().oracle_num_threads()),
+            "kind": "port", "sample": f"one FP32 oracle step at {N}^3 ({box:.0f} cMpc/h), {F} shells: {secs:.1f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +176,11 @@ def main():
     ap.add_argument("--stars-to-matter", type=float, default=0.010438)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--cpu-radii", type=int, default=8, help="radii in the CPU-baseline sample (0: skip)")
+    ap.add_argument("--ts-N", type=int, default=256, help="spin-temperature grid (0: skip that leg)")
+    ap.add_argument("--ts-box", type=float, default=200.0)
+    ap.add_argument("--ts-steps", type=int, default=5)
+    ap.add_argument("--ts-stars-to-matter", type=float, default=0.020055)
+    ap.add_argument("--ts-cpu-N", type=int, default=64, help="grid of the spin-temperature CPU sample (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -166,6 +266,8 @@ def main():
         cpu = cpu_baseline(N, box, host, args.cpu_radii, n_radii)
         log("bench: cpu " + json.dumps(cpu))
 
+    ts = ts_leg(args, torch) if args.ts_N > 0 else None
+
     value = N ** 3 * n_radii / (ms_step / 1e3)
     out = {
         "metric": "cell-radius updates/s at 512^3 (find_HII_bubbles, sharp-k, batched/fused path)",
@@ -190,6 +292,7 @@ def main():
         "plain": plain,
         "cpu_baseline": cpu,
         "xH_global": fused_xh,
+        "spin_temperature": ts,
     }
     print(json.dumps(out), flush=True)
 

@@ -278,6 +278,9 @@ typedef struct mxr_ts_globals {
   double volume_ave_TS;
   double volume_ave_TK;
   double volume_ave_xe;
+  double XrayEmissivity_HMXB;     /* volume-averaged galaxy X-ray emissivity density of the
+                                     unfiltered shell (stored_XrayEmissivity_HMXB[snapshot]);
+                                     not written in the homogeneous branch               */
   double device_ms;
 } mxr_ts_globals;
 

@@ -22,6 +22,8 @@
 // Homogeneous branch (:252-331): x_e = xion_RECFAST, T_K = T_RECFAST (1 + cT delta), T_S.
 #include <cmath>
 #include <cstring>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "mxr_passes.hpp"
@@ -110,14 +112,14 @@ __global__ void __launch_bounds__(NT) k_row_c2r_ts(CGridPtrs in, long nrows, con
   }
 }
 
-// ---- R_ct = 0 reductions (ComputeTs.c:585-683) ------------------------------------------------
+// ---- R_ct = 0 averages the tables need (ComputeTs.c:629-677): capped collapse fraction of
+// the unfiltered stars and <x_e>, both from the inputs alone, so they run before any FFT
 struct TsRed0 {
   int N;
   const float* stars;   // padded
   const float* deltax;  // padded
   const float* xe;      // padded (x_e_box_prev before the update)
-  const float* xray0;   // real: smoothed X-ray field of shell 0
-  double K1, R03, pv, ULm3;
+  double K1, R03, pv;
   double* partials;
 };
 
@@ -127,7 +129,7 @@ __global__ void __launch_bounds__(NT) k_ts_red0(TsRed0 a)
 #pragma clang fp contract(off)
   const int N = a.N, H = N / 2 + 1;
   const long n = (long)N * N * N;
-  double acc[3] = {0.0, 0.0, 0.0};
+  double acc[2] = {0.0, 0.0};
   for (long ir = (long)blockIdx.x * NT + threadIdx.x; ir < n; ir += (long)gridDim.x * NT) {
     const long ip = (ir / N) * 2 * H + ir % N;
     const double dom = 1.0 + (double)a.deltax[ip];
@@ -140,9 +142,8 @@ __global__ void __launch_bounds__(NT) k_ts_red0(TsRed0 a)
       cf = 1.0;
     acc[0] += cf;
     acc[1] += (double)a.xe[ip];
-    acc[2] += (((double)a.xray0[ir] / a.pv) * XRAY_LUMINOSITY_UNIT) * a.ULm3;
   }
-  block_sum<NT, 3>(acc, a.partials + (long)blockIdx.x * 3);
+  block_sum<NT, 2>(acc, a.partials + (long)blockIdx.x * 2);
 }
 
 // ---- microphysics (XRayHeatingFunctions.c), FP64, in the reference's evaluation order ---------
@@ -168,27 +169,26 @@ __device__ __forceinline__ double spline_eval(const TsDev* __restrict__ T, int s
   return T->ky[s][ilo] + delx * (b + delx * (ci + delx * d));
 }
 
-// kappa_10 (:1728-1822): log-log spline, 1 K floor, T^0.381 above 10^4 K
-__device__ __noinline__ double kappa_10(const TsDev* __restrict__ T, double TK)
+// kappa_10 (:1728-1822): log-log spline, 1 K floor, T^0.381 above 10^4 K (that branch as one
+// exp: exp(log kap_26 + 0.381 (log T - log T_26)) = kap_26 (T / T_26)^0.381)
+__device__ __noinline__ double kappa_10(const TsDev* __restrict__ T, double lT)
 {
 #pragma clang fp contract(off)
   double ans;
-  const double lT = log(TK);
   if (lT < T->kx[0][0])
     ans = T->ky[0][0];
   else if (lT > T->kx[0][KAP_N - 1])
-    ans = log(T->k10_E1 * pow(TK / T->k10_E2, 0.381));
+    ans = T->ky[0][KAP_N - 1] + 0.381 * (lT - T->kx[0][KAP_N - 1]);
   else
     ans = spline_eval(T, 0, lT);
   return exp(ans);
 }
 
 // kappa_10_elec / kappa_10_pH (:1827-1966): log-log spline, floor below, linear extrapolation above
-__device__ __noinline__ double kappa_tab(const TsDev* __restrict__ T, int s, double Tk)
+__device__ __noinline__ double kappa_tab(const TsDev* __restrict__ T, int s, double lT)
 {
 #pragma clang fp contract(off)
   const int n = T->kn[s];
-  const double lT = log(Tk);
   double ans;
   if (lT < T->kx[s][0])
     ans = T->ky[s][0];
@@ -201,61 +201,66 @@ __device__ __noinline__ double kappa_tab(const TsDev* __restrict__ T, int s, dou
 }
 
 // get_Ts (:1639-1684) with xcoll (:1686-1726), xalpha_tilde, taugp, Salpha_tilde, Tc_eff
-// (:1973-2008); z is the (float) redshift of the call, so Trad and (1+z)^3 come from the block
+// (:1973-2008).  z is the (float) redshift of the call, so Trad and (1+z)^3 come from the block.
+// The fixed point is iterated on 1/T_S: with u = 1/T_S the reference's update
+//   T_S' = (1 + xa + xc) / (1/Trad + xa / Tc_eff + xc / T_K),  1/Tc_eff = 1/T_K + 0.405535/T_K (u - 1/T_K)
+// and S_alpha = (A + B u) / D need no division but the one per step, and its stopping test
+// |T_S' - T_S| / T_S' > 1e-3 reads |u - u'| > 1e-3 u.
 __device__ __noinline__ float get_Ts(const TsDev* __restrict__ T, float delta, float TKf, float xef, float Jalpha,
                                      float* curr_xalpha)
 {
-#pragma clang fp contract(off)
   const double TK = TKf, xe = xef, dl = delta;
+  const double iTK = 1.0 / TK;
   const double Trad = T->Trad;
   const double opz3 = T->opz3;
-  const double nH = (((1.0 - xe) * T->No) * opz3) * (1.0 + dl);
-  const double x_HI = (((T21 / Trad) * nH) * kappa_10(T, TK)) / A10_HYPERFINE;
-  const double ne = ((xe * T->N_b0) * opz3) * (1.0 + dl);
-  const double x_e = (((T21 / Trad) * ne) * kappa_tab(T, 1, TK)) / A10_HYPERFINE;
-  const double np = ((xe * T->No) * opz3) * (1.0 + dl);
-  const double x_p = (((T21 / Trad) * np) * kappa_tab(T, 2, TK)) / A10_HYPERFINE;
+  const double lT = log(TK);
+  const double c21 = T21 / Trad / A10_HYPERFINE * opz3 * (1.0 + dl);
+  const double x_HI = c21 * ((1.0 - xe) * T->No) * kappa_10(T, lT);
+  const double x_e = c21 * (xe * T->N_b0) * kappa_tab(T, 1, lT);
+  const double x_p = c21 * (xe * T->No) * kappa_tab(T, 2, lT);
   const double xc = (x_HI + x_e) + x_p;
-  double TS, TSinv;
+  double u;
   if (Jalpha > 1.0e-20) {
-    const double Ja = Jalpha;
-    const double tgp = ((((1.342881e-7 / T->hub_z) * T->No) * opz3) * (1.0 + dl)) * (1.0 - xe);
-    const double xi = pow(((1.0e-7 * tgp) / TK) / TK, 1.0 / 3.0);
-    const double den = ((1.0 + 2.98394 * xi) + (1.53583 * xi) * xi) + ((3.85289 * xi) * xi) * xi;
-    const double opz = 1.0 + (double)T->zpf;
-    double xa = 0.0, TSold = 0.0;
-    TS = Trad;
-    // the reference iterates until converged; the cap only ends a cell that never converges
-    for (int it = 0; it < TS_MAX_ITER && fabs(TS - TSold) / TS > 1.0e-3; it++) {
-      TSold = TS;
-      double S = ((1.0 - 0.0631789 / TK) + (0.115995 / TK) / TK) - (0.401403 / TS) / TK;
-      S += ((0.336463 / TS) / TK) / TK;
-      S /= den;
-      xa = ((1.66e11 / opz) * S) * Ja;
-      double Tce = 1.0 / TK + (0.405535 / TK) * (1.0 / TS - 1.0 / TK);
-      Tce = 1.0 / Tce;
-      TSinv = ((1.0 / Trad + xa / Tce) + xc / TK) / ((1.0 + xa) + xc);
-      TS = 1.0 / TSinv;
+    const double tgp = T->c_tgp * (1.0 + dl) * (1.0 - xe);
+    const double xi = cbrt(1.0e-7 * tgp * iTK * iTK);
+    const double iden = 1.0 / (((1.0 + 2.98394 * xi) + (1.53583 * xi) * xi) + ((3.85289 * xi) * xi) * xi);
+    const double A = ((1.0 - 0.0631789 * iTK) + 0.115995 * iTK * iTK) * iden;
+    const double B = (0.336463 * iTK * iTK - 0.401403 * iTK) * iden;
+    const double ka = T->c_xa * (double)Jalpha;
+    const double base = T->iTrad + xc * iTK;
+    double xa = 0.0;
+    u = T->iTrad;
+    for (int it = 0; it < TS_MAX_ITER; it++) {
+      xa = ka * (A + B * u);
+      const double iTce = iTK + 0.405535 * iTK * (u - iTK);
+      const double un = (base + xa * iTce) / ((1.0 + xa) + xc);
+      const bool more = fabs(u - un) > 1.0e-3 * u;
+      u = un;
+      if (!more)
+        break;
     }
     *curr_xalpha = (float)xa;
   } else {
-    TSinv = (1.0 / Trad + xc / TK) / (1.0 + xc);
-    TS = 1.0 / TSinv;
+    u = (T->iTrad + xc * iTK) / (1.0 + xc);
     *curr_xalpha = 0.0f;
   }
+  double TS = 1.0 / u;
   if (TS < 0.)
     TS = fabs(TS);
   return (float)TS;
 }
 
-// alpha_A (:1576-1584): case-A recombination coefficient, Abel et al. 1997
+// alpha_A (:1576-1584): case-A recombination coefficient, Abel et al. 1997 (the powers of log T
+// as products)
 __device__ __forceinline__ double alpha_A(double Tk)
 {
 #pragma clang fp contract(off)
   const double L = log(Tk / (double)1.1604505e4);
-  const double e = -28.6130338 - 0.72411256 * L - 2.02604473e-2 * pow(L, 2) - 2.38086188e-3 * pow(L, 3) -
-                   3.21260521e-4 * pow(L, 4) - 1.42150291e-5 * pow(L, 5) + 4.98910892e-6 * pow(L, 6) +
-                   5.75561414e-7 * pow(L, 7) - 1.85676704e-8 * pow(L, 8) - 3.07113524e-9 * pow(L, 9);
+  const double L2 = L * L, L3 = L2 * L, L4 = L3 * L, L5 = L4 * L, L6 = L5 * L, L7 = L6 * L, L8 = L7 * L,
+               L9 = L8 * L;
+  const double e = -28.6130338 - 0.72411256 * L - 2.02604473e-2 * L2 - 2.38086188e-3 * L3 - 3.21260521e-4 * L4 -
+                   1.42150291e-5 * L5 + 4.98910892e-6 * L6 + 5.75561414e-7 * L7 - 1.85676704e-8 * L8 -
+                   3.07113524e-9 * L9;
   return exp(e);
 }
 
@@ -270,15 +275,29 @@ struct TsEvolve {
   double* partials;
 };
 
+// The shell sums run in FP64 over the folded per-shell factors staged in LDS (6 x 14 x F for
+// the heat / ionisation / Lyman-alpha rates, F for the stellar Lyman-alpha flux): per cell
+// and shell two coalesced FP32 loads and seven FP64 FMAs.  The per-cell tail (Euler step,
+// floors, T_S) follows the reference's evaluation order.
 template <int NT>
 __global__ void __launch_bounds__(NT) k_ts_evolve(const TsDev* __restrict__ T, TsEvolve a)
 {
 #pragma clang fp contract(off)
+  extern __shared__ double fl[];  // [6][XNH][F] then bs[F]
   const int N = a.N, H = N / 2 + 1, F = T->F;
   const long n = (long)N * N * N;
+  for (int i = threadIdx.x; i < 6 * XNH * F; i += NT) {
+    const int k = i / (XNH * F), m = (i / F) % XNH, r = i % F;
+    fl[i] = T->fold[k][m][r];
+  }
+  for (int r = threadIdx.x; r < F; r += NT)
+    fl[6 * XNH * F + r] = T->bs[r];
+  __syncthreads();
+  const double* __restrict__ bsl = fl + 6 * XNH * F;
   const float* __restrict__ sm_sfr = a.sm;
   const float* __restrict__ sm_xr = a.sm + (long)F * n;
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  const int no_light = T->no_light;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (long ir = (long)blockIdx.x * NT + threadIdx.x; ir < n; ir += (long)gridDim.x * NT) {
     const long ip = (ir / N) * 2 * H + ir % N;
     const float dx = a.deltax[ip];
@@ -293,47 +312,50 @@ __global__ void __launch_bounds__(NT) k_ts_evolve(const TsDev* __restrict__ T, T
       xcall = 1.001 * (double)T->xint[0];
     const float xcf = (float)xcall;
     int m = XNH - 1;
-    while (xcf < T->xint[m])
+    while (m > 0 && xcf < T->xint[m])
       m--;
     const double dxl = xcall - (double)T->xint[m];
-    double dheat = 0, dion = 0, dlya = 0, dstar = 0;
-    if (!T->no_light) {
+    acc[7] += (double)sm_xr[ir] * T->cx;  // <L_X> of the unfiltered shell (:603, 677-682)
+    double hs = 0, hb = 0, is = 0, ib = 0, ls = 0, lb = 0, st = 0;
+    if (!no_light) {
+      const double* __restrict__ f0 = fl + (0 * XNH + m) * F;
+      const double* __restrict__ f1 = fl + (1 * XNH + m) * F;
+      const double* __restrict__ f2 = fl + (2 * XNH + m) * F;
+      const double* __restrict__ f3 = fl + (3 * XNH + m) * F;
+      const double* __restrict__ f4 = fl + (4 * XNH + m) * F;
+      const double* __restrict__ f5 = fl + (5 * XNH + m) * F;
       for (int r = 0; r < F; r++) {
-        const float sf = sm_sfr[(long)r * n + ir];
-        const float xr = sm_xr[(long)r * n + ir];
-        const double SFR = ((((double)sf / T->pv) * T->UMUT) * T->ULm3) / SOLAR_MASS;
-        const double XR = (((double)xr / T->pv) * XRAY_LUMINOSITY_UNIT) * T->ULm3;
-        const double fh = T->slope[0][m][r] * dxl + T->base[0][m][r];
-        const double fi = T->slope[1][m][r] * dxl + T->base[1][m][r];
-        const double fl = T->slope[2][m][r] * dxl + T->base[2][m][r];
-        const double tt = T->dtd[r] * (XR * T->pw[r]);
-        dheat += tt * fh;
-        dion += tt * fi;
-        dlya += tt * fl;
-        dstar += ((((SFR * T->P2) * T->opz[r]) * T->lyn[r]) * T->dtz[r]) * T->dzz[r];
+        const double xr = sm_xr[(long)r * n + ir];
+        const double sf = sm_sfr[(long)r * n + ir];
+        hs = fma(xr, f0[r], hs);
+        hb = fma(xr, f1[r], hb);
+        is = fma(xr, f2[r], is);
+        ib = fma(xr, f3[r], ib);
+        ls = fma(xr, f4[r], ls);
+        lb = fma(xr, f5[r], lb);
+        st = fma(sf, bsl[r], st);
       }
     }
     const double x_e = xe0, Td = tk;
     const double zpd = (double)T->zpf;
     const double n_b = (T->N_b0 * T->nb3) * (double)__fadd_rn(1.0f, dx);
-    dheat *= T->prefac_gal;
-    dion *= T->prefac_gal;
-    dlya *= T->prefac_gal * n_b;
-    dstar *= T->conv;
-    const double sink = ((((alpha_A(Td) * CLUMPING_FACTOR) * x_e) * x_e) * T->f_H) * n_b;
-    const double dxe_dzp = T->dt_dzp * (((dion + 0.0) + 0.0) - sink);
-    double dadia = 3 / (1.0 + zpd);
+    const double dheat = (hs * dxl + hb) * T->prefac_gal;
+    const double dion = (is * dxl + ib) * T->prefac_gal;
+    const double dlya = (ls * dxl + lb) * (T->prefac_gal * n_b);
+    const double dstar = st * T->conv;
+    const double sink = alpha_A(Td) * CLUMPING_FACTOR * x_e * x_e * T->f_H * n_b;
+    const double dxe_dzp = T->dt_dzp * (dion - sink);
+    const double i1x = 1.0 / (1.0 + x_e);
+    double dadia = T->c_adia;
     if (fabs((double)dx) > FRACT_FLOAT_ERR)
-      dadia += T->dgrowth / (1.0 / (double)dx + T->growth);
+      dadia += T->dgrowth * (double)dx / (1.0 + T->growth * (double)dx);
     dadia *= (2.0 / 3.0) * Td;
-    const double dspec = ((-dxe_dzp) * Td) / (1 + x_e);
-    double dcomp = ((((-1.51e-4) * (x_e / ((1.0 + x_e) + T->f_He))) / T->hub_ratio) / T->h) * T->Trad4;
-    dcomp = dcomp / (1.0 + zpd);
-    dcomp *= T->Trad - Td;
-    const double dxheat_dzp = ((((dheat * T->dt_dzp) * 2.0) / 3.0) / BOLTZMANN) / (1.0 + x_e);
+    const double dspec = -dxe_dzp * Td * i1x;
+    const double dcomp = T->c_comp * (x_e / ((1.0 + x_e) + T->f_He)) * (T->Trad - Td);
+    const double dxheat_dzp = dheat * T->c_heat * i1x;
     const double d1 = ((dxheat_dzp + dcomp) + dspec) + dadia;
-    const double d2 = ((dlya + 0.0) + 0.0) + dstar;
-    const double d4 = T->dt_dzp * ((dion + 0.0) + 0.0);
+    const double d2 = dlya + dstar;
+    const double d4 = T->dt_dzp * dion;
     // Euler step and floors (:1345-1377)
     float xe1 = (float)((double)xe0 + dxe_dzp * T->dzp);
     if (xe1 > 1)
@@ -357,7 +379,7 @@ __global__ void __launch_bounds__(NT) k_ts_evolve(const TsDev* __restrict__ T, T
     acc[5] += (double)tk;
     acc[6] += (double)xe1;
   }
-  block_sum<NT, 7>(acc, a.partials + (long)blockIdx.x * 7);
+  block_sum<NT, 8>(acc, a.partials + (long)blockIdx.x * 8);
 }
 
 // homogeneous branch (ComputeTs.c:252-287) with the final averages
@@ -515,107 +537,127 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
       tables->hist_max = -1;
     }
   } else {
-    FiltArgs fa{};
-    fa.N = N;
+    // R_ct = 0 averages first: the host builds the frequency tables on its own thread while
+    // the GPU works through every shell
     {
-      const float bs = (float)p->BoxSize;  // filter() uses run_globals.params.BoxSize (Mpc/h)
-      fa.dk = (float)(2.0 * M_PI / bs);
-    }
-    fa.ftype = p->TsHeatingFilterType;
-    fa.prune = p->TsHeatingFilterType == 1 ? 1 : 0;
-    CRealPtrs rin{};
-    GridPtrs spec{}, work{};
-    CGridPtrs cspec{}, cwork{};
-    for (int i = 0; i < 2; i++) {
-      spec.p[i] = c->spec[i];
-      work.p[i] = c->work[i];
-      cspec.p[i] = c->spec[i];
-      cwork.p[i] = c->work[i];
-    }
-    FiltArgs none{};
-    none.N = N;
-    const double cplx = 8.0 * (double)Nc;
-    bool src_valid = false;
-    for (int r = 0; r < F; r++) {
-      if (r == 0 || pl.nops[r] > 0) {
-        if (pl.nops[r] > 0) {
-          TsOpsArgs oa{};
-          oa.nops = pl.nops[r];
-          oa.init = src_valid ? 0 : 1;
-          int nread = oa.init ? 1 : 1;
-          for (int k = 0; k < pl.nops[r]; k++) {
-            oa.ops[k] = pl.ops[r][k];
-            nread += (pl.ops[r][k].kind >= TS_OP_LOAD) ? 1 : 0;
-          }
-          RC(T(K_TS_SRC, 2.0 * 4.0 * npad * (nread + 1), [&] {
-            hipLaunchKernelGGL(k_ts_source, dim3(ew_blocks(npad)), dim3(256), 0, c->stream, oa, hs, hx, npad,
-                               c->ts_src[0], c->ts_src[1], npad);
-            MXR_HIP(hipGetLastError());
-            return MXR_OK;
-          }));
-          src_valid = true;
-        }
-        rin.p[0] = src_valid ? c->ts_src[0] : hs;
-        rin.p[1] = src_valid ? c->ts_src[1] : hx;
-        RC(T(K_TS_FWD, 2.0 * 3 * 2 * cplx, [&] {
-          RC(K::r2c(c, rin, spec, 2));
-          RC((K::template col<-1, COL_PLAIN>(c, cspec, spec, 2, K::ygeo(), none, 1.0)));
-          return K::template col<-1, COL_FSCALE>(c, cspec, spec, 2, K::xgeo(), none, total_n_cells);
-        }));
-      }
-      fa.R = (float)pl.R[r];
-      fa.apply = r > 0 ? 1 : 0;
-      const KeptModes km = (fa.apply && fa.prune) ? kept_modes(N, fa.dk, fa.R) : KeptModes::all(N);
-      TsStore st{{c->ts_store + (size_t)r * nreal, c->ts_store + ((size_t)F + r) * nreal}};
-      RC(T(K_TS_INV_X, 2 * 8.0 * ((double)km.k3 + (double)N * km.c2),
-           [&] { return K::template col<+1, COL_FILTER>(c, cspec, work, 2, K::xgeo(), fa, 1.0); }));
-      RC(T(K_TS_INV_Y, 2 * 8.0 * ((double)N * km.c2 + (double)N * N * km.k1),
-           [&] { return K::template col<+1, COL_YPRUNE>(c, cwork, work, 2, K::ygeo(), fa, 1.0); }));
-      RC(T(K_TS_INV_Z, 2 * (8.0 * (double)N * N * km.k1 + 4.0 * nreal), [&] {
-        const long nrows = (long)N * N;
-        dim3 grid((unsigned)((nrows + Cfg<N>::TR - 1) / Cfg<N>::TR), 2u);
-        hipLaunchKernelGGL((k_row_c2r_ts<N, Cfg<N>::ROW_NT, Cfg<N>::TR>), grid, dim3(Cfg<N>::ROW_NT), 0, c->stream,
-                           cwork, nrows, c->tw, fa, st);
+      TsRed0 ra{};
+      ra.N = N;
+      ra.stars = stars;
+      ra.deltax = dx;
+      ra.xe = xe;
+      const double h = p->Hubble_h;
+      const double box_size = p->BoxSize / h;
+      ra.K1 = (ts_rtom(p, pl.R[0]) * h) * h;
+      ra.R03 = std::pow(pl.R[0], 3.0);
+      ra.pv = std::pow(box_size / (double)N, 3);
+      ra.partials = c->partials;
+      RC(T(K_TS_RED, (double)nreal * 12, [&] {
+        hipLaunchKernelGGL((k_ts_red0<NT>), dim3(nblk), dim3(NT), 0, c->stream, ra);
         MXR_HIP(hipGetLastError());
-        return MXR_OK;
+        return launch_final_sum(c, c->partials, nblk, 2, c->sums);
       }));
-      if (r == 0) {
-        TsRed0 ra{};
-        ra.N = N;
-        ra.stars = stars;
-        ra.deltax = dx;
-        ra.xe = xe;
-        ra.xray0 = c->ts_store + (size_t)F * nreal;
-        const double h = p->Hubble_h;
-        const double box_size = p->BoxSize / h;
-        ra.K1 = (ts_rtom(p, pl.R[0]) * h) * h;
-        ra.R03 = std::pow(pl.R[0], 3.0);
-        ra.pv = std::pow(box_size / (double)N, 3);
-        ra.ULm3 = std::pow(p->UnitLength_in_cm, -3.);
-        ra.partials = c->partials;
-        RC(T(K_TS_RED, (double)nreal * 16, [&] {
-          hipLaunchKernelGGL((k_ts_red0<NT>), dim3(nblk), dim3(NT), 0, c->stream, ra);
-          MXR_HIP(hipGetLastError());
-          return launch_final_sum(c, c->partials, nblk, 3, c->sums);
-        }));
-        MXR_HIP(hipMemcpyAsync(sums, c->sums, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
-        MXR_HIP(hipEventRecord(c->ev_red, c->stream));
-      }
+      MXR_HIP(hipMemcpyAsync(sums, c->sums, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
+      MXR_HIP(hipEventRecord(c->ev_red, c->stream));
     }
-    // the host tables overlap the remaining shells on the GPU
-    MXR_HIP(hipEventSynchronize(c->ev_red));
-    const double collapse_fraction = sums[0] / total_n_cells;
-    const double x_e_ave = sums[1] / total_n_cells;
-    g->stored_fcoll[p->snapshot] = collapse_fraction;
-    RC(ts_tables(p, pl, collapse_fraction, x_e_ave, g->stored_fcoll, hd, tables));
+    int trc = MXR_OK;
+    std::string terr;
+    std::thread tables_thread([&] {
+      if (hipSetDevice(c->device) != hipSuccess || hipEventSynchronize(c->ev_red) != hipSuccess) {
+        trc = MXR_ERR_HIP;
+        terr = "waiting for the R_ct = 0 averages failed";
+        return;
+      }
+      const double collapse_fraction = sums[0] / total_n_cells;
+      const double x_e_ave = sums[1] / total_n_cells;
+      g->stored_fcoll[p->snapshot] = collapse_fraction;
+      trc = ts_tables(p, pl, collapse_fraction, x_e_ave, g->stored_fcoll, hd, tables);
+      if (trc)
+        terr = mxr_last_error();
+    });
+    auto shells = [&]() -> int {
+      FiltArgs fa{};
+      fa.N = N;
+      {
+        const float bs = (float)p->BoxSize;  // filter() uses run_globals.params.BoxSize (Mpc/h)
+        fa.dk = (float)(2.0 * M_PI / bs);
+      }
+      fa.ftype = p->TsHeatingFilterType;
+      fa.prune = p->TsHeatingFilterType == 1 ? 1 : 0;
+      CRealPtrs rin{};
+      GridPtrs spec{}, work{};
+      CGridPtrs cspec{}, cwork{};
+      for (int i = 0; i < 2; i++) {
+        spec.p[i] = c->spec[i];
+        work.p[i] = c->work[i];
+        cspec.p[i] = c->spec[i];
+        cwork.p[i] = c->work[i];
+      }
+      FiltArgs none{};
+      none.N = N;
+      const double cplx = 8.0 * (double)Nc;
+      bool src_valid = false;
+      for (int r = 0; r < F; r++) {
+        if (r == 0 || pl.nops[r] > 0) {
+          if (pl.nops[r] > 0) {
+            TsOpsArgs oa{};
+            oa.nops = pl.nops[r];
+            oa.init = src_valid ? 0 : 1;
+            int nread = 1;
+            for (int k = 0; k < pl.nops[r]; k++) {
+              oa.ops[k] = pl.ops[r][k];
+              nread += (pl.ops[r][k].kind >= TS_OP_LOAD) ? 1 : 0;
+            }
+            RC(T(K_TS_SRC, 2.0 * 4.0 * npad * (nread + 1), [&] {
+              hipLaunchKernelGGL(k_ts_source, dim3(ew_blocks(npad)), dim3(256), 0, c->stream, oa, hs, hx, npad,
+                                 c->ts_src[0], c->ts_src[1], npad);
+              MXR_HIP(hipGetLastError());
+              return MXR_OK;
+            }));
+            src_valid = true;
+          }
+          rin.p[0] = src_valid ? c->ts_src[0] : hs;
+          rin.p[1] = src_valid ? c->ts_src[1] : hx;
+          RC(T(K_TS_FWD, 2.0 * 3 * 2 * cplx, [&] {
+            RC(K::r2c(c, rin, spec, 2));
+            RC((K::template col<-1, COL_PLAIN>(c, cspec, spec, 2, K::ygeo(), none, 1.0)));
+            return K::template col<-1, COL_FSCALE>(c, cspec, spec, 2, K::xgeo(), none, total_n_cells);
+          }));
+        }
+        fa.R = (float)pl.R[r];
+        fa.apply = r > 0 ? 1 : 0;
+        const KeptModes km = (fa.apply && fa.prune) ? kept_modes(N, fa.dk, fa.R) : KeptModes::all(N);
+        TsStore st{{c->ts_store + (size_t)r * nreal, c->ts_store + ((size_t)F + r) * nreal}};
+        RC(T(K_TS_INV_X, 2 * 8.0 * ((double)km.k3 + (double)N * km.c2),
+             [&] { return K::template col<+1, COL_FILTER>(c, cspec, work, 2, K::xgeo(), fa, 1.0); }));
+        RC(T(K_TS_INV_Y, 2 * 8.0 * ((double)N * km.c2 + (double)N * N * km.k1),
+             [&] { return K::template col<+1, COL_YPRUNE>(c, cwork, work, 2, K::ygeo(), fa, 1.0); }));
+        RC(T(K_TS_INV_Z, 2 * (8.0 * (double)N * N * km.k1 + 4.0 * nreal), [&] {
+          const long nrows = (long)N * N;
+          dim3 grid((unsigned)((nrows + Cfg<N>::TR - 1) / Cfg<N>::TR), 2u);
+          hipLaunchKernelGGL((k_row_c2r_ts<N, Cfg<N>::ROW_NT, Cfg<N>::TR>), grid, dim3(Cfg<N>::ROW_NT), 0,
+                             c->stream, cwork, nrows, c->tw, fa, st);
+          MXR_HIP(hipGetLastError());
+          return MXR_OK;
+        }));
+      }
+      return MXR_OK;
+    };
+    const int src = shells();
+    tables_thread.join();
+    RC(src);
+    if (trc) {
+      set_error("%s", terr.c_str());
+      return trc;
+    }
     MXR_HIP(hipMemcpyAsync(c->ts_dev, hd, sizeof(TsDev), hipMemcpyHostToDevice, c->stream));
     TsEvolve ea{N, dx, xe, Tk, TS, c->ts_store, c->partials};
-    RC(T(K_TS_EVOLVE, (double)nreal * (4 + 8 + 8 + 4 + (hd->no_light ? 0 : 8.0 * F)), [&] {
-      hipLaunchKernelGGL((k_ts_evolve<NT>), dim3(nblk), dim3(NT), 0, c->stream, dd, ea);
+    const size_t lds = sizeof(double) * (6 * XNH + 1) * (size_t)F;
+    RC(T(K_TS_EVOLVE, (double)nreal * (4 + 8 + 8 + 4 + (hd->no_light ? 4 : 8.0 * F)), [&] {
+      hipLaunchKernelGGL((k_ts_evolve<NT>), dim3(nblk), dim3(NT), lds, c->stream, dd, ea);
       MXR_HIP(hipGetLastError());
-      return launch_final_sum(c, c->partials, nblk, 7, c->sums);
+      return launch_final_sum(c, c->partials, nblk, 8, c->sums);
     }));
-    MXR_HIP(hipMemcpyAsync(sums, c->sums, sizeof(double) * 7, hipMemcpyDeviceToHost, c->stream));
+    MXR_HIP(hipMemcpyAsync(sums, c->sums, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream));
   }
   // x_e_box = x_e_box_prev (ComputeTs.c:1459), the whole padded array
   MXR_HIP(hipMemcpyAsync(xe_out, xe, sizeof(float) * npad, hipMemcpyDeviceToDevice, c->stream));
@@ -642,6 +684,7 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
     out->volume_ave_TS = sums[4] / total_n_cells;
     out->volume_ave_TK = sums[5] / total_n_cells;
     out->volume_ave_xe = sums[6] / total_n_cells;
+    out->XrayEmissivity_HMXB = sums[7] / total_n_cells;
   }
   out->device_ms = ms;
   return MXR_OK;

@@ -83,9 +83,16 @@ struct TsDev {
   double P2, nb3;                  // pow(1 + zp_f, 2), pow(1 + zp_f, 3) with a float 1 + zp
   double Trad, Trad4, opz3, hub_z, hub_ratio;  // at z = (double)zp_f
   double K1, R03;                  // R_ct = 0 collapse fraction: RtoM(R) h h, R^3
-  double pw[TS_MAXF], dtd[TS_MAXF], opz[TS_MAXF], lyn[TS_MAXF], dtz[TS_MAXF], dzz[TS_MAXF];
-  double base[3][XNH][TS_MAXF];    // heat, ion, lya: table value at the lower x_e bin
-  double slope[3][XNH][TS_MAXF];   // (tbl[m+1] - tbl[m]) / (x[m+1] - x[m])
+  // per-cell tail constants: 1/Trad, 1.342881e-7 No (1+z)^3 / H(z), 1.66e11 / (1+z),
+  // -1.51e-4 Trad^4 / (H(z)/H0 h (1+z)), 3 / (1+z), dt/dz 2 / (3 k_B)
+  double iTrad, c_tgp, c_xa, c_comp, c_adia, c_heat;
+  // shell sums with the unit constants folded in: for k = heat, ion, lya the rate at x_e in
+  // bin m is sum_r L_X,r (fold[2k][m][r] dx + fold[2k+1][m][r]), dx = x_e - x[m], with
+  // L_X,r the smoothed X-ray field (1e40 erg/s per cell); the stellar Lyman-alpha sum is
+  // sum_r SFR_r bs[r] (SFR in internal units per cell)
+  double fold[6][XNH][TS_MAXF];
+  double bs[TS_MAXF];
+  double cx;                       // 1e40 UnitLength^-3 / V_pix: X-ray field -> emissivity density
   float xint[XNH];
   // kappa splines in log-log (x, y, c = S''/2): kappa_10 (27), e-H (20), p-H (17)
   int kn[3];

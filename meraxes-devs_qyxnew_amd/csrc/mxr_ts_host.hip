@@ -382,6 +382,7 @@ static int locate_xHII_index(const float* xint, float x)
 
 static float interp_tab(const TsData* d, const float (*tab)[XNE], float below, float En, float xHII)
 {
+#pragma clang fp contract(off)
   const float* E = d->E;
   const float* X = d->x_int_XHII;
   if (En > 0.999 * E[XNE - 1])
@@ -394,15 +395,15 @@ static float interp_tab(const TsData* d, const float (*tab)[XNE], float below, f
     xHII = (float)(1.001 * X[0]);
   const int nl = locate_energy_index(En), nh = nl + 1;
   const int ml = locate_xHII_index(X, xHII), mh = ml + 1;
-  volatile float lo = (tab[ml][nh] - tab[ml][nl]) / (E[nh] - E[nl]);
-  lo = lo * (En - E[nl]);
-  lo = lo + tab[ml][nl];
-  volatile float hi = (tab[mh][nh] - tab[mh][nl]) / (E[nh] - E[nl]);
-  hi = hi * (En - E[nl]);
-  hi = hi + tab[mh][nl];
-  volatile float fr = (hi - lo) / (X[mh] - X[ml]);
-  fr = fr * (xHII - X[ml]);
-  fr = fr + lo;
+  float lo = (tab[ml][nh] - tab[ml][nl]) / (E[nh] - E[nl]);
+  lo *= (En - E[nl]);
+  lo += tab[ml][nl];
+  float hi = (tab[mh][nh] - tab[mh][nl]) / (E[nh] - E[nl]);
+  hi *= (En - E[nl]);
+  hi += tab[mh][nl];
+  float fr = (hi - lo) / (X[mh] - X[ml]);
+  fr *= (xHII - X[ml]);
+  fr += lo;
   return fr;
 }
 
@@ -1000,10 +1001,10 @@ static double integrate_over_nu(const TsData* d, const TsCosmo& cz, double zp, d
   };
   auto Fi = [&](double nu, double thr) -> float {
     const float En = (float)((nu - thr) / NU_over_EV);
-    volatile float a = interp_tab(d, d->nHI, 0.0f, En, xf);
-    a = a + interp_tab(d, d->nHeI, 0.0f, En, xf);
-    a = a + interp_tab(d, d->nHeII, 0.0f, En, xf);
-    a = a + 1;
+    float a = interp_tab(d, d->nHI, 0.0f, En, xf);
+    a += interp_tab(d, d->nHeI, 0.0f, En, xf);
+    a += interp_tab(d, d->nHeII, 0.0f, En, xf);
+    a += 1;
     return a;
   };
   auto ion = [&](double nu) {
@@ -1063,6 +1064,11 @@ static void fill_common(const mxr_ts_params* p, const TsCosmo& cz, const TsData*
   dev->hub_z = cz.hubble(dev->zpf);
   dev->hub_ratio = cz.hubble(dev->zpf) / (HUBBLE * p->Hubble_h);
   dev->conv = (SPEED_OF_LIGHT / (4. * M_PI)) / (PROTONMASS / SOLAR_MASS);
+  dev->iTrad = 1.0 / dev->Trad;
+  dev->c_tgp = 1.342881e-7 / dev->hub_z * dev->No * dev->opz3;
+  dev->c_xa = 1.66e11 / (1.0 + zd);
+  dev->c_comp = (-1.51e-4) / dev->hub_ratio / p->Hubble_h * dev->Trad4 / (1.0 + zd);
+  dev->c_adia = 3 / (1.0 + zd);
   const int kn[3] = {KAP_N, 20, 17};
   const CSpline* sp[3] = {&d->kap10, &d->kapeH, &d->kappH};
   for (int s = 0; s < 3; s++) {
@@ -1126,38 +1132,56 @@ int ts_tables(const mxr_ts_params* p, const TsPlan& pl, double collapse_fraction
   TauCtx tc{&cz, p, stored_fcoll, x_e_ave};
   std::vector<double> ntau(F), heat((size_t)XNH * F), ion((size_t)XNH * F), lya((size_t)XNH * F);
   const double lo_thr = p->NuXrayThreshold * NU_over_EV, hi_lim = p->NuXrayMax * NU_over_EV;
-  std::atomic<int> next{0};
-  auto worker = [&]() {
-    for (;;) {
-      const int r = next.fetch_add(1);
-      if (r >= F)
-        return;
-      const double prev_zpp = r == 0 ? zp : pl.zpp_edge[r - 1];
-      const double zpp = (pl.zpp_edge[r] + prev_zpp) * 0.5;
-      ntau[r] = nu_tau_one(tc, zp, zpp, x_e_ave, p->snapshot);
-      const double lower = std::fmax(ntau[r], lo_thr);
-      for (int m = 0; m < XNH; m++) {
-        const double xm = d->x_int_XHII[m];
-        heat[(size_t)m * F + r] = integrate_over_nu(d, cz, zp, xm, lower, hi_lim, p->NuXrayThreshold,
-                                                    p->SpecIndexXrayGal, 0);
-        ion[(size_t)m * F + r] = integrate_over_nu(d, cz, zp, xm, lower, hi_lim, p->NuXrayThreshold,
-                                                   p->SpecIndexXrayGal, 1);
-        lya[(size_t)m * F + r] = integrate_over_nu(d, cz, zp, xm, lower, hi_lim, p->NuXrayThreshold,
-                                                   p->SpecIndexXrayGal, 2);
-      }
-    }
-  };
-  {
-    unsigned nth = std::thread::hardware_concurrency();
-    nth = std::max(1u, std::min(nth, 16u));
-    nth = std::min<unsigned>(nth, (unsigned)F);
+  // phase 1: nu_tau_one per shell; phase 2: the integrals per distinct lower limit and x_e bin
+  // (shells whose tau = 1 frequency lies below the threshold share one set of integrals --
+  // the same function of the same arguments, so computing it once changes nothing)
+  auto parallel = [](int ntask, auto&& fn) {
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+      for (int k = next.fetch_add(1); k < ntask; k = next.fetch_add(1))
+        fn(k);
+    };
+    unsigned nth = std::max(1u, std::min(std::thread::hardware_concurrency(), 16u));
+    nth = std::min<unsigned>(nth, (unsigned)std::max(ntask, 1));
     std::vector<std::thread> pool;
     for (unsigned t = 1; t < nth; t++)
       pool.emplace_back(worker);
     worker();
     for (auto& th : pool)
       th.join();
+  };
+  parallel(F, [&](int r) {
+    const double prev_zpp = r == 0 ? zp : pl.zpp_edge[r - 1];
+    const double zpp = (pl.zpp_edge[r] + prev_zpp) * 0.5;
+    ntau[r] = nu_tau_one(tc, zp, zpp, x_e_ave, p->snapshot);
+  });
+  std::vector<double> lowers;
+  std::vector<int> which(F);
+  for (int r = 0; r < F; r++) {
+    const double lower = std::fmax(ntau[r], lo_thr);
+    int u = 0;
+    while (u < (int)lowers.size() && lowers[u] != lower)
+      u++;
+    if (u == (int)lowers.size())
+      lowers.push_back(lower);
+    which[r] = u;
   }
+  const int nu_ = (int)lowers.size();
+  std::vector<double> uh((size_t)nu_ * XNH), ui((size_t)nu_ * XNH), ul((size_t)nu_ * XNH);
+  parallel(nu_ * XNH, [&](int k) {
+    const int u = k / XNH, m = k % XNH;
+    const double xm = d->x_int_XHII[m];
+    uh[k] = integrate_over_nu(d, cz, zp, xm, lowers[u], hi_lim, p->NuXrayThreshold, p->SpecIndexXrayGal, 0);
+    ui[k] = integrate_over_nu(d, cz, zp, xm, lowers[u], hi_lim, p->NuXrayThreshold, p->SpecIndexXrayGal, 1);
+    ul[k] = integrate_over_nu(d, cz, zp, xm, lowers[u], hi_lim, p->NuXrayThreshold, p->SpecIndexXrayGal, 2);
+  });
+  for (int r = 0; r < F; r++)
+    for (int m = 0; m < XNH; m++) {
+      const size_t k = (size_t)which[r] * XNH + m;
+      heat[(size_t)m * F + r] = uh[k];
+      ion[(size_t)m * F + r] = ui[k];
+      lya[(size_t)m * F + r] = ul[k];
+    }
   if (filling < 0)
     filling = 0;
 
@@ -1225,10 +1249,14 @@ int ts_tables(const mxr_ts_params* p, const TsPlan& pl, double collapse_fraction
   dev->growth = growth;
   dev->dgrowth = dgrowth;
   dev->dt_dzp = dt_dzp;
+  dev->c_heat = dt_dzp * 2.0 / 3.0 / BOLTZMANN;
   dev->prefac_gal = prefac;
 
-  // evolveInt's per-shell constants with its float zp (XRayHeatingFunctions.c:1378-1433)
+  // evolveInt's per-shell constants with its float zp (XRayHeatingFunctions.c:1378-1433),
+  // folded with the unit conversions of the smoothed fields (ComputeTs.c:596-602, 704-711)
   const double zpf = (double)dev->zpf;
+  dev->cx = XRAY_LUMINOSITY_UNIT * dev->ULm3 / dev->pv;
+  const double csfr = dev->UMUT * dev->ULm3 / SOLAR_MASS / dev->pv;
   for (int r = 0; r < F; r++) {
     double zpp, dzpp;
     if (r == 0) {
@@ -1239,20 +1267,16 @@ int ts_tables(const mxr_ts_params* p, const TsPlan& pl, double collapse_fraction
       dzpp = pl.zpp_edge[r - 1] - pl.zpp_edge[r];
     }
     const double dt_dzpp = cz.dtdz((float)zpp);
-    dev->pw[r] = std::pow(1 + zpp, -a);
-    dev->dtd[r] = dt_dzpp * dzpp;
-    dev->opz[r] = 1 + zpp;
-    dev->lyn[r] = sum_lyn[r];
-    dev->dtz[r] = dt_dzpp;
-    dev->dzz[r] = dzpp;
+    const double wx = dt_dzpp * dzpp * std::pow(1 + zpp, -a) * dev->cx;
+    dev->bs[r] = csfr * dev->P2 * (1 + zpp) * sum_lyn[r] * dt_dzpp * dzpp;
     for (int m = 0; m < XNH; m++) {
       const int mh = m + 1 < XNH ? m + 1 : m;
       const double dx = (double)(d->x_int_XHII[mh] - d->x_int_XHII[m]);  // float difference
       const double* tb[3] = {heat.data(), ion.data(), lya.data()};
       for (int k = 0; k < 3; k++) {
         const double lo = tb[k][(size_t)m * F + r], hi = tb[k][(size_t)mh * F + r];
-        dev->base[k][m][r] = lo;
-        dev->slope[k][m][r] = mh != m ? (hi - lo) / dx : 0.0;
+        dev->fold[2 * k][m][r] = wx * (mh != m ? (hi - lo) / dx : 0.0);
+        dev->fold[2 * k + 1][m][r] = wx * lo;
       }
     }
   }

@@ -252,6 +252,7 @@ class TsGlobals(ctypes.Structure):
         ("volume_ave_TS", ctypes.c_double),
         ("volume_ave_TK", ctypes.c_double),
         ("volume_ave_xe", ctypes.c_double),
+        ("XrayEmissivity_HMXB", ctypes.c_double),
         ("device_ms", ctypes.c_double),
     ]
 

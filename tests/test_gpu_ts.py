@@ -72,6 +72,7 @@ def _compare(N, gpu, ref, tab, tag, ref64=None):
         keys += ["volume_ave_J_alpha", "volume_ave_xalpha", "volume_ave_Xheat", "volume_ave_Xion"]
         assert tab.collapse_fraction == pytest.approx(ref["red0"][0], rel=1e-9, abs=1e-300)
         assert tab.x_e_ave == pytest.approx(ref["red0"][1], rel=1e-9)
+        assert gg["XrayEmissivity_HMXB"] == pytest.approx(ref["red0"][2], rel=1e-5, abs=1e-300)
     for k in keys:
         assert gg[k] == pytest.approx(rg[k], rel=1e-5, abs=1e-30), k
 
