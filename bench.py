@@ -72,6 +72,21 @@ def cpu_baseline(N, box, inputs_host, n_radii_sample, n_radii_total):
     }
 
 
+def profiled_traffic(kernel_class, N):
+    """HBM bytes per launch of a kernel class from the committed rocprof summary of the 512^3
+    workload (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or None."""
+    path = os.path.join(REPO, "profiles", f"find_hii_bubbles_{N}_sharpk.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic-per-launch:"):
+                _, cls, val = line.split()
+                if cls == kernel_class:
+                    return float(val)
+    except OSError:
+        pass
+    return None
+
+
 def ts_leg(args, torch):
     """One spin-temperature step per timed call on device-resident inputs."""
     import numpy as np
@@ -118,13 +133,27 @@ def ts_leg(args, torch):
     ctx.profile(False)
     kernels = {k: dict(launches=v["launches"], total_ms=round(v["total_ms"], 3),
                        GBps=round(v["alg_bytes"] / max(v["total_ms"], 1e-9) / 1e6, 1)) for k, v in ks.items()}
-    out = {"metric": f"cell-shell updates/s (ComputeTs, {N}^3, {F} shells)", "value": N ** 3 * F / (ms / 1e3),
+    f = (N // 2 + 1) / N
+    rate = N ** 3 * F / (ms / 1e3)
+    ev = ks.get("ts_evolve")
+    ts_roof = None
+    if ev:
+        # evolve: per cell 2 FP32 shell values per shell + deltax, x_e (r/w), T_K (r/w), T_S (w)
+        ev_bytes = (8.0 * F + 4 + 8 + 8 + 4) * N ** 3
+        ev_s = ev["total_ms"] / ev["launches"] / 1e3
+        ts_roof = {"bound": "hbm", "kernel": "ts_evolve", "achieved": ev_bytes / ev_s / 1e9, "peak": HBM_PEAK / 1e9,
+                   "unit": "GB/s", "frac": ev_bytes / ev_s / HBM_PEAK, "alg_bytes_per_launch": ev_bytes,
+                   "avg_launch_ms": ev_s * 1e3,
+                   "call_model_bytes_per_cell_shell": 2 * (16 * f + 16 * f + 8 * f) + 4 + 8 + 4,
+                   "call_frac": rate * (2 * (40 * f) + 16) / HBM_PEAK}
+    out = {"metric": f"cell-shell updates/s (ComputeTs, {N}^3, {F} shells)", "value": rate,
            "unit": "cell-shell updates/s", "ms_per_step": round(ms, 3),
            "device_ms": round(sum(dev) / len(dev), 3), "steps": args.ts_steps,
            "config": {"workload": f"ComputeTs N={N} box={box} cMpc/h, {F} shells, TsHeatingFilterType=1, "
                                   f"snapshot 40 (z=8) of 41, {depth} history grids each of SFR and L_X",
                       "grid_dim": N, "shells": F},
-           "kernels": kernels, "volume_ave_TS": g.volume_ave_TS, "volume_ave_TK": g.volume_ave_TK,
+           "roofline": ts_roof, "kernels": kernels, "volume_ave_TS": g.volume_ave_TS,
+           "volume_ave_TK": g.volume_ave_TK,
            "volume_ave_xe": g.volume_ave_xe}
     log("bench: ts " + json.dumps({k: v for k, v in out.items() if k != "kernels"}))
     log("bench: ts kernels " + json.dumps(kernels))
@@ -234,8 +263,15 @@ def main():
     ctx.profile(False)
     dom_name, dom = max(ks.items(), key=lambda kv: kv[1]["total_ms"])
     avg_s = dom["total_ms"] / dom["launches"] / 1e3
-    bytes_per_launch = dom["alg_bytes"] / dom["launches"]
+    touched_per_launch = dom["alg_bytes"] / dom["launches"]
+    # algorithmic bytes per cell-radius of each radius pass in the out-of-core 3-pass model
+    # (B_cr = 3 grids x 5 x 8 f + 12 B, f = (N/2+1)/N): x and y passes read and write the half
+    # spectra, the z pass reads them and updates the state (xH read + write, z_in read)
+    f = (N // 2 + 1) / N
+    model_per_cell = {"inv_fft_x_filter": 3 * 16 * f, "inv_fft_y": 3 * 16 * f, "inv_c2r_z_update": 3 * 8 * f + 12}
+    bytes_per_launch = model_per_cell.get(dom_name, 0.0) * N ** 3 or touched_per_launch
     achieved = bytes_per_launch / avg_s
+    traffic = profiled_traffic(dom_name, N)
     kernels = {k: dict(launches=v["launches"], total_ms=round(v["total_ms"], 3),
                        GBps=round(v["alg_bytes"] / max(v["total_ms"], 1e-9) / 1e6, 1)) for k, v in ks.items()}
     log("bench: kernels " + json.dumps(kernels))
@@ -286,8 +322,12 @@ def main():
                                f"ReionFilterType=1, 3 grids, z=8, ReionEfficiency=1",
                    "grid_dim": N, "radii": n_radii, "cells": N ** 3},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s * 1e3},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s * 1e3,
+                     "bytes_model": "B_cr share of the pass x N^3 (z pass: 3 x 8 f + 12 B per cell-radius)",
+                     "touched_bytes_per_launch": touched_per_launch,
+                     "touched_GBps": touched_per_launch / avg_s / 1e9,
+                     "call_frac": (N ** 3 * n_radii / (ms_step / 1e3)) * (3 * 5 * 8 * f + 12) / HBM_PEAK},
         "kernels": kernels,
         "plain": plain,
         "cpu_baseline": cpu,
