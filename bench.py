@@ -14,7 +14,10 @@ Prints ONE JSON line.  Besides the contract fields it carries
   parity       : xH of the fused vs plain path (bit-identical fields are expected),
   spin_temperature : one ComputeTs step at 256^3 with 40 shells (z = 8, 41 snapshots z = 20..8,
                  per-snapshot SFR / X-ray histories), cell-shell updates/s, with its per-kernel
-                 breakdown and the oracle's CPU time on a bounded sample.
+                 breakdown and the oracle's CPU time on a bounded sample,
+  batch        : 64 independent realisations, N uniform in {96, 128, 160}, box in [100, 150]
+                 cMpc/h, ReionEfficiency in [0.5, 2], as one captured hipGraph (relaunched, and
+                 including its capture), on one stream each, and one at a time.
 """
 from __future__ import annotations
 
@@ -165,6 +168,67 @@ def ts_leg(args, torch):
     return out
 
 
+# stars-to-matter ratios of the seeded realisations used by the parity tests at these N
+BATCH_S2M = {96: 0.01253, 128: 0.19495, 160: 0.02558}
+
+
+def batch_leg(args, torch):
+    import numpy as np
+
+    import mxr
+    import synth
+
+    n = args.batch
+    rng = np.random.default_rng(args.seed)
+    dims = [int(d) for d in rng.choice([96, 128, 160], n)]
+    boxes = rng.uniform(100.0, 150.0, n)
+    zetas = rng.uniform(0.5, 2.0, n)
+    log(f"bench: batch of {n} realisations, N counts " + json.dumps({d: dims.count(d) for d in (96, 128, 160)}))
+    params, arrays, work = [], [], 0
+    for i in range(n):
+        N = dims[i]
+        p = mxr.hii_params(N, float(boxes[i]), filter_type=1, zeta=float(zetas[i]))
+        inp = synth.realisation(N, float(boxes[i]), args.seed + 1000 + i, device="cuda",
+                                stars_to_matter=BATCH_S2M[N])
+        m = N ** 3
+        st = dict(xH=torch.ones(m, device="cuda"), z_at_ionization=torch.full((m,), -1.0, device="cuda"),
+                  J_21_at_ionization=torch.zeros(m, device="cuda"), r_bubble=torch.zeros(m, device="cuda"),
+                  temp_kinetic_all_gas=torch.zeros(m, device="cuda"))
+        params.append(p)
+        arrays.append(dict(inp, **st))
+        work += m * len(mxr.radii(p))
+    torch.cuda.synchronize()
+    b = mxr.Batch(dims)
+    res = {}
+
+    def timed(mode, reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            gl = b.find_hii_bubbles(params, arrays, mode)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, gl
+
+    b.find_hii_bubbles(params, arrays, mxr.MXR_BATCH_SERIAL)  # warm-up: buffers, tables
+    t_serial, gl = timed(mxr.MXR_BATCH_SERIAL, 1)
+    t_streams, _ = timed(mxr.MXR_BATCH_STREAMS, args.batch_steps)
+    t_capture, _ = timed(mxr.MXR_BATCH_GRAPH, 1)  # capture + instantiate + launch
+    t_graph, gl = timed(mxr.MXR_BATCH_GRAPH, args.batch_steps)
+    for name, t in (("graph", t_graph), ("graph_with_capture", t_capture), ("streams", t_streams),
+                    ("one_at_a_time", t_serial)):
+        res[name] = {"realisations_per_s": n / t, "cell_radius_per_s": work / t, "ms": round(1e3 * t, 3)}
+    out = {"metric": f"realisations/s ({n} realisations, N in {{96,128,160}}, one hipGraph)",
+           "value": n / t_graph, "unit": "realisations/s", "cell_radius_updates_per_s": work / t_graph,
+           "speedup_vs_one_at_a_time": t_serial / t_graph, "modes": res, "cell_radius_per_batch": work,
+           "device_ms_graph": gl[0].device_ms,
+           "xH_mean": float(np.mean([g.volume_weighted_global_xH for g in gl]))}
+    log("bench: batch " + json.dumps(out))
+    b.close()
+    del arrays
+    torch.cuda.empty_cache()
+    return out
+
+
 def ts_cpu_baseline(args, N_full, F):
     """The oracle's FP32 spin-temperature step on a smaller grid of the same kind, scaled to
     cell-shell updates/s (its cost is linear in cells x shells up to the FFTs' log N)."""
@@ -210,6 +274,8 @@ def main():
     ap.add_argument("--ts-steps", type=int, default=5)
     ap.add_argument("--ts-stars-to-matter", type=float, default=0.020055)
     ap.add_argument("--ts-cpu-N", type=int, default=64, help="grid of the spin-temperature CPU sample (0: skip)")
+    ap.add_argument("--batch", type=int, default=64, help="realisations in the batch leg (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=3)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -303,6 +369,7 @@ def main():
         log("bench: cpu " + json.dumps(cpu))
 
     ts = ts_leg(args, torch) if args.ts_N > 0 else None
+    batch = batch_leg(args, torch) if args.batch > 0 else None
 
     value = N ** 3 * n_radii / (ms_step / 1e3)
     out = {
@@ -333,6 +400,7 @@ def main():
         "cpu_baseline": cpu,
         "xH_global": fused_xh,
         "spin_temperature": ts,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
 

@@ -165,6 +165,26 @@ double mxr_RtoM(const mxr_hii_params* params, double R);
  * (src/core/find_HII_bubbles.c:256-285, 546).                                             */
 int mxr_hii_radii(const mxr_hii_params* params, double* radii, int capacity);
 
+/* ---- batches of independent realisations ----------------------------------------------------- */
+/* A batch owns one context per realisation (grid dimensions may differ).  Modes:
+ *   MXR_BATCH_GRAPH   every realisation's launch sequence captured as one branch of a single
+ *                     hipGraph (fork/join), instantiated once and launched; a call with the
+ *                     same parameters and grid pointers as the previous one relaunches the
+ *                     instantiated graph without capturing again;
+ *   MXR_BATCH_STREAMS the same sequences launched directly, one stream per realisation;
+ *   MXR_BATCH_SERIAL  one realisation at a time, each call synchronised (the comparator).
+ * Grids must be device memory.  globals[i].device_ms holds the whole batch's device time.
+ * No counterpart in the reference (it runs one grid per call); the realisations' results are
+ * identical to n separate mxr_find_hii_bubbles calls.                                        */
+#define MXR_BATCH_GRAPH 0
+#define MXR_BATCH_STREAMS 1
+#define MXR_BATCH_SERIAL 2
+typedef struct mxr_batch mxr_batch;
+int mxr_batch_create(mxr_batch** batch, int device, int n, const int* grid_dims);
+void mxr_batch_destroy(mxr_batch* batch);
+int mxr_find_hii_bubbles_batched(mxr_batch* batch, const mxr_hii_params* params, const mxr_hii_grids* grids,
+                                 mxr_hii_globals* globals, int mode);
+
 /* ---- 21-cm brightness temperature ---------------------------------------------------------- */
 typedef struct mxr_tb_params {
   int ReionGridDim;

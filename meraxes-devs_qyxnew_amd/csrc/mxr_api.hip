@@ -5,7 +5,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "mxr_internal.hpp"
@@ -89,29 +91,50 @@ double T_RECFAST(const char* tables_dir, float z)
     set_error("T_RECFAST: no table directory");
     return -1;
   }
-  std::string fname = std::string(tables_dir) + "/recfast_LCDM.dat";
-  FILE* F = std::fopen(fname.c_str(), "r");
-  if (!F) {
-    set_error("T_RECFAST: unable to open %s", fname.c_str());
-    return -1;
+  // the table is read once per directory and kept
+  struct Table {
+    std::string dir;
+    std::vector<double> zt, TK;
+  };
+  static std::mutex mu;
+  static std::vector<Table> cache;
+  std::vector<double> zt, TK;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    for (const Table& t : cache)
+      if (t.dir == tables_dir) {
+        zt = t.zt;
+        TK = t.TK;
+      }
   }
-  double zt[RECFAST_NPTS], TK[RECFAST_NPTS];
-  for (int i = RECFAST_NPTS - 1; i >= 0; i--) {
-    float currz, trash, currTK;
-    if (std::fscanf(F, "%f %E %E %E", &currz, &trash, &trash, &currTK) != 4) {
-      std::fclose(F);
-      set_error("T_RECFAST: %s is truncated", fname.c_str());
+  if (zt.empty()) {
+    std::string fname = std::string(tables_dir) + "/recfast_LCDM.dat";
+    FILE* F = std::fopen(fname.c_str(), "r");
+    if (!F) {
+      set_error("T_RECFAST: unable to open %s", fname.c_str());
       return -1;
     }
-    zt[i] = currz;
-    TK[i] = currTK;
+    zt.assign(RECFAST_NPTS, 0.0);
+    TK.assign(RECFAST_NPTS, 0.0);
+    for (int i = RECFAST_NPTS - 1; i >= 0; i--) {
+      float currz, trash, currTK;
+      if (std::fscanf(F, "%f %E %E %E", &currz, &trash, &trash, &currTK) != 4) {
+        std::fclose(F);
+        set_error("T_RECFAST: %s is truncated", fname.c_str());
+        return -1;
+      }
+      zt[i] = currz;
+      TK[i] = currTK;
+    }
+    std::fclose(F);
+    std::lock_guard<std::mutex> lk(mu);
+    cache.push_back(Table{tables_dir, zt, TK});
   }
-  std::fclose(F);
   if (z > zt[RECFAST_NPTS - 1]) {
     set_error("T_RECFAST called with z=%f beyond the table", (double)z);
     return -1;
   }
-  return natural_spline(zt, TK, RECFAST_NPTS, z);
+  return natural_spline(zt.data(), TK.data(), RECFAST_NPTS, z);
 }
 
 int check_hii_params(const mxr_hii_params* p, int N)
@@ -437,6 +460,194 @@ int mxr_filter(mxr_ctx* h, void* box, int mem, int local_ix_start, int slab_nx, 
   }
   hipFree(d);
   return rc;
+}
+
+// ---- batches -----------------------------------------------------------------------------
+}  // extern "C" (reopened below)
+
+struct mxr_batch {
+  int device = 0;
+  std::vector<mxr_ctx*> ctx;
+  hipStream_t cap = nullptr;  // origin stream of the capture and of the graph launch
+  hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<unsigned char> key;  // parameters and grids the instantiated graph was captured with
+};
+
+static void batch_drop_graph(mxr_batch* b)
+{
+  if (b->exec)
+    (void)hipGraphExecDestroy(b->exec);
+  if (b->graph)
+    (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->key.clear();
+}
+
+extern "C" {
+
+int mxr_batch_create(mxr_batch** out, int device, int n, const int* grid_dims)
+{
+  if (!out || n <= 0 || !grid_dims) {
+    set_error("bad batch arguments");
+    return MXR_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  mxr_batch* b = new (std::nothrow) mxr_batch();
+  if (!b) {
+    set_error("host allocation failed");
+    return MXR_ERR_OOM;
+  }
+  b->device = device;
+  for (int i = 0; i < n; i++) {
+    mxr_ctx* c = nullptr;
+    const int rc = mxr_create(&c, device, grid_dims[i]);
+    if (rc) {
+      mxr_batch_destroy(b);
+      return rc;
+    }
+    b->ctx.push_back(c);
+  }
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreate(&b->t0)) != hipSuccess || (e = hipEventCreate(&b->t1)) != hipSuccess) {
+    const int rc = hip_fail(e, "batch stream/event creation", __FILE__, __LINE__);
+    mxr_batch_destroy(b);
+    return rc;
+  }
+  b->join.assign(n, nullptr);
+  for (int i = 0; i < n; i++)
+    if ((e = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming)) != hipSuccess) {
+      const int rc = hip_fail(e, "hipEventCreate", __FILE__, __LINE__);
+      mxr_batch_destroy(b);
+      return rc;
+    }
+  *out = b;
+  return MXR_OK;
+}
+
+void mxr_batch_destroy(mxr_batch* b)
+{
+  if (!b)
+    return;
+  (void)hipSetDevice(b->device);
+  if (b->cap)
+    (void)hipStreamSynchronize(b->cap);
+  batch_drop_graph(b);
+  for (auto e : b->join)
+    if (e)
+      (void)hipEventDestroy(e);
+  for (hipEvent_t e : {b->fork, b->t0, b->t1})
+    if (e)
+      (void)hipEventDestroy(e);
+  if (b->cap)
+    (void)hipStreamDestroy(b->cap);
+  for (auto c : b->ctx)
+    mxr_destroy(c);
+  delete b;
+}
+
+int mxr_find_hii_bubbles_batched(mxr_batch* b, const mxr_hii_params* params, const mxr_hii_grids* grids,
+                                 mxr_hii_globals* globals, int mode)
+{
+  if (!b || !params || !grids || !globals ||
+      (mode != MXR_BATCH_GRAPH && mode != MXR_BATCH_STREAMS && mode != MXR_BATCH_SERIAL)) {
+    set_error("bad batch call arguments (mode %d)", mode);
+    return MXR_ERR_INVALID_ARG;
+  }
+  const int n = (int)b->ctx.size();
+  for (int i = 0; i < n; i++) {
+    const int rc = check_hii_params(&params[i], b->ctx[i]->c.N);
+    if (rc)
+      return rc;
+    const mxr_hii_grids& g = grids[i];
+    if (g.mem != MXR_MEM_DEVICE || !g.deltax || !g.stars || !g.weighted_sfr || !g.xH || !g.z_at_ionization ||
+        !g.J_21_at_ionization || !g.r_bubble || !g.temp_kinetic_all_gas ||
+        (params[i].Flag_IncludeSpinTemp && (!g.x_e_box || !g.Tk_box))) {
+      set_error("realisation %d: batched grids must all be device memory", i);
+      return MXR_ERR_INVALID_ARG;
+    }
+  }
+  if (hipSetDevice(b->device) != hipSuccess) {
+    set_error("hipSetDevice(%d) failed", b->device);
+    return MXR_ERR_HIP;
+  }
+  float ms = 0.f;
+  if (mode == MXR_BATCH_SERIAL) {
+    double total = 0;
+    for (int i = 0; i < n; i++) {
+      const int rc = run_find_hii_bubbles(&b->ctx[i]->c, &params[i], &grids[i], &globals[i]);
+      if (rc)
+        return rc;
+      total += globals[i].device_ms;
+    }
+    for (int i = 0; i < n; i++)
+      globals[i].device_ms = total;
+    return MXR_OK;
+  }
+  for (int i = 0; i < n; i++) {
+    Ctx& c = b->ctx[i]->c;
+    const int rc = ensure_buffers(&c, params[i].Flag_IncludeSpinTemp ? 4 : 3);  // no allocation inside a capture
+    if (rc)
+      return rc;
+    c.prof = false;
+  }
+  auto enqueue_all = [&]() -> int {
+    MXR_HIP(hipEventRecord(b->fork, b->cap));
+    for (int i = 0; i < n; i++) {
+      Ctx& c = b->ctx[i]->c;
+      MXR_HIP(hipStreamWaitEvent(c.stream, b->fork, 0));
+      c.enqueue_only = true;
+      const int rc = run_find_hii_bubbles(&c, &params[i], &grids[i], &globals[i]);
+      c.enqueue_only = false;
+      if (rc)
+        return rc;
+      MXR_HIP(hipEventRecord(b->join[i], c.stream));
+      MXR_HIP(hipStreamWaitEvent(b->cap, b->join[i], 0));
+    }
+    return MXR_OK;
+  };
+  if (mode == MXR_BATCH_STREAMS) {
+    MXR_HIP(hipEventRecord(b->t0, b->cap));
+    const int rc = enqueue_all();
+    if (rc)
+      return rc;
+    MXR_HIP(hipEventRecord(b->t1, b->cap));
+  } else {
+    std::vector<unsigned char> key(sizeof(mxr_hii_params) * n + sizeof(mxr_hii_grids) * n);
+    std::memcpy(key.data(), params, sizeof(mxr_hii_params) * n);
+    std::memcpy(key.data() + sizeof(mxr_hii_params) * n, grids, sizeof(mxr_hii_grids) * n);
+    if (!b->exec || key != b->key) {
+      batch_drop_graph(b);
+      MXR_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue_all();
+      hipGraph_t gph = nullptr;
+      const hipError_t e = hipStreamEndCapture(b->cap, &gph);
+      if (rc || e != hipSuccess) {
+        if (gph)
+          (void)hipGraphDestroy(gph);
+        return rc ? rc : hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+      }
+      b->graph = gph;
+      MXR_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->key = key;
+      // the host-side finalisation data of each realisation was recorded while capturing
+    }
+    MXR_HIP(hipEventRecord(b->t0, b->cap));
+    MXR_HIP(hipGraphLaunch(b->exec, b->cap));
+    MXR_HIP(hipEventRecord(b->t1, b->cap));
+  }
+  MXR_HIP(hipStreamSynchronize(b->cap));
+  MXR_HIP(hipEventElapsedTime(&ms, b->t0, b->t1));
+  for (int i = 0; i < n; i++) {
+    hii_globals(&b->ctx[i]->c, &globals[i]);
+    globals[i].device_ms = ms;
+  }
+  return MXR_OK;
 }
 
 int mxr_compute_ts(mxr_ctx* h, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_globals* globals,

@@ -615,7 +615,8 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
   const double cplx = 8.0 * (double)Nc;  // bytes of one half spectrum
   const double G = (double)ngrid;
 
-  MXR_HIP(hipEventRecord(c->ev0, c->stream));
+  if (!c->enqueue_only)
+    MXR_HIP(hipEventRecord(c->ev0, c->stream));
   // ---- forward transforms (:163-253)
   FiltArgs none{};
   none.N = N;
@@ -659,7 +660,9 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
 
     if (!plain) {
       // kept-mode counts of this radius (all modes when not pruning)
-      KeptModes km = (fa.apply && fa.prune && fa.ftype == 1) ? kept_modes(N, fa.dk, fa.R) : KeptModes::all(N);
+      // kept-mode counts of this radius, for the profiler's byte counts only
+      KeptModes km = (c->prof && fa.apply && fa.prune && fa.ftype == 1) ? kept_modes(N, fa.dk, fa.R)
+                                                                        : KeptModes::all(N);
       // flags read (+ written on the first radius); on the last radius xH, r_bubble and temp
       // written and the deltax / temp (or Tk_box) needed for T_HI read
       const double state_b = last ? (double)nreal * (1 + 4 + 4 + 4 + 4 + 4) : (double)nreal * 1.0;
@@ -717,8 +720,14 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
     MXR_HIP(hipGetLastError());
     return MXR_OK;
   }));
-  MXR_HIP(hipEventRecord(c->ev1, c->stream));
+  if (!c->enqueue_only)
+    MXR_HIP(hipEventRecord(c->ev1, c->stream));
   MXR_HIP(hipMemcpyAsync(c->sums_host, c->sums, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
+  c->fin_total_n_cells = total_n_cells;
+  c->fin_sfr_conv = UM / UT * SEC_PER_YEAR / SOLAR_MASS;
+  c->fin_n_radii = nR;
+  if (c->enqueue_only)  // batched: the caller synchronises and calls hii_globals
+    return MXR_OK;
   if (g->mem == MXR_MEM_HOST) {
     float* hdst[5] = {g->xH, g->z_at_ionization, g->J_21_at_ionization, g->r_bubble, g->temp_kinetic_all_gas};
     for (int i = 0; i < 5; i++)
@@ -728,19 +737,23 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
   float ms = 0.f;
   MXR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   RC(prof_resolve(c));
-
-  const double* s = c->sums_host;
-  std::memset(out, 0, sizeof(*out));
-  out->volume_weighted_global_xH = s[0] / total_n_cells;
-  out->mass_weighted_global_xH = s[1] / s[2];
-  out->volume_weighted_global_weighted_sfr = s[3] / total_n_cells * (UM / UT * SEC_PER_YEAR / SOLAR_MASS);
-  out->volume_weighted_global_temp_kinetic_all_gas = s[4] / total_n_cells;
-  out->mass_weighted_global_temp_kinetic_all_gas = s[5] / s[2];
-  out->n_radii = nR;
+  hii_globals(c, out);
   out->device_ms = ms;
   return MXR_OK;
 }
 
+// globals of the last find_HII_bubbles call from its FP64 sums (find_HII_bubbles.c:661-708)
+void hii_globals(const Ctx* c, mxr_hii_globals* out)
+{
+  const double* s = c->sums_host;
+  std::memset(out, 0, sizeof(*out));
+  out->volume_weighted_global_xH = s[0] / c->fin_total_n_cells;
+  out->mass_weighted_global_xH = s[1] / s[2];
+  out->volume_weighted_global_weighted_sfr = s[3] / c->fin_total_n_cells * c->fin_sfr_conv;
+  out->volume_weighted_global_temp_kinetic_all_gas = s[4] / c->fin_total_n_cells;
+  out->mass_weighted_global_temp_kinetic_all_gas = s[5] / s[2];
+  out->n_radii = c->fin_n_radii;
+}
 
 bool supported_dim(int N)
 {

@@ -104,7 +104,12 @@ struct Ctx {
   size_t ts_hist_cap = 0;          // floats
   void* ts_dev = nullptr;          // device copy of the per-call table block
   void* ts_dev_host = nullptr;     // pinned host copy
-  hipEvent_t ev_red = nullptr;     // R_ct = 0 reductions ready
+  hipEvent_t ev_red = nullptr;     // R_ct = 0 averages ready
+  // batched find_HII_bubbles: enqueue without timing events or a final synchronisation (so the
+  // sequence can be captured into a graph); the globals follow from sums_host afterwards
+  bool enqueue_only = false;
+  double fin_total_n_cells = 0, fin_sfr_conv = 0;
+  int fin_n_radii = 0;
 };
 
 inline long n_complex(int N) { return (long)N * N * (N / 2 + 1); }
@@ -123,6 +128,7 @@ int run_find_hii_bubbles(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g
 int run_filter(Ctx* c, float2* box, int local_ix_start, int slab_nx, int grid_dim, float dk, float R, int ftype);
 int run_brightness_temperature(Ctx* c, const mxr_tb_params* p, const mxr_tb_grids* g, double* ave);
 int launch_final_sum(Ctx* c, const double* partials, int nblk, int nv, double* out);  // fixed-order FP64 sum
+void hii_globals(const Ctx* c, mxr_hii_globals* out);  // after a synchronised enqueue_only call
 int run_compute_ts(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_globals* out,
                    mxr_ts_tables* tables);
 int ensure_buffers(Ctx* c, int ngrid);

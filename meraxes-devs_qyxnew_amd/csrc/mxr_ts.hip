@@ -625,7 +625,7 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
         }
         fa.R = (float)pl.R[r];
         fa.apply = r > 0 ? 1 : 0;
-        const KeptModes km = (fa.apply && fa.prune) ? kept_modes(N, fa.dk, fa.R) : KeptModes::all(N);
+        const KeptModes km = (c->prof && fa.apply && fa.prune) ? kept_modes(N, fa.dk, fa.R) : KeptModes::all(N);
         TsStore st{{c->ts_store + (size_t)r * nreal, c->ts_store + ((size_t)F + r) * nreal}};
         RC(T(K_TS_INV_X, 2 * 8.0 * ((double)km.k3 + (double)N * km.c2),
              [&] { return K::template col<+1, COL_FILTER>(c, cspec, work, 2, K::xgeo(), fa, 1.0); }));

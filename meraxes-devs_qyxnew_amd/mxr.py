@@ -34,6 +34,9 @@ MXR_MEM_DEVICE = 1
 MXR_PATH_FUSED = 0
 MXR_PATH_PLAIN = 1
 MXR_PATH_FUSED_DENSE = 2
+MXR_BATCH_GRAPH = 0
+MXR_BATCH_STREAMS = 1
+MXR_BATCH_SERIAL = 2
 
 _fp = ctypes.POINTER(ctypes.c_float)
 
@@ -399,6 +402,13 @@ def lib():
     L.mxr_ts_build_tables.restype = ctypes.c_int
     L.mxr_ts_history_depth.argtypes = [ctypes.POINTER(TsParams), ctypes.c_int]
     L.mxr_ts_history_depth.restype = ctypes.c_int
+    L.mxr_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.mxr_batch_create.restype = ctypes.c_int
+    L.mxr_batch_destroy.argtypes = [vp]
+    L.mxr_batch_destroy.restype = None
+    L.mxr_find_hii_bubbles_batched.argtypes = [vp, ctypes.POINTER(HiiParams), ctypes.POINTER(HiiGrids),
+                                               ctypes.POINTER(HiiGlobals), ctypes.c_int]
+    L.mxr_find_hii_bubbles_batched.restype = ctypes.c_int
     L.mxr_profile_enable.argtypes = [vp, ctypes.c_int]
     L.mxr_profile_enable.restype = ctypes.c_int
     L.mxr_profile_read.argtypes = [vp, ctypes.POINTER(KernelStat), ctypes.c_int]
@@ -547,6 +557,41 @@ class Context:
     def close(self):
         if self._h:
             lib().mxr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """A batch of independent find_HII_bubbles realisations (grid dimensions may differ), run as
+    one captured hipGraph (MXR_BATCH_GRAPH), on one stream each (MXR_BATCH_STREAMS) or one at a
+    time (MXR_BATCH_SERIAL).  Grids must be device (torch cuda) tensors."""
+
+    def __init__(self, grid_dims, device: int = 0):
+        L = lib()
+        h = ctypes.c_void_p()
+        dims = (ctypes.c_int * len(grid_dims))(*grid_dims)
+        check(L.mxr_batch_create(ctypes.byref(h), device, len(grid_dims), dims), "mxr_batch_create")
+        self._h = h
+        self.dims = list(grid_dims)
+
+    def find_hii_bubbles(self, params_list, arrays_list, mode: int = MXR_BATCH_GRAPH):
+        n = len(self.dims)
+        assert len(params_list) == n and len(arrays_list) == n
+        P = (HiiParams * n)(*params_list)
+        G = (HiiGrids * n)(*[hii_grids(**a) for a in arrays_list])
+        out = (HiiGlobals * n)()
+        _sync_torch()
+        check(lib().mxr_find_hii_bubbles_batched(self._h, P, G, out, mode), "mxr_find_hii_bubbles_batched")
+        return list(out)
+
+    def close(self):
+        if self._h:
+            lib().mxr_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):
