@@ -268,6 +268,7 @@ def main():
     # bisected once on this GPU path so that seed 2024 ends at a global xH of 0.500
     ap.add_argument("--stars-to-matter", type=float, default=0.010438)
     ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--bf16-steps", type=int, default=3, help="timed calls of the BF16-storage variant (0: skip)")
     ap.add_argument("--cpu-radii", type=int, default=8, help="radii in the CPU-baseline sample (0: skip)")
     ap.add_argument("--ts-N", type=int, default=256, help="spin-temperature grid (0: skip that leg)")
     ap.add_argument("--ts-box", type=float, default=200.0)
@@ -321,6 +322,36 @@ def main():
     ms_step = 1e3 * wall / args.steps
     fused_xh = g.volume_weighted_global_xH
     log(f"bench: fused {ms_step:.2f} ms/step (device {sum(dev_ms) / len(dev_ms):.2f} ms), xH={fused_xh:.6f}")
+
+    # ---- BF16-storage variant: stars and weighted_sfr held as bfloat16 in HBM, FP32 compute
+    bf16 = None
+    if args.bf16_steps > 0:
+        xh32 = st.xH.clone()
+        bf_inputs = dict(inputs, stars=inputs["stars"].to(torch.bfloat16),
+                         weighted_sfr=inputs["weighted_sfr"].to(torch.bfloat16))
+        reset_state(st)
+
+        def call_bf16():
+            arrays = dict(bf_inputs)
+            arrays.update(st.arrays())
+            return ctx.find_hii_bubbles(params, **arrays)
+
+        call_bf16()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.bf16_steps):
+            gb = call_bf16()
+        torch.cuda.synchronize()
+        bms = 1e3 * (time.perf_counter() - t0) / args.bf16_steps
+        flips = int(((st.xH < 1e-5) != (xh32 < 1e-5)).sum().item())
+        bf16 = {"value": N ** 3 * n_radii / (bms / 1e3), "ms_per_step": round(bms, 3),
+                "xH": gb.volume_weighted_global_xH, "dxH_vs_fp32": gb.volume_weighted_global_xH - fused_xh,
+                "flag_mismatch_vs_fp32_frac": flips / N ** 3,
+                "storage": "stars, weighted_sfr bfloat16 (round to nearest even); deltax and spectra FP32"}
+        log("bench: bf16 " + json.dumps(bf16))
+        del bf_inputs, xh32
+        reset_state(st)
+        call(ctx)
 
     # ---- per-kernel timing (one profiled call)
     ctx.profile(True)
@@ -397,6 +428,7 @@ def main():
                      "call_frac": (N ** 3 * n_radii / (ms_step / 1e3)) * (3 * 5 * 8 * f + 12) / HBM_PEAK},
         "kernels": kernels,
         "plain": plain,
+        "bf16_storage": bf16,
         "cpu_baseline": cpu,
         "xH_global": fused_xh,
         "spin_temperature": ts,

@@ -98,7 +98,12 @@ typedef struct mxr_hii_grids {
   float* J_21_at_ionization;    /* in/out real                                            */
   float* r_bubble;              /* out real                                               */
   float* temp_kinetic_all_gas;  /* in/out real                                            */
+  int storage;                  /* MXR_STORAGE_F32 (0), or MXR_STORAGE_BF16: stars and
+                                   weighted_sfr hold bfloat16 bit patterns (uint16, same padded
+                                   layout), widened to FP32 on load; fused paths only        */
 } mxr_hii_grids;
+#define MXR_STORAGE_F32 0
+#define MXR_STORAGE_BF16 1
 
 /* Globals written back by the reference (src/core/find_HII_bubbles.c:686-702). */
 typedef struct mxr_hii_globals {

@@ -409,8 +409,9 @@ int mxr_find_hii_bubbles(mxr_ctx* h, const mxr_hii_params* params, const mxr_hii
   if (!grids->deltax || !grids->stars || !grids->weighted_sfr || !grids->xH || !grids->z_at_ionization ||
       !grids->J_21_at_ionization || !grids->r_bubble || !grids->temp_kinetic_all_gas ||
       (params->Flag_IncludeSpinTemp && (!grids->x_e_box || !grids->Tk_box)) ||
-      (grids->mem != MXR_MEM_HOST && grids->mem != MXR_MEM_DEVICE)) {
-    set_error("missing grid pointer or bad memory kind");
+      (grids->mem != MXR_MEM_HOST && grids->mem != MXR_MEM_DEVICE) ||
+      (grids->storage != MXR_STORAGE_F32 && grids->storage != MXR_STORAGE_BF16)) {
+    set_error("missing grid pointer, bad memory kind or bad storage type");
     return MXR_ERR_INVALID_ARG;
   }
   if (hipSetDevice(h->c.device) != hipSuccess) {

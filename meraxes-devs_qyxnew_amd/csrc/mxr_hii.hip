@@ -45,6 +45,7 @@ struct HiiEpi {
   float* temp;
   const float* deltax;  // unfiltered input (padded)
   const float* sfr;     // unfiltered input (padded)
+  int sfr_bf16;         // sfr holds bfloat16 bit patterns
   const float* Tk_box;  // real (spin only)
   double* partials;     // 6 per block on the last radius
 };
@@ -394,7 +395,7 @@ __global__ void __launch_bounds__(NT) k_postloop(int N, HiiEpi ea)
     acc[0] += (double)xh;
     acc[1] += (double)xh * dom;
     acc[2] += dom;
-    acc[3] += (double)ea.sfr[ip];
+    acc[3] += (double)(ea.sfr_bf16 ? bf16_to_f32(reinterpret_cast<const unsigned short*>(ea.sfr)[ip]) : ea.sfr[ip]);
     acc[4] += (double)temp;
     acc[5] += (double)temp * dom;
   }
@@ -531,6 +532,11 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
   constexpr int H = N / 2 + 1;
   const long Nc = n_complex(N), nreal = n_real(N);
   const int spin = p->Flag_IncludeSpinTemp ? 1 : 0;
+  const bool bf16 = g->storage == MXR_STORAGE_BF16;  // stars and weighted_sfr as bfloat16
+  if (bf16 && c->path == MXR_PATH_PLAIN) {
+    set_error("BF16 storage is implemented on the fused paths only");
+    return MXR_ERR_UNSUPPORTED;
+  }
   const int ngrid = spin ? 4 : 3;
   RC(ensure_buffers(c, ngrid));
 
@@ -540,7 +546,7 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
   float* dst[5] = {g->xH, g->z_at_ionization, g->J_21_at_ionization, g->r_bubble, g->temp_kinetic_all_gas};
   if (g->mem == MXR_MEM_HOST) {
     for (int i = 0; i < ngrid; i++)
-      RC(stage_in(c, i, din[i], 2 * Nc, &din[i]));
+      RC(stage_in(c, i, din[i], (bf16 && (i == 1 || i == 2)) ? Nc : 2 * Nc, &din[i]));  // bf16: half the bytes
     if (spin)
       RC(stage_in(c, 4, dTk, nreal, &dTk));
     for (int i = 0; i < 5; i++)
@@ -578,6 +584,7 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
   ea.temp = dst[4];
   ea.deltax = din[0];
   ea.sfr = din[2];
+  ea.sfr_bf16 = bf16 ? 1 : 0;
   ea.Tk_box = dTk;
   ea.partials = c->partials;
 
@@ -595,6 +602,7 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
   CGridPtrs cspec{}, cwork{};
   for (int i = 0; i < ngrid; i++) {
     rin.p[i] = din[i];
+  rin.bf16 = bf16 ? (1u << 1) | (1u << 2) : 0u;
     spec.p[i] = c->spec[i];
     work.p[i] = c->work[i];
     cspec.p[i] = c->spec[i];

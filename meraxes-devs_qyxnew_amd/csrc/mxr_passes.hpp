@@ -19,7 +19,10 @@ struct CGridPtrs {
 };
 struct CRealPtrs {
   const float* p[4];
+  unsigned bf16;  // bit g: p[g] holds bfloat16 bit patterns (uint16), widened on load
 };
+
+__device__ __forceinline__ float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
 // column c of a column pass starts at (c / cpp) * ps + (c % cpp); its elements are es apart
 struct ColGeom {
@@ -198,7 +201,13 @@ __global__ void __launch_bounds__(NT) k_row_r2c(CRealPtrs in, GridPtrs out, long
       const int n = j + r * SL::B0;
       float2 x = make_float2(0.f, 0.f);
       if (valid && (SL::B0 % T == 0 || j < SL::B0))
-        x = *reinterpret_cast<const float2*>(src + 2 * n);
+        x = (in.bf16 >> g) & 1u
+              ? [&] {
+                  const unsigned v = *reinterpret_cast<const unsigned*>(
+                    reinterpret_cast<const unsigned short*>(pick(in, g)) + (valid ? row : 0) * 2 * H + 2 * n);
+                  return make_float2(bf16_to_f32((unsigned short)(v & 0xffffu)), bf16_to_f32((unsigned short)(v >> 16)));
+                }()
+              : *reinterpret_cast<const float2*>(src + 2 * n);
       v[q * SL::R0 + r] = x;
     }
   }

@@ -34,6 +34,8 @@ MXR_MEM_DEVICE = 1
 MXR_PATH_FUSED = 0
 MXR_PATH_PLAIN = 1
 MXR_PATH_FUSED_DENSE = 2
+MXR_STORAGE_F32 = 0
+MXR_STORAGE_BF16 = 1
 MXR_BATCH_GRAPH = 0
 MXR_BATCH_STREAMS = 1
 MXR_BATCH_SERIAL = 2
@@ -89,6 +91,7 @@ class HiiGrids(ctypes.Structure):
         ("J_21_at_ionization", ctypes.c_void_p),
         ("r_bubble", ctypes.c_void_p),
         ("temp_kinetic_all_gas", ctypes.c_void_p),
+        ("storage", ctypes.c_int),
     ]
 
 
@@ -442,11 +445,21 @@ HII_IN = ("deltax", "stars", "weighted_sfr", "x_e_box", "Tk_box")
 HII_OUT = ("xH", "z_at_ionization", "J_21_at_ionization", "r_bubble", "temp_kinetic_all_gas")
 
 
+def _is_bf16(a):
+    return a is not None and not isinstance(a, np.ndarray) and str(getattr(a, "dtype", "")) == "torch.bfloat16"
+
+
 def hii_grids(**arrays) -> HiiGrids:
+    """Grid pointers; stars and weighted_sfr given as torch.bfloat16 (or numpy uint16 bit
+    patterns with storage=MXR_STORAGE_BF16) select the BF16-storage variant."""
+    storage = arrays.pop("storage", None)
     g = HiiGrids()
     g.mem = _mem_of(list(arrays.values()))
     for name in HII_IN + HII_OUT:
         setattr(g, name, _ptr(arrays.get(name)))
+    if storage is None:
+        storage = MXR_STORAGE_BF16 if _is_bf16(arrays.get("stars")) else MXR_STORAGE_F32
+    g.storage = storage
     return g
 
 
