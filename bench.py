@@ -25,6 +25,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
@@ -199,6 +200,15 @@ def batch_leg(args, torch):
         work += m * len(mxr.radii(p))
     torch.cuda.synchronize()
     b = mxr.Batch(dims)
+    t0 = time.perf_counter()
+    tuning_src = "default"
+    if args.autotune != "never":
+        if args.autotune == "auto" and b.load_wisdom(params, args.wisdom) == mxr.MXR_OK:
+            tuning_src = "wisdom"
+        else:
+            b.autotune(params, arrays, wisdom=args.wisdom)
+            tuning_src = "autotuned"
+    log(f"bench: batch tuning {tuning_src} in {time.perf_counter() - t0:.1f} s")
     res = {}
 
     def timed(mode, reps):
@@ -220,7 +230,7 @@ def batch_leg(args, torch):
     out = {"metric": f"realisations/s ({n} realisations, N in {{96,128,160}}, one hipGraph)",
            "value": n / t_graph, "unit": "realisations/s", "cell_radius_updates_per_s": work / t_graph,
            "speedup_vs_one_at_a_time": t_serial / t_graph, "modes": res, "cell_radius_per_batch": work,
-           "device_ms_graph": gl[0].device_ms,
+           "device_ms_graph": gl[0].device_ms, "tuning": tuning_src,
            "xH_mean": float(np.mean([g.volume_weighted_global_xH for g in gl]))}
     log("bench: batch " + json.dumps(out))
     b.close()
@@ -257,6 +267,24 @@ def ts_cpu_baseline(args, N_full, F):
             "kind": "port", "sample": f"one FP32 oracle step at {N}^3 ({box:.0f} cMpc/h), {F} shells: {secs:.1f} s"}
 
 
+def tune(args, ctx, params, inputs):
+    """Launch shapes for this context: the wisdom file's entry when it has one (--autotune
+    auto), else measured by mxr_autotune on these inputs and stored there; before any timing."""
+    import mxr
+
+    t0 = time.perf_counter()
+    source = "default"
+    if args.autotune != "never":
+        if args.autotune == "auto" and ctx.load_wisdom(params, args.wisdom) == mxr.MXR_OK:
+            source = "wisdom"
+        else:
+            ctx.autotune(params, wisdom=args.wisdom, **inputs)
+            source = "autotuned"
+    out = dict(ctx.tuning().as_dict(), source=source, seconds=round(time.perf_counter() - t0, 2), wisdom=args.wisdom)
+    log("bench: tuning " + json.dumps(out))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,6 +305,9 @@ def main():
     ap.add_argument("--ts-cpu-N", type=int, default=64, help="grid of the spin-temperature CPU sample (0: skip)")
     ap.add_argument("--batch", type=int, default=64, help="realisations in the batch leg (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--autotune", choices=("auto", "always", "never"), default="auto",
+                    help="launch shapes: wisdom entry if present else autotune (auto), always tune, or defaults")
+    ap.add_argument("--wisdom", default=os.path.join(tempfile.gettempdir(), "mxr-wisdom.json"))
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -300,6 +331,7 @@ def main():
     log(f"bench: inputs ready in {time.time() - t0:.1f} s")
     st = make_state(torch, N ** 3)
     ctx = mxr.Context(N)
+    tuning = tune(args, ctx, params, inputs)
 
     torch.cuda.synchronize()
 
@@ -429,6 +461,7 @@ def main():
         "kernels": kernels,
         "plain": plain,
         "bf16_storage": bf16,
+        "tuning": tuning,
         "cpu_baseline": cpu,
         "xH_global": fused_xh,
         "spin_temperature": ts,

@@ -31,6 +31,7 @@ extern "C" {
 #define MXR_ERR_TABLES (-5)       /* a physics table could not be read or is out of range   */
 #define MXR_ERR_NO_DEVICE (-6)    /* no usable gfx950 device                               */
 #define MXR_ERR_FILTER (-7)       /* ReionFilterType not in {0,1,2} (reference aborts)      */
+#define MXR_ERR_NO_WISDOM (-8)    /* the wisdom file holds no entry for this device and shape */
 
 /* ---- memory location of the pointers handed to an entry point ------------------------- */
 #define MXR_MEM_HOST 0
@@ -143,6 +144,40 @@ const char* mxr_last_error(void);
 /* Select MXR_PATH_FUSED (default) or MXR_PATH_PLAIN. */
 int mxr_set_path(mxr_ctx* ctx, int path);
 
+/* Launch shapes of the fused path, per context.  No counterpart in the reference, where the
+ * FFTW plans and their wisdom file fill this role (src/core/reionization.c:726-758, 907-997).
+ * Every field has a gfx950 default.                                                        */
+typedef struct mxr_tuning {
+  int zdirect_kmax; /* z pass by direct summation on sharp-k radii keeping <= this many k_z
+                       (0: never; at most 64; applies when the grid dimension is a multiple
+                       of 256)                                                               */
+  int zdirect_rpb;  /* rows per wavefront of the direct z pass (1 .. 4096)                  */
+  int live_tiles;   /* 1: x and y pass grids cover only the tiles that can hold kept sharp-k
+                       columns; 0: every tile is launched and empty ones return at once    */
+  int col_tc_x;     /* columns per workgroup of the per-radius x pass (4, 8 or 16)          */
+  int col_tc_y;     /* columns per workgroup of the per-radius y pass (4, 8 or 16)          */
+} mxr_tuning;
+int mxr_get_tuning(const mxr_ctx* ctx, mxr_tuning* t);
+int mxr_set_tuning(mxr_ctx* ctx, const mxr_tuning* t);
+
+/* Autotuner: time candidate launch shapes with find_HII_bubbles on these inputs (device or
+ * host grids; the calls run on scratch state, the caller's state grids are left untouched),
+ * keep the fastest in the context, return it in `chosen` (may be NULL) and, when
+ * `wisdom_path` is not NULL, merge it into that JSON wisdom file under the key (device
+ * architecture, compute units, grid dimension, ReionFilterType, Flag_IncludeSpinTemp).
+ * Takes the place of the reference's FFTW planning with its wisdom file
+ * (src/core/reionization.c:726-758, 907-997).                                              */
+int mxr_autotune(mxr_ctx* ctx, const mxr_hii_params* params, const mxr_hii_grids* grids, const char* wisdom_path,
+                 mxr_tuning* chosen);
+/* Apply the wisdom file's entry for this context's device and grid dimension and the
+ * params' filter type and spin flag; MXR_ERR_NO_WISDOM (tuning unchanged) when it has none. */
+int mxr_load_wisdom(mxr_ctx* ctx, const mxr_hii_params* params, const char* wisdom_path);
+/* The wisdom file itself, without a device: look an entry up, or add / replace one.      */
+int mxr_wisdom_lookup(const char* wisdom_path, const char* arch, int cus, int grid_dim, int filter_type, int spin,
+                      mxr_tuning* t);
+int mxr_wisdom_store(const char* wisdom_path, const char* arch, int cus, int grid_dim, int filter_type, int spin,
+                     const mxr_tuning* t, double ms);
+
 /* ---- the hot path ------------------------------------------------------------------------ */
 
 /* One excursion-set ionisation step: forward r2c of deltax/stars/weighted_sfr (+x_e_box),
@@ -189,6 +224,13 @@ int mxr_batch_create(mxr_batch** batch, int device, int n, const int* grid_dims)
 void mxr_batch_destroy(mxr_batch* batch);
 int mxr_find_hii_bubbles_batched(mxr_batch* batch, const mxr_hii_params* params, const mxr_hii_grids* grids,
                                  mxr_hii_globals* globals, int mode);
+/* mxr_autotune once per distinct (grid dimension, filter type, spin flag) of the batch, on the
+ * first realisation with that shape, applied to every realisation of that shape; and
+ * mxr_load_wisdom for each realisation (MXR_ERR_NO_WISDOM if any has no entry; the others
+ * are applied).                                                                              */
+int mxr_batch_autotune(mxr_batch* batch, const mxr_hii_params* params, const mxr_hii_grids* grids,
+                       const char* wisdom_path);
+int mxr_batch_load_wisdom(mxr_batch* batch, const mxr_hii_params* params, const char* wisdom_path);
 
 /* ---- 21-cm brightness temperature ---------------------------------------------------------- */
 typedef struct mxr_tb_params {

@@ -231,6 +231,7 @@ const char* mxr_strerror(int code)
     case MXR_ERR_TABLES: return "physics table unavailable";
     case MXR_ERR_NO_DEVICE: return "no usable gfx950 device";
     case MXR_ERR_FILTER: return "undefined ReionFilterType";
+    case MXR_ERR_NO_WISDOM: return "no wisdom entry for this device and shape";
     default: return "unknown error";
   }
 }
@@ -303,40 +304,40 @@ void mxr_destroy(mxr_ctx* h)
   if (!h)
     return;
   Ctx& c = h->c;
-  hipSetDevice(c.device);
+  (void)hipSetDevice(c.device);
   if (c.stream)
-    hipStreamSynchronize(c.stream);
-  hipFree(c.tw);
+    (void)hipStreamSynchronize(c.stream);
+  (void)hipFree(c.tw);
   for (int g = 0; g < 4; g++) {
-    hipFree(c.spec[g]);
-    hipFree(c.work[g]);
+    (void)hipFree(c.spec[g]);
+    (void)hipFree(c.work[g]);
   }
   for (int g = 0; g < 5; g++) {
-    hipFree(c.in_stage[g]);
-    hipFree(c.out_stage[g]);
+    (void)hipFree(c.in_stage[g]);
+    (void)hipFree(c.out_stage[g]);
   }
-  hipFree(c.ion);
+  (void)hipFree(c.ion);
   for (int g = 0; g < 2; g++)
-    hipFree(c.ts_src[g]);
-  hipFree(c.ts_store);
-  hipFree(c.ts_hist);
-  hipFree(c.ts_dev);
+    (void)hipFree(c.ts_src[g]);
+  (void)hipFree(c.ts_store);
+  (void)hipFree(c.ts_hist);
+  (void)hipFree(c.ts_dev);
   if (c.ts_dev_host)
-    hipHostFree(c.ts_dev_host);
+    (void)hipHostFree(c.ts_dev_host);
   if (c.ev_red)
-    hipEventDestroy(c.ev_red);
-  hipFree(c.partials);
-  hipFree(c.sums);
+    (void)hipEventDestroy(c.ev_red);
+  (void)hipFree(c.partials);
+  (void)hipFree(c.sums);
   if (c.sums_host)
-    hipHostFree(c.sums_host);
+    (void)hipHostFree(c.sums_host);
   for (auto e : c.evpool)
-    hipEventDestroy(e);
+    (void)hipEventDestroy(e);
   if (c.ev0)
-    hipEventDestroy(c.ev0);
+    (void)hipEventDestroy(c.ev0);
   if (c.ev1)
-    hipEventDestroy(c.ev1);
+    (void)hipEventDestroy(c.ev1);
   if (c.stream)
-    hipStreamDestroy(c.stream);
+    (void)hipStreamDestroy(c.stream);
   delete h;
 }
 
@@ -347,6 +348,93 @@ int mxr_set_path(mxr_ctx* h, int path)
     return MXR_ERR_INVALID_ARG;
   }
   h->c.path = path;
+  return MXR_OK;
+}
+
+int mxr_get_tuning(const mxr_ctx* h, mxr_tuning* t)
+{
+  if (!h || !t) {
+    set_error("null context or tuning");
+    return MXR_ERR_INVALID_ARG;
+  }
+  tuning_get(h->c, t);
+  return MXR_OK;
+}
+
+int mxr_set_tuning(mxr_ctx* h, const mxr_tuning* t)
+{
+  if (!h || !t) {
+    set_error("null context or tuning");
+    return MXR_ERR_INVALID_ARG;
+  }
+  RC(tuning_check(t));
+  tuning_set(h->c, *t);
+  return MXR_OK;
+}
+
+int mxr_wisdom_lookup(const char* path, const char* arch, int cus, int grid_dim, int filter_type, int spin,
+                      mxr_tuning* t)
+{
+  if (!path || !arch || !t) {
+    set_error("null wisdom path, architecture or tuning");
+    return MXR_ERR_INVALID_ARG;
+  }
+  return wisdom_lookup(path, arch, cus, grid_dim, filter_type, spin ? 1 : 0, t);
+}
+
+int mxr_wisdom_store(const char* path, const char* arch, int cus, int grid_dim, int filter_type, int spin,
+                     const mxr_tuning* t, double ms)
+{
+  if (!path || !arch || !t) {
+    set_error("null wisdom path, architecture or tuning");
+    return MXR_ERR_INVALID_ARG;
+  }
+  return wisdom_store(path, arch, cus, grid_dim, filter_type, spin ? 1 : 0, *t, ms);
+}
+
+int mxr_load_wisdom(mxr_ctx* h, const mxr_hii_params* p, const char* path)
+{
+  if (!h || !p || !path) {
+    set_error("null context, params or wisdom path");
+    return MXR_ERR_INVALID_ARG;
+  }
+  std::string arch;
+  int cus = 0;
+  RC(device_key(h->c, arch, cus));
+  mxr_tuning t;
+  RC(wisdom_lookup(path, arch.c_str(), cus, h->c.N, p->ReionFilterType, p->Flag_IncludeSpinTemp ? 1 : 0, &t));
+  tuning_set(h->c, t);
+  return MXR_OK;
+}
+
+int mxr_autotune(mxr_ctx* h, const mxr_hii_params* p, const mxr_hii_grids* g, const char* path, mxr_tuning* chosen)
+{
+  if (!h || !p || !g) {
+    set_error("null context, params or grids");
+    return MXR_ERR_INVALID_ARG;
+  }
+  RC(check_hii_params(p, h->c.N));
+  if (!g->deltax || !g->stars || !g->weighted_sfr || (p->Flag_IncludeSpinTemp && (!g->x_e_box || !g->Tk_box)) ||
+      (g->mem != MXR_MEM_HOST && g->mem != MXR_MEM_DEVICE) ||
+      (g->storage != MXR_STORAGE_F32 && g->storage != MXR_STORAGE_BF16)) {
+    set_error("missing input grid, bad memory kind or bad storage type");
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (hipSetDevice(h->c.device) != hipSuccess) {
+    set_error("hipSetDevice(%d) failed", h->c.device);
+    return MXR_ERR_HIP;
+  }
+  mxr_tuning best;
+  double ms = 0;
+  RC(autotune(&h->c, p, g, &best, &ms));
+  if (chosen)
+    *chosen = best;
+  if (path) {
+    std::string arch;
+    int cus = 0;
+    RC(device_key(h->c, arch, cus));
+    RC(wisdom_store(path, arch.c_str(), cus, h->c.N, p->ReionFilterType, p->Flag_IncludeSpinTemp ? 1 : 0, best, ms));
+  }
   return MXR_OK;
 }
 
@@ -459,7 +547,7 @@ int mxr_filter(mxr_ctx* h, void* box, int mem, int local_ix_start, int slab_nx, 
     if (e != hipSuccess)
       rc = hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
   }
-  hipFree(d);
+  (void)hipFree(d);
   return rc;
 }
 
@@ -619,9 +707,16 @@ int mxr_find_hii_bubbles_batched(mxr_batch* b, const mxr_hii_params* params, con
       return rc;
     MXR_HIP(hipEventRecord(b->t1, b->cap));
   } else {
-    std::vector<unsigned char> key(sizeof(mxr_hii_params) * n + sizeof(mxr_hii_grids) * n);
+    // the captured kernels depend on the parameters, the grids and each context's launch shapes
+    std::vector<unsigned char> key((sizeof(mxr_hii_params) + sizeof(mxr_hii_grids) + sizeof(mxr_tuning)) * n);
     std::memcpy(key.data(), params, sizeof(mxr_hii_params) * n);
     std::memcpy(key.data() + sizeof(mxr_hii_params) * n, grids, sizeof(mxr_hii_grids) * n);
+    for (int i = 0; i < n; i++) {
+      mxr_tuning t;
+      tuning_get(b->ctx[i]->c, &t);
+      std::memcpy(key.data() + (sizeof(mxr_hii_params) + sizeof(mxr_hii_grids)) * n + sizeof(mxr_tuning) * i, &t,
+                  sizeof t);
+    }
     if (!b->exec || key != b->key) {
       batch_drop_graph(b);
       MXR_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
@@ -647,6 +742,55 @@ int mxr_find_hii_bubbles_batched(mxr_batch* b, const mxr_hii_params* params, con
   for (int i = 0; i < n; i++) {
     hii_globals(&b->ctx[i]->c, &globals[i]);
     globals[i].device_ms = ms;
+  }
+  return MXR_OK;
+}
+
+int mxr_batch_autotune(mxr_batch* b, const mxr_hii_params* params, const mxr_hii_grids* grids, const char* path)
+{
+  if (!b || !params || !grids) {
+    set_error("null batch, params or grids");
+    return MXR_ERR_INVALID_ARG;
+  }
+  struct Done {
+    int N, filter, spin;
+    mxr_tuning t;
+  };
+  std::vector<Done> done;
+  for (size_t i = 0; i < b->ctx.size(); i++) {
+    const int N = b->ctx[i]->c.N, f = params[i].ReionFilterType, sp = params[i].Flag_IncludeSpinTemp ? 1 : 0;
+    const Done* hit = nullptr;
+    for (const Done& d : done)
+      if (d.N == N && d.filter == f && d.spin == sp)
+        hit = &d;
+    if (hit) {
+      tuning_set(b->ctx[i]->c, hit->t);
+      continue;
+    }
+    Done d{N, f, sp, {}};
+    RC(mxr_autotune(b->ctx[i], &params[i], &grids[i], path, &d.t));
+    done.push_back(d);
+  }
+  return MXR_OK;
+}
+
+int mxr_batch_load_wisdom(mxr_batch* b, const mxr_hii_params* params, const char* path)
+{
+  if (!b || !params || !path) {
+    set_error("null batch, params or wisdom path");
+    return MXR_ERR_INVALID_ARG;
+  }
+  int missing = 0;
+  for (size_t i = 0; i < b->ctx.size(); i++) {
+    const int rc = mxr_load_wisdom(b->ctx[i], &params[i], path);
+    if (rc == MXR_ERR_NO_WISDOM)
+      missing++;
+    else if (rc)
+      return rc;
+  }
+  if (missing) {
+    set_error("%d realisations have no wisdom entry in %s", missing, path);
+    return MXR_ERR_NO_WISDOM;
   }
   return MXR_OK;
 }

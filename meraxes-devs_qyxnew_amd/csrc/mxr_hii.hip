@@ -264,6 +264,118 @@ __global__ void __launch_bounds__(NT) k_row_c2r(CGridPtrs in, int ngrid, long nr
   }
 }
 
+// One cell of the fused update (the per-cell form of hii_pair, same order of operations).
+__device__ __forceinline__ void hii_cell(const HiiEpi& ea, int ir, int ip, float df, float sf, float xf)
+{
+#pragma clang fp contract(off)
+  const bool was = !ea.first && ea.ion[ir] != 0;
+  bool ion = was, cross = false;
+  float xh = was ? 0.0f : 1.0f;
+  if (!was) {
+    double lhs = 0.0, nf = 1.0;
+    ion = hii_criterion(ea, df, sf, xf, ea.last != 0, lhs, nf);
+    if (ion) {
+      xh = 0.0f;
+    } else if (ea.last) {
+      const float v = (float)(nf - lhs);
+      xh = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+    }
+    cross = xh < REL_TOL;
+  }
+  if (ea.first || ion != was)
+    ea.ion[ir] = ion ? 1 : 0;
+  if (cross && ea.z_in[ir] < 0) {
+    ea.z_in[ir] = ea.z;
+    if (ea.uvb == 1)
+      ea.J21[ir] = __fmul_rn(sf, ea.J21c);
+  }
+  if (ea.last) {
+    ea.xH[ir] = xh;
+    ea.r_bubble[ir] = 0.0f;
+    if (!ion)
+      ea.temp[ir] = ea.spin ? ea.Tk_box[ir] : thermal_T(ea.TK, ea.cT_ad, ea.deltax[ip]);
+  }
+}
+
+// ---- z-pass inverse by direct summation, for sharp-k radii that keep few k_z -------------
+// When only the kc < N/2 leading modes of a row are nonzero, the c2r output is
+//   y[n] = sum_{k<kc} w_k Re(X_k exp(+2 pi i k n / N)),  w_0 = 1, w_k = 2 (Im X_0 ignored),
+// which costs kc multiply-adds per output instead of a full-length transform.  One lane owns
+// p in [0, N/4) and one set of twiddles (C_k, S_k) = w_k (cos, sin)(2 pi k p / N); with
+// A_j = sum_{k = j mod 4} a_k C_k and B_j = sum_{k = j mod 4} b_k S_k (X_k = a_k + i b_k):
+//   y[p] = sA - sB,  y[N-p] = sA + sB,  y[N/2+p] = dA - dB,  y[N/2-p] = dA + dB,
+//   sA = (A0+A2)+(A1+A3), dA = (A0+A2)-(A1+A3) (same for B).
+// Lane p = 0 takes (C_k, S_k) = w_k (1, 1) and forms y[0] = sA, y[N/2] = dA,
+// y[N/4] = (A0-A2) - (B1-B3), y[3N/4] = (A0-A2) + (B1-B3).  A block is one wavefront owning 64
+// consecutive p of `rpb` rows; the twiddles stay in registers across the rows and a row's
+// coefficients, uniform over the wave, are read through the scalar cache.
+typedef const float __attribute__((address_space(4)))* UniformF;
+
+template <int KMAX>
+__device__ __forceinline__ void direct_row(const float2* grid, long row, int H, int kc, const float* C, const float* S,
+                                           bool lane0, float* o)
+{
+  const UniformF src = (UniformF)(grid + row * H);
+  float A[4] = {0.f, 0.f, 0.f, 0.f}, B[4] = {0.f, 0.f, 0.f, 0.f};
+  // KMAX < H: every load stays inside the row.  KMAX - 8 < kc <= KMAX; entries k >= kc hold
+  // stale values (the y pass skips cut columns) and are replaced by zero
+#pragma unroll
+  for (int k = 0; k < KMAX; k++) {
+    const unsigned m = (k < KMAX - 8 || k < kc) ? ~0u : 0u;  // uniform: stays on the scalar unit
+    const float a = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(src[2 * k]) & m));
+    const float b = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(src[2 * k + 1]) & m));
+    A[k & 3] = fmaf(a, C[k], A[k & 3]);
+    B[k & 3] = fmaf(b, S[k], B[k & 3]);
+  }
+  const float e02 = A[0] + A[2], e13 = A[1] + A[3], f02 = B[0] + B[2], f13 = B[1] + B[3];
+  const float sA = e02 + e13, dA = e02 - e13, sB = f02 + f13, dB = f02 - f13;
+  const float a02 = A[0] - A[2], b13 = B[1] - B[3];
+  o[0] = lane0 ? sA : sA - sB;
+  o[1] = lane0 ? dA : sA + sB;
+  o[2] = lane0 ? a02 - b13 : dA - dB;
+  o[3] = lane0 ? a02 + b13 : dA + dB;
+}
+
+template <int N, int KMAX, int EPI>
+__global__ void __launch_bounds__(64) k_row_direct(CGridPtrs in, long nrows, int rpb, const float2* __restrict__ twg,
+                                                   int kc, HiiEpi ea)
+{
+  constexpr int H = N / 2 + 1, Q = N / 4, SEGS = Q / 64;
+  static_assert(Q % 64 == 0, "N/4 must be a multiple of the wavefront");
+  static_assert(KMAX < H, "direct summation covers filtered radii only");
+  const int seg = (int)(blockIdx.x % SEGS);
+  const long row0 = (long)(blockIdx.x / SEGS) * rpb;
+  const int p = seg * 64 + (int)threadIdx.x;
+  const bool lane0 = p == 0;
+  float C[KMAX], S[KMAX];
+#pragma unroll
+  for (int k = 0; k < KMAX; k++) {
+    const float2 w = twg[(k * p) % N];  // exp(-2 pi i k p / N)
+    const float wk = k == 0 ? 1.f : 2.f;
+    C[k] = lane0 ? wk : wk * w.x;
+    S[k] = lane0 ? wk : -wk * w.y;
+  }
+  const int n[4] = {lane0 ? 0 : p, lane0 ? N / 2 : N - p, lane0 ? N / 4 : N / 2 + p, lane0 ? 3 * N / 4 : N / 2 - p};
+  const long row1 = row0 + rpb < nrows ? row0 + rpb : nrows;
+  for (long row = row0; row < row1; row++) {
+    float sfr[4], xe[4] = {0.f, 0.f, 0.f, 0.f}, st[4], dx[4];
+    direct_row<KMAX>(in.p[2], row, H, kc, C, S, lane0, sfr);  // weighted_sfr: transformed, clamped, dropped
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      float w = fmaxf(sfr[e], 0.0f);
+      w = w < ABS_TOL ? 0.0f : w;
+      asm volatile("" ::"v"(w));
+    }
+    if constexpr (EPI == C2R_HII_SPIN)
+      direct_row<KMAX>(in.p[3], row, H, kc, C, S, lane0, xe);
+    direct_row<KMAX>(in.p[1], row, H, kc, C, S, lane0, st);
+    direct_row<KMAX>(in.p[0], row, H, kc, C, S, lane0, dx);
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+      hii_cell(ea, (int)row * N + n[e], (int)row * 2 * H + n[e], dx[e], st[e], xe[e]);
+  }
+}
+
 // ---- plain-path kernels (one kernel per reference step) -------------------------------
 __global__ void k_fill(float* __restrict__ p, float v, long n)
 {
@@ -480,7 +592,31 @@ struct Hii : Passes<N> {
     return MXR_OK;
   }
 
+  // direct-summation z pass for a radius keeping kc leading k_z (kc <= 64, N a multiple of 256)
+  template <int EPI>
+  static int c2r_direct(Ctx* c, CGridPtrs in, int kc, int rpb, const HiiEpi& ea)
+  {
+    if constexpr (N % 256 != 0 || N / 2 + 1 <= 64) {
+      return MXR_ERR_UNSUPPORTED;
+    } else {
+      const long nrows = (long)N * N;
+      constexpr int SEGS = N / 4 / 64;
+      const dim3 grid((unsigned)(SEGS * ((nrows + rpb - 1) / rpb)));
+      switch ((kc + 7) / 8) {
+#define L(m)                                                                                                         \
+  case m:                                                                                                            \
+    hipLaunchKernelGGL((k_row_direct<N, 8 * m, EPI>), grid, dim3(64), 0, c->stream, in, nrows, rpb, c->tw, kc, ea);  \
+    break;
+        L(1) L(2) L(3) L(4) L(5) L(6) L(7) L(8)
+#undef L
+        default: return MXR_ERR_UNSUPPORTED;
+      }
+      MXR_HIP(hipGetLastError());
+      return MXR_OK;
+    }
+  }
 };
+
 
 
 int ensure_buffers(Ctx* c, int ngrid)
@@ -497,7 +633,7 @@ int ensure_buffers(Ctx* c, int ngrid)
   // per-block FP64 partials: up to 8 values for each of at most 8192 blocks (ew_blocks)
   const size_t need = 8 * 8192 + 6 * (size_t)(((long)c->N * c->N + 15) / 16 + 16);
   if (c->partials_cap < need) {
-    hipFree(c->partials);
+    (void)hipFree(c->partials);
     c->partials = nullptr;
     MXR_HIP(hipMalloc(&c->partials, sizeof(double) * need));
     c->partials_cap = need;
@@ -667,18 +803,25 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
     fa.apply = last ? 0 : 1;
 
     if (!plain) {
-      // kept-mode counts of this radius (all modes when not pruning)
       // kept-mode counts of this radius, for the profiler's byte counts only
       KeptModes km = (c->prof && fa.apply && fa.prune && fa.ftype == 1) ? kept_modes(N, fa.dk, fa.R)
                                                                         : KeptModes::all(N);
       // flags read (+ written on the first radius); on the last radius xH, r_bubble and temp
       // written and the deltax / temp (or Tk_box) needed for T_HI read
       const double state_b = last ? (double)nreal * (1 + 4 + 4 + 4 + 4 + 4) : (double)nreal * 1.0;
+      const int kc = kept_kz(fa);
+      const bool direct = (N % 256 == 0) && kc < H && kc <= c->zdirect_kmax && kc <= 64;
+      // live-tile grids for the x and y passes when the window cuts k_z (the kept columns then
+      // have |n_y| < kc and k_z < kc)
+      fa.live = (kc < H && c->live_tiles) ? kc : 0;
       RC(T(K_INV_X, G * 8.0 * ((double)km.k3 + (double)N * km.c2),
-           [&] { return K::template col<+1, COL_FILTER>(c, cspec, work, ngrid, K::xgeo(), fa, 1.0); }));
+           [&] { return K::template col<+1, COL_FILTER>(c, cspec, work, ngrid, K::xgeo(), fa, 1.0, c->col_tc_x); }));
       RC(T(K_INV_Y, G * 8.0 * ((double)N * km.c2 + (double)N * N * km.k1),
-           [&] { return K::template col<+1, COL_YPRUNE>(c, cwork, work, ngrid, K::ygeo(), fa, 1.0); }));
+           [&] { return K::template col<+1, COL_YPRUNE>(c, cwork, work, ngrid, K::ygeo(), fa, 1.0, c->col_tc_y); }));
       RC(T(K_INV_Z, G * 8.0 * (double)N * N * km.k1 + state_b, [&] {
+        if (direct)
+          return spin ? K::template c2r_direct<C2R_HII_SPIN>(c, cwork, kc, c->zdirect_rpb, ea)
+                      : K::template c2r_direct<C2R_HII>(c, cwork, kc, c->zdirect_rpb, ea);
         return spin ? K::template c2r<C2R_HII_SPIN>(c, cwork, ngrid, fa, ea)
                     : K::template c2r<C2R_HII>(c, cwork, ngrid, fa, ea);
       }));

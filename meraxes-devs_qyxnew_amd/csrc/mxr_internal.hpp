@@ -37,6 +37,14 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
       return ::mxr::hip_fail(e_, #call, __FILE__, __LINE__);                                                         \
   } while (0)
 
+// propagate a nonzero status
+#define RC(x)                                                                                                        \
+  do {                                                                                                               \
+    int rc_ = (x);                                                                                                   \
+    if (rc_)                                                                                                         \
+      return rc_;                                                                                                    \
+  } while (0)
+
 // Kernel classes timed by the profiler.
 enum KClass {
   K_FWD_R2C = 0,  // z-pass r2c of the inputs
@@ -108,6 +116,11 @@ struct Ctx {
   // batched find_HII_bubbles: enqueue without timing events or a final synchronisation (so the
   // sequence can be captured into a graph); the globals follow from sums_host afterwards
   bool enqueue_only = false;
+  // launch shapes (defaults, or the autotuner's choice)
+  int zdirect_kmax = 16;           // z pass by direct summation when a radius keeps <= this many k_z
+  int zdirect_rpb = 4;             // rows per wavefront of the direct z pass
+  int live_tiles = 1;              // x/y pass grids over the tiles that can hold kept columns
+  int col_tc_x = 16, col_tc_y = 16; // columns per workgroup of the per-radius x and y passes
   double fin_total_n_cells = 0, fin_sfr_conv = 0;
   int fin_n_radii = 0;
 };
@@ -136,6 +149,16 @@ int prof_begin(Ctx* c);                              // returns the event index 
 int prof_end(Ctx* c, int ev, int cls, double bytes); // records the launch
 int prof_resolve(Ctx* c);                            // after a stream sync
 const char* kclass_name(int cls);
+
+// launch-shape tuning and its wisdom file (mxr_tune.hip)
+int tuning_check(const mxr_tuning* t);
+void tuning_get(const Ctx& c, mxr_tuning* t);
+void tuning_set(Ctx& c, const mxr_tuning& t);
+int device_key(const Ctx& c, std::string& arch, int& cus);
+int wisdom_lookup(const char* path, const char* arch, int cus, int N, int filter, int spin, mxr_tuning* t);
+int wisdom_store(const char* path, const char* arch, int cus, int N, int filter, int spin, const mxr_tuning& t,
+                 double ms);
+int autotune(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_tuning* best, double* best_ms);
 
 }  // namespace mxr
 

@@ -37,6 +37,7 @@ struct FiltArgs {
   int ftype;
   int apply;
   int prune;  // skip the modes the sharp-k window zeroes
+  int live;   // > 0: the grid covers only tiles that can hold kept columns, `live` = kept k_z count
 };
 
 // COL_SCALE divides by `div` in double (find_HII_bubbles.c:235-253), COL_FSCALE by (float)div
@@ -96,8 +97,23 @@ __global__ void __launch_bounds__(NT) k_col(CGridPtrs in, GridPtrs out, ColGeom 
   const int g = blockIdx.y;
   const int c = threadIdx.x % TC;
   const int t = threadIdx.x / TC;
-  const long col = (long)blockIdx.x * TC + c;
-  const bool valid = col < geo.ncol;
+  long col = (long)blockIdx.x * TC + c;
+  bool valid = col < geo.ncol;
+  if constexpr (MODE == COL_FILTER || MODE == COL_YPRUNE) {
+    if (fa.live > 0) {
+      // live-tile grid: blocks enumerate (outer index, k_z tile) pairs whose k_z tile starts
+      // below the kept count; for the x pass the outer index is j with |n_y| < live
+      const int H = fa.N / 2 + 1;
+      const int nkt = (fa.live + TC - 1) / TC;
+      const int jj = (int)(blockIdx.x / nkt), kt = (int)(blockIdx.x % nkt);
+      int outer = jj;
+      if constexpr (MODE == COL_FILTER)
+        outer = jj < fa.live ? jj : fa.N - (2 * fa.live - 1 - jj);
+      const int k = kt * TC + c;
+      col = (long)outer * H + k;
+      valid = k < H;
+    }
+  }
   const long base = valid ? (col / geo.cpp) * geo.ps + (col % geo.cpp) : 0;
   for (int m = threadIdx.x; m < L; m += NT)
     tw[m] = twg[m];
@@ -344,13 +360,6 @@ static inline unsigned ew_blocks(long n)
   return (unsigned)(b < 1 ? 1 : b);
 }
 
-#define RC(x)                                                                                                        \
-  do {                                                                                                               \
-    int rc_ = (x);                                                                                                   \
-    if (rc_)                                                                                                         \
-      return rc_;                                                                                                    \
-  } while (0)
-
 // Modes the sharp-k window keeps at radius R (host restatement of sharpk_keep, used only to
 // count algorithmic bytes): k1 = kept k_z on the k_x = k_y = 0 line, c2 = kept (k_y, k_z)
 // columns at k_x = 0, k3 = kept modes overall.
@@ -403,6 +412,19 @@ static inline KeptModes kept_modes(int N, float dk, float R)
   return km;
 }
 
+// kept k_z on the k_x = k_y = 0 line at this radius (a prefix: |k| grows with k_z); H when the
+// window is not applied or does not zero modes
+static inline int kept_kz(const FiltArgs& fa)
+{
+  const int H = fa.N / 2 + 1;
+  if (!(fa.apply && fa.prune && fa.ftype == 1))
+    return H;
+  int kc = 0;
+  while (kc < H && keep_host(0, 0, kc, fa.N, fa.dk, fa.R))
+    kc++;
+  return kc;
+}
+
 // grid dimensions the library instantiates
 #define MXR_DIMS(X) X(16) X(32) X(64) X(96) X(128) X(160) X(256) X(512)
 
@@ -414,14 +436,33 @@ struct Passes {
   static ColGeom xgeo() { return ColGeom{(long)N * C::H, 0, (long)N * C::H, (long)N * C::H}; }
   static ColGeom ygeo() { return ColGeom{(long)C::H, (long)N * C::H, (long)C::H, (long)N * C::H}; }
 
-  template <int S, int MODE>
-  static int col(Ctx* c, CGridPtrs in, GridPtrs out, int ngrid, ColGeom geo, FiltArgs fa, double div)
+  // one column pass with TC columns per workgroup (the per-column arithmetic does not depend on TC)
+  template <int S, int MODE, int TC>
+  static int col_tc(Ctx* c, CGridPtrs in, GridPtrs out, int ngrid, ColGeom geo, FiltArgs fa, double div)
   {
-    dim3 grid((unsigned)((geo.ncol + C::TC - 1) / C::TC), (unsigned)ngrid);
-    hipLaunchKernelGGL((k_col<N, C::COL_NT, C::TC, S, MODE>), grid, dim3(C::COL_NT), 0, c->stream, in, out, geo, c->tw,
-                       fa, div);
+    constexpr int NT = TC * C::COL_T;
+    long nblk = (geo.ncol + TC - 1) / TC;
+    if ((MODE == COL_FILTER || MODE == COL_YPRUNE) && fa.live > 0) {
+      const long nkt = (fa.live + TC - 1) / TC;
+      nblk = (MODE == COL_FILTER ? 2L * fa.live - 1 : (long)N) * nkt;
+    }
+    dim3 grid((unsigned)nblk, (unsigned)ngrid);
+    hipLaunchKernelGGL((k_col<N, NT, TC, S, MODE>), grid, dim3(NT), 0, c->stream, in, out, geo, c->tw, fa, div);
     MXR_HIP(hipGetLastError());
     return MXR_OK;
+  }
+
+  // the per-radius inverse passes take their tile width from `tc` (4, 8 or 16); the others use 16
+  template <int S, int MODE>
+  static int col(Ctx* c, CGridPtrs in, GridPtrs out, int ngrid, ColGeom geo, FiltArgs fa, double div, int tc = C::TC)
+  {
+    if constexpr (S > 0 && (MODE == COL_FILTER || MODE == COL_YPRUNE)) {
+      if (tc == 4)
+        return col_tc<S, MODE, 4>(c, in, out, ngrid, geo, fa, div);
+      if (tc == 8)
+        return col_tc<S, MODE, 8>(c, in, out, ngrid, geo, fa, div);
+    }
+    return col_tc<S, MODE, C::TC>(c, in, out, ngrid, geo, fa, div);
   }
 
   static int r2c(Ctx* c, CRealPtrs in, GridPtrs out, int ngrid)

@@ -626,11 +626,13 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
         fa.R = (float)pl.R[r];
         fa.apply = r > 0 ? 1 : 0;
         const KeptModes km = (c->prof && fa.apply && fa.prune) ? kept_modes(N, fa.dk, fa.R) : KeptModes::all(N);
+        const int kc = kept_kz(fa);
+        fa.live = (kc < N / 2 + 1 && c->live_tiles) ? kc : 0;  // live-tile x / y grids
         TsStore st{{c->ts_store + (size_t)r * nreal, c->ts_store + ((size_t)F + r) * nreal}};
         RC(T(K_TS_INV_X, 2 * 8.0 * ((double)km.k3 + (double)N * km.c2),
-             [&] { return K::template col<+1, COL_FILTER>(c, cspec, work, 2, K::xgeo(), fa, 1.0); }));
+             [&] { return K::template col<+1, COL_FILTER>(c, cspec, work, 2, K::xgeo(), fa, 1.0, c->col_tc_x); }));
         RC(T(K_TS_INV_Y, 2 * 8.0 * ((double)N * km.c2 + (double)N * N * km.k1),
-             [&] { return K::template col<+1, COL_YPRUNE>(c, cwork, work, 2, K::ygeo(), fa, 1.0); }));
+             [&] { return K::template col<+1, COL_YPRUNE>(c, cwork, work, 2, K::ygeo(), fa, 1.0, c->col_tc_y); }));
         RC(T(K_TS_INV_Z, 2 * (8.0 * (double)N * N * km.k1 + 4.0 * nreal), [&] {
           const long nrows = (long)N * N;
           dim3 grid((unsigned)((nrows + Cfg<N>::TR - 1) / Cfg<N>::TR), 2u);

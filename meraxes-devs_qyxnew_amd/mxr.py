@@ -17,7 +17,7 @@ from dataclasses import dataclass
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG_DIR, "libmxr.so")
+LIB_PATH = os.environ.get("MXR_LIB") or os.path.join(PKG_DIR, "libmxr.so")  # MXR_LIB: another build
 TABLES_DIR = os.path.join(PKG_DIR, "tables")
 
 MXR_OK = 0
@@ -28,6 +28,7 @@ MXR_ERR_OOM = -4
 MXR_ERR_TABLES = -5
 MXR_ERR_NO_DEVICE = -6
 MXR_ERR_FILTER = -7
+MXR_ERR_NO_WISDOM = -8
 
 MXR_MEM_HOST = 0
 MXR_MEM_DEVICE = 1
@@ -93,6 +94,15 @@ class HiiGrids(ctypes.Structure):
         ("temp_kinetic_all_gas", ctypes.c_void_p),
         ("storage", ctypes.c_int),
     ]
+
+
+class Tuning(ctypes.Structure):
+    """mxr_tuning: launch shapes of the fused path (include/mxr.h)."""
+    _fields_ = [("zdirect_kmax", ctypes.c_int), ("zdirect_rpb", ctypes.c_int), ("live_tiles", ctypes.c_int),
+                ("col_tc_x", ctypes.c_int), ("col_tc_y", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class HiiGlobals(ctypes.Structure):
@@ -384,6 +394,10 @@ def lib():
     L.mxr_last_error.restype = ctypes.c_char_p
     L.mxr_set_path.argtypes = [vp, ctypes.c_int]
     L.mxr_set_path.restype = ctypes.c_int
+    L.mxr_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    L.mxr_get_tuning.restype = ctypes.c_int
+    L.mxr_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    L.mxr_set_tuning.restype = ctypes.c_int
     L.mxr_find_hii_bubbles.argtypes = [vp, ctypes.POINTER(HiiParams), ctypes.POINTER(HiiGrids),
                                        ctypes.POINTER(HiiGlobals)]
     L.mxr_find_hii_bubbles.restype = ctypes.c_int
@@ -412,6 +426,21 @@ def lib():
     L.mxr_find_hii_bubbles_batched.argtypes = [vp, ctypes.POINTER(HiiParams), ctypes.POINTER(HiiGrids),
                                                ctypes.POINTER(HiiGlobals), ctypes.c_int]
     L.mxr_find_hii_bubbles_batched.restype = ctypes.c_int
+    cp = ctypes.c_char_p
+    L.mxr_autotune.argtypes = [vp, ctypes.POINTER(HiiParams), ctypes.POINTER(HiiGrids), cp, ctypes.POINTER(Tuning)]
+    L.mxr_autotune.restype = ctypes.c_int
+    L.mxr_load_wisdom.argtypes = [vp, ctypes.POINTER(HiiParams), cp]
+    L.mxr_load_wisdom.restype = ctypes.c_int
+    L.mxr_wisdom_lookup.argtypes = [cp, cp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(Tuning)]
+    L.mxr_wisdom_lookup.restype = ctypes.c_int
+    L.mxr_wisdom_store.argtypes = [cp, cp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(Tuning), ctypes.c_double]
+    L.mxr_wisdom_store.restype = ctypes.c_int
+    L.mxr_batch_autotune.argtypes = [vp, ctypes.POINTER(HiiParams), ctypes.POINTER(HiiGrids), cp]
+    L.mxr_batch_autotune.restype = ctypes.c_int
+    L.mxr_batch_load_wisdom.argtypes = [vp, ctypes.POINTER(HiiParams), cp]
+    L.mxr_batch_load_wisdom.restype = ctypes.c_int
     L.mxr_profile_enable.argtypes = [vp, ctypes.c_int]
     L.mxr_profile_enable.restype = ctypes.c_int
     L.mxr_profile_read.argtypes = [vp, ctypes.POINTER(KernelStat), ctypes.c_int]
@@ -509,6 +538,42 @@ class Context:
     def set_path(self, path: int):
         check(lib().mxr_set_path(self._h, path), "mxr_set_path")
 
+    def tuning(self) -> "Tuning":
+        t = Tuning()
+        check(lib().mxr_get_tuning(self._h, ctypes.byref(t)), "mxr_get_tuning")
+        return t
+
+    def set_tuning(self, **fields) -> "Tuning":
+        """Change the named launch-shape fields (mxr_tuning); returns the previous setting."""
+        old = self.tuning()
+        t = self.tuning()
+        for k, v in fields.items():
+            if k not in dict(Tuning._fields_):
+                raise KeyError(k)
+            setattr(t, k, v)
+        check(lib().mxr_set_tuning(self._h, ctypes.byref(t)), "mxr_set_tuning")
+        return old
+
+    def restore_tuning(self, t: "Tuning"):
+        check(lib().mxr_set_tuning(self._h, ctypes.byref(t)), "mxr_set_tuning")
+
+    def autotune(self, params: HiiParams, wisdom: str | None = None, **arrays) -> "Tuning":
+        """mxr_autotune on these input grids (state grids are not needed or touched)."""
+        g = hii_grids(**_with_dummy_state(arrays))
+        if g.mem == MXR_MEM_DEVICE:
+            _sync_torch()
+        t = Tuning()
+        check(lib().mxr_autotune(self._h, ctypes.byref(params), ctypes.byref(g), _path(wisdom), ctypes.byref(t)),
+              "mxr_autotune")
+        return t
+
+    def load_wisdom(self, params: HiiParams, wisdom: str) -> int:
+        """mxr_load_wisdom; returns MXR_OK or MXR_ERR_NO_WISDOM (other errors raise)."""
+        rc = lib().mxr_load_wisdom(self._h, ctypes.byref(params), _path(wisdom))
+        if rc not in (MXR_OK, MXR_ERR_NO_WISDOM):
+            check(rc, "mxr_load_wisdom")
+        return rc
+
     def find_hii_bubbles(self, params: HiiParams, **arrays) -> HiiGlobals:
         g = hii_grids(**arrays)
         if g.mem == MXR_MEM_DEVICE:
@@ -579,6 +644,36 @@ class Context:
             pass
 
 
+def _path(p):
+    return None if p is None else os.fsencode(p)
+
+
+def _with_dummy_state(arrays: dict) -> dict:
+    """Input grids plus placeholder state pointers for calls that never touch the state (the
+    autotuner runs on its own scratch state)."""
+    a = dict(arrays)
+    ref = a["deltax"]
+    for k in ("xH", "z_at_ionization", "J_21_at_ionization", "r_bubble", "temp_kinetic_all_gas"):
+        a.setdefault(k, ref)
+    return a
+
+
+def wisdom_lookup(path: str, arch: str, cus: int, grid_dim: int, filter_type: int, spin: int = 0):
+    """The tuning a wisdom file holds for this key, or None."""
+    t = Tuning()
+    rc = lib().mxr_wisdom_lookup(_path(path), arch.encode(), cus, grid_dim, filter_type, spin, ctypes.byref(t))
+    if rc == MXR_ERR_NO_WISDOM:
+        return None
+    check(rc, "mxr_wisdom_lookup")
+    return t
+
+
+def wisdom_store(path: str, arch: str, cus: int, grid_dim: int, filter_type: int, spin: int, tuning: "Tuning",
+                 ms: float = 0.0):
+    check(lib().mxr_wisdom_store(_path(path), arch.encode(), cus, grid_dim, filter_type, spin, ctypes.byref(tuning),
+                                 ms), "mxr_wisdom_store")
+
+
 class Batch:
     """A batch of independent find_HII_bubbles realisations (grid dimensions may differ), run as
     one captured hipGraph (MXR_BATCH_GRAPH), on one stream each (MXR_BATCH_STREAMS) or one at a
@@ -601,6 +696,21 @@ class Batch:
         _sync_torch()
         check(lib().mxr_find_hii_bubbles_batched(self._h, P, G, out, mode), "mxr_find_hii_bubbles_batched")
         return list(out)
+
+    def autotune(self, params_list, arrays_list, wisdom: str | None = None):
+        n = len(self.dims)
+        P = (HiiParams * n)(*params_list)
+        G = (HiiGrids * n)(*[hii_grids(**_with_dummy_state(a)) for a in arrays_list])
+        _sync_torch()
+        check(lib().mxr_batch_autotune(self._h, P, G, _path(wisdom)), "mxr_batch_autotune")
+
+    def load_wisdom(self, params_list, wisdom: str) -> int:
+        n = len(self.dims)
+        P = (HiiParams * n)(*params_list)
+        rc = lib().mxr_batch_load_wisdom(self._h, P, _path(wisdom))
+        if rc not in (MXR_OK, MXR_ERR_NO_WISDOM):
+            check(rc, "mxr_batch_load_wisdom")
+        return rc
 
     def close(self):
         if self._h:

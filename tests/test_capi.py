@@ -64,4 +64,7 @@ def test_null_arguments_are_errors():
     L = mxr.lib()
     assert L.mxr_find_hii_bubbles(None, None, None, None) == mxr.MXR_ERR_INVALID_ARG
     assert L.mxr_set_path(None, 0) == mxr.MXR_ERR_INVALID_ARG
+    t = mxr.Tuning()
+    assert L.mxr_get_tuning(None, ctypes.byref(t)) == mxr.MXR_ERR_INVALID_ARG
+    assert L.mxr_set_tuning(None, ctypes.byref(t)) == mxr.MXR_ERR_INVALID_ARG
     assert L.mxr_profile_enable(None, 1) == mxr.MXR_ERR_INVALID_ARG

@@ -63,17 +63,62 @@ def test_parity_vs_oracle(ctxs, N, box, ftype):
                - 1) < 1e-4 + st["mismatch"] / N ** 3 * 10
 
 
-def test_parity_256(ctxs):
+@pytest.mark.parametrize("zdirect_kmax", [64, 0])
+def test_parity_256(ctxs, zdirect_kmax):
+    """256^3 sharp-k against the oracle, with the z pass by direct summation on the radii that
+    keep few k_z (64) and by the row FFT on every radius (0)."""
     N, box = 256, 200.0
-    print("parity N=256", flush=True)
+    print(f"parity N=256 zdirect_kmax={zdirect_kmax}", flush=True)
     inp = realisation(N, box, 1)
     p = mxr.hii_params(N, box, filter_type=1)
-    sg, sr = _state(N), _state(N)
-    gg = _run_gpu(ctxs(N), p, inp, sg)
+    ctx = ctxs(N)
+    old = ctx.set_tuning(zdirect_kmax=zdirect_kmax)
+    try:
+        sg, sr = _state(N), _state(N)
+        gg = _run_gpu(ctx, p, inp, sg)
+    finally:
+        ctx.restore_tuning(old)
     rg, margin = O.find_hii_bubbles(p, inp, sr, margin=True)
     st = compare(N, _as_dict(sg), _as_dict(sr), gg, rg, margin)
     print(st)
     check(st)
+
+
+def test_direct_z_pass_error_vs_fp64_256(ctxs):
+    """The direct-summation z pass is no less accurate than a row FFT: at 256^3 its per-cell
+    partial-xH error against the FP64 oracle stays within twice the FP32 restatement's, and so
+    does the GPU's row-FFT form."""
+    N, box = 256, 200.0
+    print("direct z pass vs FP64 N=256", flush=True)
+    inp = realisation(N, box, 1)
+    p = mxr.hii_params(N, box, filter_type=1)
+    ctx = ctxs(N)
+    res = {}
+    for kmax in (64, 0):
+        old = ctx.set_tuning(zdirect_kmax=kmax)
+        try:
+            st = _state(N)
+            res[kmax] = (_run_gpu(ctx, p, inp, st), np.asarray(st.xH).copy())
+        finally:
+            ctx.restore_tuning(old)
+    s32, s64 = _state(N), _state(N)
+    g32 = O.find_hii_bubbles(p, inp, s32)
+    g64, margin = O.find_hii_bubbles(p, inp, s64, precision="f64", margin=True)
+    m32 = (s32.xH == 0) != (s64.xH == 0)
+    for kmax, (gg, xh) in res.items():
+        both = (xh > 0) & (s32.xH > 0) & (s64.xH > 0)
+        e_gpu = np.abs(xh[both] - s64.xH[both])
+        e_32 = np.abs(s32.xH[both] - s64.xH[both])
+        mg = (xh == 0) != (s64.xH == 0)
+        print(f"zdirect_kmax={kmax}: xH error vs FP64 max {e_gpu.max():.3e} p99.9 {np.quantile(e_gpu, 0.999):.3e} "
+              f"flags {int(mg.sum())} | f32 restatement max {e_32.max():.3e} p99.9 {np.quantile(e_32, 0.999):.3e} "
+              f"flags {int(m32.sum())}", flush=True)
+        assert e_gpu.max() <= 2 * e_32.max() + 1e-6
+        assert np.quantile(e_gpu, 0.999) <= 2 * np.quantile(e_32, 0.999) + 1e-6
+        assert mg.sum() <= max(2, 2 * m32.sum())
+        assert np.all(np.abs(margin[mg]) < 1e-3)
+        assert abs(gg.volume_weighted_global_xH - g64.volume_weighted_global_xH) <= \
+            2 * abs(g32.volume_weighted_global_xH - g64.volume_weighted_global_xH) + 1e-6
 
 
 def test_spin_temp_grid(ctxs):
@@ -117,7 +162,9 @@ def test_paths_bit_identical(ctxs, ftype):
 
 
 def test_pruned_equals_dense_512(ctxs):
-    """At the benchmark size the mode-skipping path equals the dense fused path bit for bit."""
+    """At the benchmark size the mode-skipping path with the row-FFT z pass equals the dense
+    fused path bit for bit; with the direct-summation z pass it differs only by rounding: flag
+    mismatch within the parity bound, global xH within 1e-5 plus the flipped cells' share."""
     import synth
     import torch
 
@@ -125,21 +172,35 @@ def test_pruned_equals_dense_512(ctxs):
     print("512 pruned vs dense", flush=True)
     inp = {k: v.contiguous() for k, v in synth.realisation(N, box, 42, device="cuda", stars_to_matter=0.05).items()}
     p = mxr.hii_params(N, box, filter_type=1)
+    n = N ** 3
     res = []
-    for path in (mxr.MXR_PATH_FUSED, mxr.MXR_PATH_FUSED_DENSE):
-        n = N ** 3
+    for path, kmax in ((mxr.MXR_PATH_FUSED, 0), (mxr.MXR_PATH_FUSED_DENSE, 0), (mxr.MXR_PATH_FUSED, 64)):
         st = mxr.HiiState(torch.ones(n, device="cuda"), torch.full((n,), -1.0, device="cuda"),
                           torch.zeros(n, device="cuda"), torch.zeros(n, device="cuda"), torch.zeros(n, device="cuda"))
-        g = _run_gpu(ctxs(N, path), p, inp, st)
+        ctx = ctxs(N, path)
+        old = ctx.set_tuning(zdirect_kmax=kmax)
+        try:
+            g = _run_gpu(ctx, p, inp, st)
+        finally:
+            ctx.restore_tuning(old)
         torch.cuda.synchronize()
         res.append((st, g))
-    (a, ga), (b, gb) = res
+    (a, ga), (b, gb), (d, gd) = res
     for k in ("xH", "z_at_ionization", "J_21_at_ionization", "temp_kinetic_all_gas"):
         assert torch.equal(getattr(a, k), getattr(b, k)), k
-    xh = a.xH
-    assert float(xh.min()) >= 0.0 and float(xh.max()) <= 1.0
-    assert abs(ga.volume_weighted_global_xH - float(xh.double().mean())) < 1e-9
-    print("512 xH", ga.volume_weighted_global_xH, "ms", ga.device_ms, gb.device_ms)
+    for st, g in ((a, ga), (d, gd)):
+        xh = st.xH
+        assert float(xh.min()) >= 0.0 and float(xh.max()) <= 1.0
+        assert abs(g.volume_weighted_global_xH - float(xh.double().mean())) < 1e-9
+    flips = int(((a.xH == 0) != (d.xH == 0)).sum())
+    agree = (a.xH == 0) == (d.xH == 0)
+    z_diff = int((a.z_at_ionization[agree] != d.z_at_ionization[agree]).sum())
+    dxh = abs(ga.volume_weighted_global_xH - gd.volume_weighted_global_xH)
+    print(f"512 xH {ga.volume_weighted_global_xH} direct {gd.volume_weighted_global_xH} flips {flips} "
+          f"({flips / n:.2e}) z_in diffs {z_diff} ms fft {ga.device_ms:.2f} dense {gb.device_ms:.2f} "
+          f"direct {gd.device_ms:.2f}", flush=True)
+    assert flips <= 5e-4 * n
+    assert dxh <= 1e-5 + flips / n
     ctxs(N, mxr.MXR_PATH_FUSED)
 
 
