@@ -79,7 +79,7 @@ def cpu_baseline(N, box, inputs_host, n_radii_sample, n_radii_total):
 def profiled_traffic(kernel_class, N):
     """HBM bytes per launch of a kernel class from the committed rocprof summary of the 512^3
     workload (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or None."""
-    path = os.path.join(REPO, "profiles", f"find_hii_bubbles_{N}_sharpk.md")
+    path = os.path.join(REPO, "profiles", f"find_hii_bubbles_{N}_sharpk_traffic.txt")
     try:
         for line in open(path):
             if line.startswith("traffic-per-launch:"):
