@@ -196,6 +196,11 @@ int mxr_find_hii_bubbles(mxr_ctx* ctx, const mxr_hii_params* params, const mxr_h
 int mxr_filter(mxr_ctx* ctx, void* box, int mem, int local_ix_start, int slab_nx, int grid_dim,
                double BoxSize, float R, int filter_type);
 
+/* velocity_gradient (src/core/reionization.c:3435-3450, declared reionization.h:88): each
+ * half-spectrum value of slab_nx x-planes times i k_z, k_z = n_z (float)(2 pi/(float)BoxSize).
+ * box: complex64 half spectrum, host or device (mem).                                      */
+int mxr_velocity_gradient(mxr_ctx* ctx, void* box, int mem, int slab_nx, int grid_dim, double BoxSize);
+
 /* Mass enclosed by a filter of radius R (internal units).  Replaces RtoM
  * (src/core/find_HII_bubbles.c:27-46, declared src/core/find_HII_bubbles.h:11).          */
 double mxr_RtoM(const mxr_hii_params* params, double R);
@@ -233,27 +238,44 @@ int mxr_batch_autotune(mxr_batch* batch, const mxr_hii_params* params, const mxr
 int mxr_batch_load_wisdom(mxr_batch* batch, const mxr_hii_params* params, const char* wisdom_path);
 
 /* ---- 21-cm brightness temperature ---------------------------------------------------------- */
+/* run_params_t.TreesID (src/meraxes.h:254-258): selects the velocity grid's units */
+#define MXR_TREES_VELOCIRAPTOR 0
+#define MXR_TREES_GBPTREES 1
+#define MXR_TREES_VELOCIRAPTOR_AUG 2
+
 typedef struct mxr_tb_params {
   int ReionGridDim;
+  double BoxSize;                 /* Mpc/h                                                   */
   double Hubble_h;
   double OmegaM;
+  double OmegaLambda;
+  double OmegaR;
   double BaryonFrac;
   double redshift;
   int Flag_IncludeSpinTemp;
-  int Flag_IncludePecVelsFor21cm; /* must be 0 (MXR_ERR_UNSUPPORTED otherwise)            */
+  int Flag_IncludePecVelsFor21cm; /* 0..3; velocity terms apply with Flag_IncludeSpinTemp    */
+  int TreesID;                    /* MXR_TREES_*                                             */
 } mxr_tb_params;
 
 typedef struct mxr_tb_grids {
   int mem;
-  const float* xH;      /* real                                  */
-  const float* deltax;  /* padded real                           */
-  const float* TS_box;  /* real (Flag_IncludeSpinTemp only)      */
-  float* delta_T;       /* out real                              */
+  const float* xH;      /* real                                                          */
+  const float* deltax;  /* padded real                                                   */
+  const float* TS_box;  /* real (Flag_IncludeSpinTemp only)                              */
+  float* vel;           /* in/out padded real, line-of-sight (z) velocity (PecVels > 0 with
+                           Flag_IncludeSpinTemp only): converted in place to comoving
+                           velocity as the reference does                                */
+  float* delta_T;       /* out real                                                      */
 } mxr_tb_grids;
 
-/* delta_T = T0 xH (1+delta) [(1 - T_CMB/T_S)] per cell and its FP64 volume mean.
+/* delta_T = T0 xH (1+delta) with the spin-temperature factor (1 - T_CMB/T_S), or, for
+ * Flag_IncludePecVelsFor21cm > 1, the optical-depth form; then the peculiar-velocity terms:
+ * the velocity gradient dv/dz from the velocity grid (r2c, /N^3, x Hubble_h,
+ * velocity_gradient, c2r), capped at 0.2 H(z) for PecVels = 1, uncapped in the optical
+ * depth for 2 and 3, and for 3 the line-of-sight redshift-space redistribution in 50
+ * sub-cells per cell.  Returns the FP64 volume mean.
  * Replaces ComputeBrightnessTemperatureBox (src/core/BrightnessTemperature.c:24-549,
- * declared src/core/BrightnessTemperature.h:9) for Flag_IncludePecVelsFor21cm = 0.       */
+ * declared src/core/BrightnessTemperature.h:9), without mini-haloes.                    */
 int mxr_brightness_temperature(mxr_ctx* ctx, const mxr_tb_params* params, const mxr_tb_grids* grids,
                                double* volume_ave_Tb);
 

@@ -551,6 +551,42 @@ int mxr_filter(mxr_ctx* h, void* box, int mem, int local_ix_start, int slab_nx, 
   return rc;
 }
 
+int mxr_velocity_gradient(mxr_ctx* h, void* box, int mem, int slab_nx, int grid_dim, double BoxSize)
+{
+  if (!h || !box || slab_nx <= 0 || slab_nx > grid_dim || grid_dim != h->c.N || !(BoxSize > 0) ||
+      (mem != MXR_MEM_HOST && mem != MXR_MEM_DEVICE)) {
+    set_error("bad velocity_gradient arguments");
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (hipSetDevice(h->c.device) != hipSuccess) {
+    set_error("hipSetDevice(%d) failed", h->c.device);
+    return MXR_ERR_HIP;
+  }
+  const float box_size = (float)BoxSize;  // reionization.c:3438-3439
+  const float dk = (float)(2.0 * M_PI / box_size);
+  Ctx& c = h->c;
+  const long n = (long)slab_nx * grid_dim * (grid_dim / 2 + 1);
+  if (mem == MXR_MEM_DEVICE)
+    return run_velocity_gradient(&c, (float2*)box, slab_nx, grid_dim, dk);
+  float2* d = nullptr;
+  MXR_HIP(hipMalloc(&d, sizeof(float2) * n));
+  int rc = MXR_OK;
+  hipError_t e = hipMemcpyAsync(d, box, sizeof(float2) * n, hipMemcpyHostToDevice, c.stream);
+  if (e == hipSuccess)
+    rc = run_velocity_gradient(&c, d, slab_nx, grid_dim, dk);
+  else
+    rc = hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
+  if (rc == MXR_OK) {
+    e = hipMemcpyAsync(box, d, sizeof(float2) * n, hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess)
+      e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess)
+      rc = hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
 // ---- batches -----------------------------------------------------------------------------
 }  // extern "C" (reopened below)
 
@@ -908,9 +944,13 @@ int mxr_brightness_temperature(mxr_ctx* h, const mxr_tb_params* p, const mxr_tb_
     set_error("bad brightness-temperature arguments");
     return MXR_ERR_INVALID_ARG;
   }
-  if (p->Flag_IncludePecVelsFor21cm) {
-    set_error("Flag_IncludePecVelsFor21cm > 0 is not implemented by this library");
-    return MXR_ERR_UNSUPPORTED;
+  if (p->Flag_IncludePecVelsFor21cm < 0 || p->Flag_IncludePecVelsFor21cm > 3) {
+    set_error("Flag_IncludePecVelsFor21cm %d is not in 0..3", p->Flag_IncludePecVelsFor21cm);
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (p->Flag_IncludeSpinTemp && p->Flag_IncludePecVelsFor21cm > 0 && (!g->vel || !(p->BoxSize > 0))) {
+    set_error("peculiar-velocity terms need the velocity grid and BoxSize > 0");
+    return MXR_ERR_INVALID_ARG;
   }
   if (hipSetDevice(h->c.device) != hipSuccess) {
     set_error("hipSetDevice(%d) failed", h->c.device);

@@ -548,6 +548,8 @@ struct TbArgs {
   float const_factor;
   double T_rad;
   int spin;
+  int tau;   // PecVels > 1: optical-depth form (1 + z) / (1000 T_S)
+  float z;
   const float* xH;
   const float* deltax;
   const float* TS;
@@ -568,13 +570,181 @@ __global__ void __launch_bounds__(NT) k_tb(TbArgs a)
     const double pixel_x_HI = a.xH[ir];
     float dT = (float)((double)a.const_factor * pixel_x_HI * (1.0 + pixel_deltax));
     if (a.spin) {
-      const double f = (1 - a.T_rad / (double)a.TS[ir]);
+      const double f = a.tau ? (1. + (double)a.z) / (1000. * (double)a.TS[ir]) : (1 - a.T_rad / (double)a.TS[ir]);
       dT = (float)((double)dT * f);
     }
     a.dT[ir] = dT;
     acc[0] += (double)dT;
   }
   block_sum<NT, 1>(acc, a.partials + blockIdx.x);
+}
+
+// ---- peculiar-velocity terms (BrightnessTemperature.c:158-510), each in the reference's types
+
+// velocity grid to comoving velocity, in place on the N^3 cells of the padded grid (:169-198)
+__global__ void k_tb_velconv(float* __restrict__ vel, int N, int trees, float z)
+{
+#pragma clang fp contract(off)
+  const int H = N / 2 + 1;
+  const long n = (long)N * N * N;
+  const double opz = 1. + (double)z;
+  for (long ir = (long)blockIdx.x * blockDim.x + threadIdx.x; ir < n; ir += (long)gridDim.x * blockDim.x) {
+    const long ip = (ir / N) * 2 * H + ir % N;
+    float v = vel[ip];
+    if (trees == MXR_TREES_GBPTREES)
+      v = (float)(sqrt(1. / opz) * (double)v / 1000.);
+    else if (trees == MXR_TREES_VELOCIRAPTOR || trees == MXR_TREES_VELOCIRAPTOR_AUG)
+      v = (float)(opz * (double)v);
+    v = (float)((double)v * (1000. * 100. / MPC));
+    v = (float)((double)v / (1. / opz));
+    vel[ip] = v;
+  }
+}
+
+// optionally x Hubble_h (double product, float result), then velocity_gradient
+// (reionization.c:3435-3450): (a + ib)(0 + i k_z), k_z = n_z dk in float
+__global__ void k_vgrad(float2* __restrict__ s, long n, int H, float dk, double h, int scale_h)
+{
+#pragma clang fp contract(off)
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    float2 x = s[i];
+    if (scale_h)
+      x = make_float2((float)((double)x.x * h), (float)((double)x.y * h));
+    const float kz = (float)(int)(i % H) * dk;
+    s[i] = make_float2(x.x * 0.0f - x.y * kz, x.x * kz + x.y * 0.0f);
+  }
+}
+
+struct TbVelArgs {
+  int N, pecvel;
+  float H_z, z;
+  double T_rad, max_v_deriv;
+  const float* grad;  // dv/dz, padded real
+  const float* TS;
+  float* dT;
+};
+
+// PecVels 1: capped gradient (:217-248); 2 and 3: optical depth with the uncapped gradient (:250-309)
+__global__ void k_tb_pecvel(TbVelArgs a)
+{
+#pragma clang fp contract(off)
+  const int N = a.N, H = N / 2 + 1;
+  const long n = (long)N * N * N;
+  const double Hz = (double)a.H_z, opz = 1. + (double)a.z;
+  for (long ir = (long)blockIdx.x * blockDim.x + threadIdx.x; ir < n; ir += (long)gridDim.x * blockDim.x) {
+    const long ip = (ir / N) * 2 * H + ir % N;
+    const double g = (double)a.grad[ip];
+    if (a.pecvel == 1) {
+      double dvdx = g;
+      if (fabs(dvdx) > a.max_v_deriv)
+        dvdx = dvdx < 0 ? -a.max_v_deriv : a.max_v_deriv;
+      a.dT[ir] = (float)((double)a.dT[ir] / (dvdx / Hz + 1.0));
+    } else {
+      const double gc = fabs(g / Hz + 1.0);
+      const double TS = (double)a.TS[ir];
+      if (gc < 1e-7)
+        a.dT[ir] = (float)(1000. * (TS - a.T_rad) / opz);
+      else
+        a.dT[ir] = (float)((1. - exp(-(double)a.dT[ir] / gc)) * 1000. * (TS - a.T_rad) / opz);
+    }
+  }
+}
+
+constexpr int N_RSD_STEPS = 50;  // reionization.h:34
+struct TbRsdArgs {
+  int N;
+  float H_z;
+  double sw;                        // sub-cell width (BoxSize/h / N / 50)
+  double cw;                        // BoxSize/h / (float)N
+  float x_pos[N_RSD_STEPS], x_off[N_RSD_STEPS];
+  float* dT;
+  const float* vel;                 // comoving velocity, padded
+  const float* xH;
+  float* los;                       // N^3 scratch
+};
+
+__device__ __forceinline__ int wrap_cell(long k, int N) { return (int)(((k % N) + N) % N); }
+
+// PecVels 3: line-of-sight redshift-space redistribution (:311-510).  One thread per (i, j)
+// row walks the cells and sub-cells in the reference's order, so every destination sums its
+// contributions in the same order; destinations wrap modulo N (identical to the reference's
+// periodic index wherever that stays inside the row).
+__global__ void __launch_bounds__(64) k_tb_rsd(TbRsdArgs a)
+{
+#pragma clang fp contract(off)
+  const int N = a.N, H = N / 2 + 1;
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (long)N * N)
+    return;
+  float* los = a.los + r * N;
+  float* dT = a.dT + r * N;
+  const float* v = a.vel + r * 2 * H;
+  const float* xH = a.xH + r * N;
+  for (int k = 0; k < N; k++)
+    los[k] = 0.0f;
+  const double fw_unit = a.sw / a.cw;
+  const double half = (a.sw / 2.) / a.cw;
+  for (int kk = 0; kk < N; kk++) {
+    const float d = dT[kk];
+    if (!(fabs((double)d) >= 1e-7 && (double)xH[kk] >= 1e-7))
+      continue;
+    const double d1_low = (double)__fdiv_rn(v[kk == 0 ? N - 1 : kk - 1], a.H_z);
+    const double d2_low = (double)__fdiv_rn(v[kk], a.H_z);
+    const double d1_high = d2_low;
+    const double d2_high = (double)__fdiv_rn(v[kk == N - 1 ? 0 : kk + 1], a.H_z);
+    const float d50 = __fdiv_rn(d, (float)N_RSD_STEPS);
+    for (int s = 0; s < N_RSD_STEPS; s++) {
+      const double xp = (double)a.x_pos[s];
+      const double disp = xp <= 0.5 ? d1_low + ((xp + 0.5) - 0.) * (d2_low - d1_low) / (1. - 0.)
+                                    : d1_high + ((xp - 0.5) - 0.) * (d2_high - d1_high) / (1. - 0.);
+      const double pos = ((double)a.x_off[s] + disp) / a.cw;
+      const double lo = pos - half, hi = pos + half;
+      auto add = [&](long k, double f) { los[wrap_cell(k, N)] = (float)((double)los[wrap_cell(k, N)] + f * (double)d / (double)N_RSD_STEPS); };
+      if (lo >= 0.0 && hi < 1.0) {
+        los[kk] = los[kk] + d50;
+      } else if (lo < 0.0 && hi < 0.0) {
+        const double cd = ceil(fabs(lo)) - 1.;
+        if (fabs(hi) > cd) {
+          const int k = wrap_cell(kk - ((long)cd + 1), N);
+          los[k] = los[k] + d50;
+        } else {
+          const double fo = (fabs(lo) - cd) / fw_unit;
+          add(kk - (long)cd, 1. - fo);
+          add(kk - ((long)cd + 1), fo);
+        }
+      } else if (lo < 0.0 && (hi > 0.0 && hi < 1.0)) {
+        const double fwi = hi / fw_unit;
+        add(kk - 1, 1. - fwi);
+        add(kk, fwi);
+      } else if ((lo >= 0.0 && lo < 1.0) && (hi >= 1.0)) {
+        const double fo = (hi - 1.) / fw_unit;
+        add(kk, 1. - fo);
+        add(kk + 1, fo);
+      } else {
+        const double cd = floor(fabs(hi));
+        if (lo >= cd) {
+          const int k = wrap_cell(kk + (long)cd, N);
+          los[k] = los[k] + d50;
+        } else {
+          const double fo = (hi - cd) / fw_unit;
+          add(kk + (long)cd - 1, 1. - fo);
+          add(kk + (long)cd, fo);
+        }
+      }
+    }
+  }
+  for (int k = 0; k < N; k++)
+    dT[k] = los[k];
+}
+
+// FP64 block sums of a real grid
+template <int NT>
+__global__ void __launch_bounds__(NT) k_sum_f(const float* __restrict__ x, long n, double* partials)
+{
+  double acc[1] = {0.0};
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT)
+    acc[0] += (double)x[i];
+  block_sum<NT, 1>(acc, partials + blockIdx.x);
 }
 
 template <int N>
@@ -941,20 +1111,72 @@ int run_filter(Ctx* c, float2* box, int ix0, int nx, int N, float dk, float R, i
   return MXR_OK;
 }
 
+int run_velocity_gradient(Ctx* c, float2* box, int slab_nx, int grid_dim, float dk)
+{
+  const int H = grid_dim / 2 + 1;
+  const long n = (long)slab_nx * grid_dim * H;
+  hipLaunchKernelGGL(k_vgrad, dim3(ew_blocks(n)), dim3(256), 0, c->stream, box, n, H, dk, 1.0, 0);
+  MXR_HIP(hipGetLastError());
+  MXR_HIP(hipStreamSynchronize(c->stream));
+  return MXR_OK;
+}
+
+// hubble(float z) (XRayHeatingFunctions.c:443-453): 1 + z is a float sum
+static double tb_hubble(const mxr_tb_params* p, float z)
+{
+  const float opz = 1 + z;
+  return HUBBLE * p->Hubble_h *
+         std::sqrt(p->OmegaM * std::pow((double)opz, 3) + p->OmegaR * std::pow((double)opz, 4) + p->OmegaLambda);
+}
+
+// dv/dz of the (converted) velocity grid into c->spec[0] as a padded real grid
+// (BrightnessTemperature.c:200-215): r2c, /N^3 and x Hubble_h, x i k_z, c2r
+template <int N>
+static int tb_velocity_gradient(Ctx* c, const float* vel, const mxr_tb_params* p)
+{
+  using K = Hii<N>;
+  const long Nc = n_complex(N);
+  CRealPtrs rin{};
+  rin.p[0] = vel;
+  GridPtrs spec{};
+  spec.p[0] = c->spec[0];
+  CGridPtrs cspec{};
+  cspec.p[0] = c->spec[0];
+  FiltArgs none{};
+  none.N = N;
+  RC(K::r2c(c, rin, spec, 1));
+  RC((K::template col<-1, COL_PLAIN>(c, cspec, spec, 1, K::ygeo(), none, 1.0)));
+  RC((K::template col<-1, COL_SCALE>(c, cspec, spec, 1, K::xgeo(), none, std::pow((double)N, 3))));
+  const float box_size = (float)p->BoxSize;  // reionization.c:3438-3439
+  const float dk = (float)(2.0 * M_PI / box_size);
+  hipLaunchKernelGGL(k_vgrad, dim3(ew_blocks(Nc)), dim3(256), 0, c->stream, c->spec[0], Nc, N / 2 + 1, dk,
+                     p->Hubble_h, 1);
+  MXR_HIP(hipGetLastError());
+  RC((K::template col<+1, COL_PLAIN>(c, cspec, spec, 1, K::xgeo(), none, 1.0)));
+  RC((K::template col<+1, COL_PLAIN>(c, cspec, spec, 1, K::ygeo(), none, 1.0)));
+  HiiEpi ea{};
+  return K::template c2r<C2R_STORE>(c, cspec, 1, none, ea);
+}
+
 int run_brightness_temperature(Ctx* c, const mxr_tb_params* p, const mxr_tb_grids* g, double* ave)
 {
   const int N = c->N;
   const long nreal = n_real(N), npad = 2 * n_complex(N);
-  RC(ensure_buffers(c, 0));
+  const int spin = p->Flag_IncludeSpinTemp ? 1 : 0;
+  const int pecvel = spin ? p->Flag_IncludePecVelsFor21cm : 0;  // velocity terms need T_S
+  RC(ensure_buffers(c, pecvel ? 2 : 0));
   const float* xH = g->xH;
   const float* dx = g->deltax;
   const float* TS = g->TS_box;
+  float* vel = g->vel;
   float* dT = g->delta_T;
   if (g->mem == MXR_MEM_HOST) {
     RC(stage_in(c, 0, xH, nreal, &xH));
     RC(stage_in(c, 1, dx, npad, &dx));
-    if (p->Flag_IncludeSpinTemp)
+    if (spin)
       RC(stage_in(c, 2, TS, nreal, &TS));
+    if (pecvel)
+      RC(stage_out(c, 1, vel, npad, &vel));
     if (!c->out_stage[0])
       MXR_HIP(hipMalloc(&c->out_stage[0], sizeof(float) * npad));
     dT = c->out_stage[0];
@@ -969,7 +1191,9 @@ int run_brightness_temperature(Ctx* c, const mxr_tb_params* p, const mxr_tb_grid
   a.const_factor = (float)(27.0 * (OmegaB * Hubble_h * Hubble_h / 0.023) *
                            std::sqrt((0.15 / OmegaM / Hubble_h / Hubble_h) * (1.0 + redshift) / 10.0));
   a.T_rad = TCMB * (1. + redshift);
-  a.spin = p->Flag_IncludeSpinTemp;
+  a.spin = spin;
+  a.tau = spin && p->Flag_IncludePecVelsFor21cm > 1;
+  a.z = redshift;
   a.xH = xH;
   a.deltax = dx;
   a.TS = TS;
@@ -977,11 +1201,63 @@ int run_brightness_temperature(Ctx* c, const mxr_tb_params* p, const mxr_tb_grid
   a.partials = c->partials;
   const int nblk = (int)ew_blocks(nreal);
   hipLaunchKernelGGL((k_tb<256>), dim3(nblk), dim3(256), 0, c->stream, a);
+  MXR_HIP(hipGetLastError());
+  if (pecvel) {
+    const float H_z = (float)(float)tb_hubble(p, redshift);
+    hipLaunchKernelGGL(k_tb_velconv, dim3(ew_blocks(nreal)), dim3(256), 0, c->stream, vel, N, p->TreesID, redshift);
+    MXR_HIP(hipGetLastError());
+    int rc = MXR_ERR_UNSUPPORTED;
+    switch (N) {
+#define X(n)                                                                                                         \
+  case n:                                                                                                            \
+    rc = tb_velocity_gradient<n>(c, vel, p);                                                                         \
+    break;
+      MXR_DIMS(X)
+#undef X
+    }
+    RC(rc);
+    TbVelArgs va{};
+    va.N = N;
+    va.pecvel = pecvel;
+    va.H_z = H_z;
+    va.z = redshift;
+    va.T_rad = a.T_rad;
+    va.max_v_deriv = std::fabs(0.2f * H_z);  // fabs(MAX_DVDR * H_z), a float product
+    va.grad = (const float*)c->spec[0];
+    va.TS = TS;
+    va.dT = dT;
+    hipLaunchKernelGGL(k_tb_pecvel, dim3(ew_blocks(nreal)), dim3(256), 0, c->stream, va);
+    MXR_HIP(hipGetLastError());
+    if (pecvel == 3) {
+      TbRsdArgs ra{};
+      ra.N = N;
+      ra.H_z = H_z;
+      const double BoxSize = p->BoxSize / p->Hubble_h;
+      ra.sw = (BoxSize / ((double)N)) / (double)N_RSD_STEPS;
+      ra.cw = BoxSize / (float)N;
+      for (int s = 0; s < N_RSD_STEPS; s++) {
+        ra.x_off[s] = (float)(ra.sw * (float)s + ra.sw / 2.);
+        ra.x_pos[s] = (float)(ra.x_off[s] / (BoxSize / (float)N));
+      }
+      ra.dT = dT;
+      ra.vel = vel;
+      ra.xH = xH;
+      ra.los = (float*)c->work[1];
+      const long rows = (long)N * N;
+      hipLaunchKernelGGL(k_tb_rsd, dim3((unsigned)((rows + 63) / 64)), dim3(64), 0, c->stream, ra);
+      MXR_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL((k_sum_f<256>), dim3(nblk), dim3(256), 0, c->stream, dT, nreal, c->partials);
+    MXR_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, c->stream, c->partials, nblk, 1, c->sums);
   MXR_HIP(hipGetLastError());
   MXR_HIP(hipMemcpyAsync(c->sums_host, c->sums, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  if (g->mem == MXR_MEM_HOST)
+  if (g->mem == MXR_MEM_HOST) {
     MXR_HIP(hipMemcpyAsync(g->delta_T, dT, sizeof(float) * nreal, hipMemcpyDeviceToHost, c->stream));
+    if (pecvel)  // the reference converts the velocity grid in place
+      MXR_HIP(hipMemcpyAsync(g->vel, vel, sizeof(float) * npad, hipMemcpyDeviceToHost, c->stream));
+  }
   MXR_HIP(hipStreamSynchronize(c->stream));
   *ave = c->sums_host[0] / std::pow((double)N, 3);
   return MXR_OK;

@@ -21,6 +21,7 @@ constexpr double PROTONMASS = 1.6726e-24;
 constexpr double HUBBLE = 3.2407789e-18;
 constexpr double SEC_PER_YEAR = 3.155e7;
 constexpr double TCMB = 2.728;
+constexpr double MPC = 3.086e24;          // cm
 constexpr double T_RE = 1e4;
 constexpr float REL_TOL = 1e-5f;
 constexpr float ABS_TOL = 1e-8f;
@@ -139,6 +140,7 @@ int check_hii_params(const mxr_hii_params* p, int N);
 bool supported_dim(int N);
 int run_find_hii_bubbles(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_hii_globals* out);
 int run_filter(Ctx* c, float2* box, int local_ix_start, int slab_nx, int grid_dim, float dk, float R, int ftype);
+int run_velocity_gradient(Ctx* c, float2* box, int slab_nx, int grid_dim, float dk);
 int run_brightness_temperature(Ctx* c, const mxr_tb_params* p, const mxr_tb_grids* g, double* ave);
 int launch_final_sum(Ctx* c, const double* partials, int nblk, int nv, double* out);  // fixed-order FP64 sum
 void hii_globals(const Ctx* c, mxr_hii_globals* out);  // after a synchronised enqueue_only call

@@ -283,7 +283,9 @@ __device__ __forceinline__ void block_sum(double* acc, double* out)
 }
 
 // One grid's rows through the inverse z-pass: load the half-spectrum rows into LDS (only the
-// kc leading k the window keeps -- a prefix, since |k| grows with k_z), pre-twiddle, M-point inverse FFT.  out receives the
+// kc leading k the window keeps -- a prefix, since |k| grows with k_z; the real parts of the
+// k_z = 0 and N/2 bins, as a c2r of the Hermitian extension uses), pre-twiddle, M-point
+// inverse FFT.  out receives the
 // last-stage slots: slot (q, r) = y[2n] + i y[2n+1], n = t + T q + r BL.  A row's T threads
 // are consecutive lanes of one wavefront, so the stages synchronise per wave, not per block.
 template <int N, int T, int TR>
@@ -299,6 +301,8 @@ __device__ __forceinline__ void row_c2r_grid(const float2* __restrict__ base, bo
     float2 x = make_float2(0.f, 0.f);
     if (valid && k < kc)
       x = src[k];
+    if (k == 0 || k == M)
+      x.y = 0.f;  // c2r semantics: the imaginary parts of the k_z = 0 and N/2 bins do not contribute
     lds[LI::at(c, k)] = x;
   }
   SyncWave::sync();

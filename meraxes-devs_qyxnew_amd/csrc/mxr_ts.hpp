@@ -16,7 +16,6 @@ constexpr int KAP_N = 27;                  // kappa_10 knots (XRayHeatingFunctio
 // XRayHeatingFunctions.h:20-33
 constexpr double SPEED_OF_LIGHT = 2.9979e10;
 constexpr double BOLTZMANN = 1.3806e-16;
-constexpr double MPC = 3.086e24;
 constexpr double XRAY_LUMINOSITY_UNIT = 1.0e40;
 constexpr double NU_over_EV = 1.60217646e-12 / PLANCK;
 constexpr double NUIONIZATION = 13.60 * NU_over_EV;

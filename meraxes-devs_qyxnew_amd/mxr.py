@@ -139,12 +139,16 @@ class KernelStat(ctypes.Structure):
 class TbParams(ctypes.Structure):
     _fields_ = [
         ("ReionGridDim", ctypes.c_int),
+        ("BoxSize", ctypes.c_double),
         ("Hubble_h", ctypes.c_double),
         ("OmegaM", ctypes.c_double),
+        ("OmegaLambda", ctypes.c_double),
+        ("OmegaR", ctypes.c_double),
         ("BaryonFrac", ctypes.c_double),
         ("redshift", ctypes.c_double),
         ("Flag_IncludeSpinTemp", ctypes.c_int),
         ("Flag_IncludePecVelsFor21cm", ctypes.c_int),
+        ("TreesID", ctypes.c_int),
     ]
 
 
@@ -154,6 +158,7 @@ class TbGrids(ctypes.Structure):
         ("xH", ctypes.c_void_p),
         ("deltax", ctypes.c_void_p),
         ("TS_box", ctypes.c_void_p),
+        ("vel", ctypes.c_void_p),
         ("delta_T", ctypes.c_void_p),
     ]
 
@@ -317,15 +322,25 @@ def hii_params(N: int, box: float, filter_type: int = 1, zeta: float = 1.0, reds
     return p
 
 
-def tb_params(N: int, redshift: float = 8.0, spin_temp: bool = False) -> TbParams:
+MXR_TREES_VELOCIRAPTOR = 0
+MXR_TREES_GBPTREES = 1
+MXR_TREES_VELOCIRAPTOR_AUG = 2
+
+
+def tb_params(N: int, redshift: float = 8.0, spin_temp: bool = False, box: float = 100.0, pecvel: int = 0,
+              trees_id: int = MXR_TREES_VELOCIRAPTOR) -> TbParams:
     p = TbParams()
     p.ReionGridDim = N
+    p.BoxSize = box
     p.Hubble_h = GENESIS["Hubble_h"]
     p.OmegaM = GENESIS["OmegaM"]
+    p.OmegaLambda = GENESIS["OmegaLambda"]
+    p.OmegaR = 0.0
     p.BaryonFrac = GENESIS["BaryonFrac"]
     p.redshift = redshift
     p.Flag_IncludeSpinTemp = int(spin_temp)
-    p.Flag_IncludePecVelsFor21cm = 0
+    p.Flag_IncludePecVelsFor21cm = pecvel
+    p.TreesID = trees_id
     return p
 
 
@@ -408,6 +423,8 @@ def lib():
     L.mxr_RtoM.restype = ctypes.c_double
     L.mxr_hii_radii.argtypes = [ctypes.POINTER(HiiParams), ctypes.POINTER(ctypes.c_double), ctypes.c_int]
     L.mxr_hii_radii.restype = ctypes.c_int
+    L.mxr_velocity_gradient.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    L.mxr_velocity_gradient.restype = ctypes.c_int
     L.mxr_brightness_temperature.argtypes = [vp, ctypes.POINTER(TbParams), ctypes.POINTER(TbGrids),
                                              ctypes.POINTER(ctypes.c_double)]
     L.mxr_brightness_temperature.restype = ctypes.c_int
@@ -589,16 +606,26 @@ class Context:
         check(lib().mxr_filter(self._h, _ptr(box), mem, local_ix_start, slab_nx, grid_dim, box_size, R,
                                filter_type), "mxr_filter")
 
-    def brightness_temperature(self, params: TbParams, xH, deltax, delta_T, TS_box=None) -> float:
+    def brightness_temperature(self, params: TbParams, xH, deltax, delta_T, TS_box=None, vel=None) -> float:
+        """mxr_brightness_temperature; `vel` (padded, with PecVels > 0) is converted in place."""
         g = TbGrids()
-        g.mem = _mem_of([xH, deltax, delta_T, TS_box])
+        g.mem = _mem_of([xH, deltax, delta_T, TS_box, vel])
         if g.mem == MXR_MEM_DEVICE:
             _sync_torch()
         g.xH, g.deltax, g.TS_box, g.delta_T = _ptr(xH), _ptr(deltax), _ptr(TS_box), _ptr(delta_T)
+        g.vel = _ptr(vel)
         ave = ctypes.c_double()
         check(lib().mxr_brightness_temperature(self._h, ctypes.byref(params), ctypes.byref(g), ctypes.byref(ave)),
               "mxr_brightness_temperature")
         return ave.value
+
+    def velocity_gradient(self, box, slab_nx: int, grid_dim: int, box_size: float):
+        """mxr_velocity_gradient on a complex64 half spectrum (numpy or torch, in place)."""
+        mem = _mem_of([box])
+        if mem == MXR_MEM_DEVICE:
+            _sync_torch()
+        check(lib().mxr_velocity_gradient(self._h, _ptr(box), mem, slab_nx, grid_dim, box_size),
+              "mxr_velocity_gradient")
 
     def compute_ts(self, params: TsParams, deltax, stars, sfr_histories, xray_luminosity_histories, n_histories: int,
                    x_e_box_prev, x_e_box, Tk_box, TS_box, stored_fcoll: np.ndarray, want_tables: bool = True):

@@ -29,6 +29,7 @@
 #define HUBBLE 3.2407789e-18
 #define SEC_PER_YEAR 3.155e7
 #define TCMB 2.728
+#define MPC 3.086e24
 #define T_RE (double)(1e4)
 #define REL_TOL (float)1e-5
 #define ABS_TOL (float)1e-8
@@ -398,24 +399,64 @@ int oracle_hii_sample_f32(const mxr_hii_params* p, const mxr_hii_grids* g, int n
 }
 
 /* ---------------------------------------------------------------- brightness box */
-/* BrightnessTemperature.c:24-124, 512-545 (Flag_IncludePecVelsFor21cm = 0) */
+#define N_RSD_STEPS 50            /* reionization.h:34 */
+#define MAX_DVDR (float)(0.2)     /* reionization.h:45 */
+#define TB_FRACT_FLOAT_ERR (double)(1e-7)
+
+/* velocity_gradient (reionization.c:3435-3450): each half-spectrum value times i k_z,
+ * k_z = n_z (float)(2 pi / (float)BoxSize), as the complex product (a + ib)(0 + i k_z). */
+int oracle_velocity_gradient(float* box, int slab_nx, int grid_dim, double BoxSize)
+{
+  if (!box || slab_nx < 0 || grid_dim < 2)
+    return MXR_ERR_INVALID_ARG;
+  const int middle = grid_dim / 2;
+  const float box_size = (float)BoxSize;
+  const float delta_k = (float)(2.0 * M_PI / box_size);
+  for (int n_x = 0; n_x < slab_nx; n_x++)
+    for (int n_y = 0; n_y < grid_dim; n_y++)
+      for (int n_z = 0; n_z <= middle; n_z++) {
+        const float k_z = n_z * delta_k;
+        const long idx = n_z + (long)(middle + 1) * (n_y + (long)grid_dim * n_x);
+        const float re = box[2 * idx], im = box[2 * idx + 1];
+        box[2 * idx] = re * 0.0f - im * k_z;
+        box[2 * idx + 1] = re * k_z + im * 0.0f;
+      }
+  return 0;
+}
+
+/* hubble(float z), XRayHeatingFunctions.c:443-453: 1 + z is a float sum */
+static double tb_hubble(const mxr_tb_params* p, float z)
+{
+  return HUBBLE * p->Hubble_h * sqrt(p->OmegaM * pow(1 + z, 3) + p->OmegaR * pow(1 + z, 4) + p->OmegaLambda);
+}
+
+static long wrap_index(long k, int N) { return ((k % N) + N) % N; }
+
+/* BrightnessTemperature.c:24-549 (no mini-haloes).  Destination cells of the line-of-sight
+ * redistribution wrap modulo N, which equals the reference's periodic index wherever the
+ * reference's own index stays inside the row. */
 int oracle_brightness_temperature(const mxr_tb_params* p, const mxr_tb_grids* g, double* volume_ave_Tb)
 {
   if (!p || !g || !g->xH || !g->deltax || !g->delta_T || (p->Flag_IncludeSpinTemp && !g->TS_box))
     return MXR_ERR_INVALID_ARG;
-  if (p->Flag_IncludePecVelsFor21cm)
-    return MXR_ERR_UNSUPPORTED;
+  if (p->Flag_IncludePecVelsFor21cm < 0 || p->Flag_IncludePecVelsFor21cm > 3)
+    return MXR_ERR_INVALID_ARG;
+  const int pecvel = p->Flag_IncludeSpinTemp ? p->Flag_IncludePecVelsFor21cm : 0;
+  if (pecvel > 0 && !g->vel)
+    return MXR_ERR_INVALID_ARG;
   const int N = p->ReionGridDim;
   const long H = N / 2 + 1;
   const double total_n_cells = pow((double)N, 3);
+  const double BoxSize = p->BoxSize / p->Hubble_h;
   float redshift = (float)(float)p->redshift;
   float Hubble_h = (float)(float)p->Hubble_h;
   float OmegaM = (float)(float)p->OmegaM;
   float OmegaB = (float)(OmegaM * p->BaryonFrac);
   float const_factor = (float)(27.0 * (OmegaB * Hubble_h * Hubble_h / 0.023) *
                                sqrt((0.15 / OmegaM / Hubble_h / Hubble_h) * (1.0 + redshift) / 10.0));
+  const float H_z = (float)(float)tb_hubble(p, redshift);
   double T_rad = TCMB * (1. + redshift);
-  double Ave_Tb = 0.0;
+  float* delta_T = g->delta_T;
   for (int ii = 0; ii < N; ii++)
     for (int jj = 0; jj < N; jj++)
       for (int kk = 0; kk < N; kk++) {
@@ -423,14 +464,147 @@ int oracle_brightness_temperature(const mxr_tb_params* p, const mxr_tb_grids* g,
         const long i_padded = kk + 2 * H * (jj + (long)N * ii);
         double pixel_deltax = g->deltax[i_padded];
         double pixel_x_HI = g->xH[i_real];
-        g->delta_T[i_real] = (float)(const_factor * pixel_x_HI * (1.0 + pixel_deltax));
+        delta_T[i_real] = (float)(const_factor * pixel_x_HI * (1.0 + pixel_deltax));
         if (p->Flag_IncludeSpinTemp) {
-          double pixel_Ts_factor = (1 - T_rad / g->TS_box[i_real]);
-          g->delta_T[i_real] *= pixel_Ts_factor;
+          if (p->Flag_IncludePecVelsFor21cm > 1) {
+            delta_T[i_real] *= (1. + redshift) / (1000. * g->TS_box[i_real]);
+          } else {
+            double pixel_Ts_factor = (1 - T_rad / g->TS_box[i_real]);
+            delta_T[i_real] *= pixel_Ts_factor;
+          }
         }
       }
+  if (pecvel > 0) {
+    const double subcell_width = (BoxSize / ((double)N)) / (double)N_RSD_STEPS;
+    float* vel = g->vel;
+    for (int ii = 0; ii < N; ii++)
+      for (int jj = 0; jj < N; jj++)
+        for (int kk = 0; kk < N; kk++) {
+          const long i_padded = kk + 2 * H * (jj + (long)N * ii);
+          switch (p->TreesID) {
+            case MXR_TREES_GBPTREES:
+              vel[i_padded] = (float)(sqrt(1. / (1. + redshift)) * vel[i_padded] / 1000.);
+              break;
+            case MXR_TREES_VELOCIRAPTOR:
+            case MXR_TREES_VELOCIRAPTOR_AUG:
+              vel[i_padded] = (float)((1. + redshift) * vel[i_padded]);
+              break;
+          }
+          vel[i_padded] *= 1000. * 100. / MPC;
+          vel[i_padded] = (float)(vel[i_padded] / (1. / (1. + redshift)));
+        }
+    float* grad = (float*)malloc(sizeof(float) * 2 * (size_t)N * N * H);
+    if (!grad)
+      return MXR_ERR_OOM;
+    oracle_r2c3d_f32(N, vel, grad);
+    for (long ii = 0; ii < (long)N * N * H; ii++) {
+      grad[2 * ii] = (float)((double)grad[2 * ii] / total_n_cells);
+      grad[2 * ii + 1] = (float)((double)grad[2 * ii + 1] / total_n_cells);
+      grad[2 * ii] = (float)((double)grad[2 * ii] * p->Hubble_h);
+      grad[2 * ii + 1] = (float)((double)grad[2 * ii + 1] * p->Hubble_h);
+    }
+    oracle_velocity_gradient(grad, N, N, p->BoxSize);
+    oracle_c2r3d_f32(N, grad);
+    const double max_v_deriv = fabs(MAX_DVDR * H_z);
+    for (int ii = 0; ii < N; ii++)
+      for (int jj = 0; jj < N; jj++)
+        for (int kk = 0; kk < N; kk++) {
+          const long i_real = kk + (long)N * (jj + (long)N * ii);
+          const long i_padded = kk + 2 * H * (jj + (long)N * ii);
+          if (pecvel == 1) {
+            double dvdx = (double)grad[i_padded];
+            if (fabs(dvdx) > max_v_deriv)
+              dvdx = dvdx < 0 ? -max_v_deriv : max_v_deriv;
+            delta_T[i_real] /= (dvdx / H_z + 1.0);
+          } else {
+            const double gradient_component = fabs((double)grad[i_padded] / H_z + 1.0);
+            if (gradient_component < TB_FRACT_FLOAT_ERR)
+              delta_T[i_real] = (float)(1000. * (g->TS_box[i_real] - T_rad) / (1. + redshift));
+            else
+              delta_T[i_real] = (float)((1. - exp(-delta_T[i_real] / gradient_component)) * 1000. *
+                                        (g->TS_box[i_real] - T_rad) / (1. + redshift));
+          }
+        }
+    free(grad);
+    if (pecvel == 3) {
+      float x_pos[N_RSD_STEPS], x_pos_offset[N_RSD_STEPS];
+      const double x_val1 = 0., x_val2 = 1.;
+      for (int iii = 0; iii < N_RSD_STEPS; iii++) {
+        x_pos_offset[iii] = (float)(subcell_width * (float)iii + subcell_width / 2.);
+        x_pos[iii] = (float)(x_pos_offset[iii] / (BoxSize / (float)N));
+      }
+      float* los = (float*)calloc((size_t)N, sizeof(float));
+      if (!los)
+        return MXR_ERR_OOM;
+      const double cell = BoxSize / (float)N;
+      const double frac_w = subcell_width / cell;
+      for (int ii = 0; ii < N; ii++)
+        for (int jj = 0; jj < N; jj++) {
+          const long row = (long)N * (jj + (long)N * ii);
+          const float* vrow = vel + 2 * H * (jj + (long)N * ii);
+          for (int kk = 0; kk < N; kk++)
+            los[kk] = 0.0f;
+          for (int kk = 0; kk < N; kk++) {
+            const float dT = delta_T[row + kk];
+            if (!((fabs(dT) >= TB_FRACT_FLOAT_ERR) && (g->xH[row + kk] >= TB_FRACT_FLOAT_ERR)))
+              continue;
+            const double d1_low = vrow[kk == 0 ? N - 1 : kk - 1] / H_z;
+            const double d2_low = vrow[kk] / H_z;
+            const double d1_high = vrow[kk] / H_z;
+            const double d2_high = vrow[kk == N - 1 ? 0 : kk + 1] / H_z;
+            for (int iii = 0; iii < N_RSD_STEPS; iii++) {
+              double disp;
+              if (x_pos[iii] <= 0.5)
+                disp = d1_low + ((x_pos[iii] + 0.5) - x_val1) * (d2_low - d1_low) / (x_val2 - x_val1);
+              else
+                disp = d1_high + ((x_pos[iii] - 0.5) - x_val1) * (d2_high - d1_high) / (x_val2 - x_val1);
+              const double pos = (x_pos_offset[iii] + disp) / (BoxSize / (float)N);
+              const double lo = pos - (subcell_width / 2.) / (BoxSize / (float)N);
+              const double hi = pos + (subcell_width / 2.) / (BoxSize / (float)N);
+              if (lo >= 0.0 && hi < 1.0) {
+                los[kk] += dT / (float)N_RSD_STEPS;
+              } else if (lo < 0.0 && hi < 0.0) {
+                const double cd = ceil(fabs(lo)) - 1.;
+                if (fabs(hi) > cd) {
+                  los[wrap_index(kk - ((long)cd + 1), N)] += dT / (float)N_RSD_STEPS;
+                } else {
+                  const double fo = (fabs(lo) - cd) / frac_w;
+                  const double fw = 1. - fo;
+                  los[wrap_index(kk - (long)cd, N)] += fw * dT / (float)N_RSD_STEPS;
+                  los[wrap_index(kk - ((long)cd + 1), N)] += fo * dT / (float)N_RSD_STEPS;
+                }
+              } else if (lo < 0.0 && (hi > 0.0 && hi < 1.0)) {
+                const double fw = hi / frac_w;
+                const double fo = 1. - fw;
+                los[wrap_index(kk - 1, N)] += fo * dT / (float)N_RSD_STEPS;
+                los[kk] += fw * dT / (float)N_RSD_STEPS;
+              } else if ((lo >= 0.0 && lo < 1.0) && (hi >= 1.0)) {
+                const double fo = (hi - 1.) / frac_w;
+                const double fw = 1. - fo;
+                los[kk] += fw * dT / (float)N_RSD_STEPS;
+                los[wrap_index(kk + 1, N)] += fo * dT / (float)N_RSD_STEPS;
+              } else {
+                const double cd = floor(fabs(hi));
+                if (lo >= cd) {
+                  los[wrap_index(kk + (long)cd, N)] += dT / (float)N_RSD_STEPS;
+                } else {
+                  const double fo = (hi - cd) / frac_w;
+                  const double fw = 1. - fo;
+                  los[wrap_index(kk + (long)cd - 1, N)] += fw * dT / (float)N_RSD_STEPS;
+                  los[wrap_index(kk + (long)cd, N)] += fo * dT / (float)N_RSD_STEPS;
+                }
+              }
+            }
+          }
+          for (int kk = 0; kk < N; kk++)
+            delta_T[row + kk] = los[kk];
+        }
+      free(los);
+    }
+  }
+  double Ave_Tb = 0.0;
   for (long i = 0; i < (long)N * N * N; i++)
-    Ave_Tb += (double)g->delta_T[i];
+    Ave_Tb += (double)delta_T[i];
   Ave_Tb /= total_n_cells;
   if (volume_ave_Tb)
     *volume_ave_Tb = Ave_Tb;

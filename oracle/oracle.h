@@ -58,6 +58,7 @@ int oracle_hii_sample_f32(const mxr_hii_params* p, const mxr_hii_grids* g, int n
                           double* seconds_fwd);
 
 int oracle_brightness_temperature(const mxr_tb_params* p, const mxr_tb_grids* g, double* volume_ave_Tb);
+int oracle_velocity_gradient(float* box, int slab_nx, int grid_dim, double BoxSize);
 
 /* One spin-temperature step on host grids, with the grid-independent tables `tb` the library
  * reported for the same call (mxr_ts_tables).  red0 (optional, 3 doubles) receives the R_ct = 0

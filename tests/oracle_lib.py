@@ -65,6 +65,8 @@ def lib():
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.oracle_brightness_temperature.argtypes = [ctypes.POINTER(mxr.TbParams), ctypes.POINTER(mxr.TbGrids),
                                                 ctypes.POINTER(ctypes.c_double)]
+    L.oracle_velocity_gradient.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    L.oracle_velocity_gradient.restype = ctypes.c_int
     L.oracle_num_threads.restype = ctypes.c_int
     for name in ("oracle_ts_step_f32", "oracle_ts_step_f64"):
         getattr(L, name).argtypes = [ctypes.POINTER(mxr.TsParams), ctypes.POINTER(mxr.TsTables),
@@ -152,17 +154,30 @@ def hii_sample(params, inputs: dict, state, n_radii: int):
     return n, secs.value, fwd.value
 
 
-def brightness_temperature(params, xH, deltax, TS_box=None):
+def brightness_temperature(params, xH, deltax, TS_box=None, vel=None, return_vel=False):
+    """The oracle's brightness box on copies of the inputs; with return_vel also the velocity grid
+    after the in-place unit conversion."""
     N = params.ReionGridDim
     dT = np.zeros(N ** 3, np.float32)
+    v = None if vel is None else np.ascontiguousarray(vel, np.float32).copy()
     g = mxr.TbGrids()
     g.mem = 0
     g.xH, g.deltax, g.TS_box, g.delta_T = _p(xH), _p(deltax), (_p(TS_box) if TS_box is not None else None), _p(dT)
+    g.vel = _p(v) if v is not None else None
     ave = ctypes.c_double()
     rc = lib().oracle_brightness_temperature(ctypes.byref(params), ctypes.byref(g), ctypes.byref(ave))
     if rc:
         raise mxr.MxrError(rc, "oracle_brightness_temperature")
-    return dT, ave.value
+    return (dT, ave.value, v) if return_vel else (dT, ave.value)
+
+
+def velocity_gradient(box: np.ndarray, N: int, box_size: float) -> np.ndarray:
+    """velocity_gradient on a copy of a complex64 half spectrum (N, N, N/2+1)."""
+    out = np.ascontiguousarray(box, np.complex64).copy()
+    rc = lib().oracle_velocity_gradient(out.ctypes.data, N, N, box_size)
+    if rc:
+        raise mxr.MxrError(rc, "oracle_velocity_gradient")
+    return out
 
 
 def ts_step(params, tables, deltax, stars, sfr_histories, xray_histories, n_hist, x_e_box_prev, Tk_box,
