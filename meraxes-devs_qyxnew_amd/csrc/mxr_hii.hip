@@ -120,12 +120,14 @@ __device__ __forceinline__ bool hii_criterion(const HiiEpi& ea, float df, float&
 // xH (0, or the clamped partial 1 - f_coll) and the not-ionised ones their partial-ionisation
 // temperature T_HI (ComputePartiallyIoinizedTemperature(T_HI, xH = 1) = T_HI); the post-loop
 // temperatures and the FP64 sums follow in k_postloop.
-__device__ __forceinline__ void hii_pair(const HiiEpi& ea, int ir, int ip, float2 df, float2 sf, float2 xf)
+// `was` holds the two cells' flags, loaded by the caller ahead of the transform (ignored on the
+// first radius, where no flag is read).
+__device__ __forceinline__ void hii_pair(const HiiEpi& ea, int ir, int ip, float2 df, float2 sf, float2 xf,
+                                         uchar2 was)
 {
 #pragma clang fp contract(off)
-  uchar2 was = make_uchar2(0, 0);
-  if (!ea.first)
-    was = *reinterpret_cast<const uchar2*>(ea.ion + ir);
+  if (ea.first)
+    was = make_uchar2(0, 0);
   const float dfs[2] = {df.x, df.y};
   float sfs[2] = {sf.x, sf.y};
   const float xfs[2] = {xf.x, xf.y};
@@ -186,8 +188,8 @@ __device__ __forceinline__ void hii_pair(const HiiEpi& ea, int ir, int ip, float
 // weighted_sfr is transformed and clamped like the reference does (find_HII_bubbles.c:382-385)
 // but, with recombinations off, nothing reads it afterwards.
 template <int N, int NT, int TR, int EPI>
-__global__ void __launch_bounds__(NT) k_row_c2r(CGridPtrs in, int ngrid, long nrows, const float2* __restrict__ twg,
-                                                FiltArgs fa, HiiEpi ea)
+__device__ __forceinline__ void row_c2r_body(CGridPtrs in, int ngrid, long nrows, const float2* __restrict__ twg,
+                                             FiltArgs fa, const HiiEpi& ea)
 {
   constexpr int M = N / 2, H = N / 2 + 1, T = NT / TR, P = M + M / 8 + 1;
   using SL = Slots<M, T>;
@@ -210,6 +212,25 @@ __global__ void __launch_bounds__(NT) k_row_c2r(CGridPtrs in, int ngrid, long nr
     }
   }
   __syncthreads();
+  // the flags of this thread's cells, loaded now so that their latency hides behind the
+  // transforms (the epilogue reads them only after all three grids)
+  // The loads are unconditional (in-bounds addresses for every lane; the first radius ignores
+  // the values), so they issue back to back; two cells' flags per 16 bits, packed two slots per
+  // register after the first transform.
+  unsigned short fraw[EPI == C2R_STORE ? 1 : NS];
+  unsigned flags[EPI == C2R_STORE ? 1 : (NS + 1) / 2];
+  if constexpr (EPI != C2R_STORE) {
+    const int rr = valid ? (int)row : 0;
+#pragma unroll
+    for (int q = 0; q < SL::QL; q++) {
+      const int j = t + T * q;
+#pragma unroll
+      for (int r = 0; r < SL::RL; r++) {
+        const int n = (SL::BL % T == 0 || j < SL::BL) ? j + r * SL::BL : 0;
+        fraw[q * SL::RL + r] = *reinterpret_cast<const unsigned short*>(ea.ion + rr * N + 2 * n);
+      }
+    }
+  }
   // grid order keeps few fields live: weighted_sfr first (transformed, clamped, dropped), then
   // deltax (and x_e) kept, stars last and consumed straight from the transform's registers
   float2 res[NK][NS];
@@ -217,6 +238,13 @@ __global__ void __launch_bounds__(NT) k_row_c2r(CGridPtrs in, int ngrid, long nr
   if constexpr (EPI != C2R_STORE) {
     float2 wsfr[NS];
     row_c2r_grid<N, T, TR>(in.p[2], valid, row, lds, tw, kc, c, t, wsfr);  // weighted_sfr
+    // pack the flags here: their loads stay ahead of the first transform instead of being sunk
+    // into the epilogue (they complete with that transform's own loads)
+#pragma unroll
+    for (int i = 0; i < (NS + 1) / 2; i++) {
+      flags[i] = (unsigned)fraw[2 * i] | (2 * i + 1 < NS ? (unsigned)fraw[2 * i + 1] << 16 : 0u);
+      asm volatile("" : "+v"(flags[i]));
+    }
 #pragma unroll
     for (int s = 0; s < NS; s++) {
       float wx = fmaxf(wsfr[s].x, 0.0f), wy = fmaxf(wsfr[s].y, 0.0f);
@@ -258,17 +286,34 @@ __global__ void __launch_bounds__(NT) k_row_c2r(CGridPtrs in, int ngrid, long nr
           const float2 xe = (EPI == C2R_HII_SPIN) ? res[NK - 1][s] : make_float2(0.f, 0.f);
           const int ir = (int)row * N + 2 * n;  // N^3 < 2^31 for N <= 1024
           const int ip = (int)row * 2 * H + 2 * n;
-          hii_pair(ea, ir, ip, res[0][s], res[1][s], xe);
+          const unsigned f = flags[s / 2] >> (16 * (s % 2));
+          hii_pair(ea, ir, ip, res[0][s], res[1][s], xe, make_uchar2(f & 0xffu, (f >> 8) & 0xffu));
         }
     }
   }
 }
 
+// Three waves per SIMD for the store and find_HII epilogues (their registers fit without
+// spilling); the x_e epilogue keeps its registers.
+template <int N, int NT, int TR, int EPI>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3)))
+k_row_c2r(CGridPtrs in, int ngrid, long nrows, const float2* __restrict__ twg, FiltArgs fa, HiiEpi ea)
+{
+  row_c2r_body<N, NT, TR, EPI>(in, ngrid, nrows, twg, fa, ea);
+}
+template <int N, int NT, int TR>
+__global__ void __launch_bounds__(NT) k_row_c2r_spin(CGridPtrs in, int ngrid, long nrows,
+                                                     const float2* __restrict__ twg, FiltArgs fa, HiiEpi ea)
+{
+  row_c2r_body<N, NT, TR, C2R_HII_SPIN>(in, ngrid, nrows, twg, fa, ea);
+}
+
 // One cell of the fused update (the per-cell form of hii_pair, same order of operations).
-__device__ __forceinline__ void hii_cell(const HiiEpi& ea, int ir, int ip, float df, float sf, float xf)
+__device__ __forceinline__ void hii_cell(const HiiEpi& ea, int ir, int ip, float df, float sf, float xf,
+                                         unsigned char flag)
 {
 #pragma clang fp contract(off)
-  const bool was = !ea.first && ea.ion[ir] != 0;
+  const bool was = !ea.first && flag != 0;
   bool ion = was, cross = false;
   float xh = was ? 0.0f : 1.0f;
   if (!was) {
@@ -357,7 +402,18 @@ __global__ void __launch_bounds__(64) k_row_direct(CGridPtrs in, long nrows, int
   }
   const int n[4] = {lane0 ? 0 : p, lane0 ? N / 2 : N - p, lane0 ? N / 4 : N / 2 + p, lane0 ? 3 * N / 4 : N / 2 - p};
   const long row1 = row0 + rpb < nrows ? row0 + rpb : nrows;
+  // flags one row ahead, so their latency hides behind the row's sums
+  unsigned char fl[4] = {0, 0, 0, 0};
+  if (!ea.first && row0 < row1)
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+      fl[e] = ea.ion[(int)row0 * N + n[e]];
   for (long row = row0; row < row1; row++) {
+    unsigned char nfl[4] = {0, 0, 0, 0};
+    if (!ea.first && row + 1 < row1)
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        nfl[e] = ea.ion[(int)(row + 1) * N + n[e]];
     float sfr[4], xe[4] = {0.f, 0.f, 0.f, 0.f}, st[4], dx[4];
     direct_row<KMAX>(in.p[2], row, H, kc, C, S, lane0, sfr);  // weighted_sfr: transformed, clamped, dropped
 #pragma unroll
@@ -372,7 +428,10 @@ __global__ void __launch_bounds__(64) k_row_direct(CGridPtrs in, long nrows, int
     direct_row<KMAX>(in.p[0], row, H, kc, C, S, lane0, dx);
 #pragma unroll
     for (int e = 0; e < 4; e++)
-      hii_cell(ea, (int)row * N + n[e], (int)row * 2 * H + n[e], dx[e], st[e], xe[e]);
+      hii_cell(ea, (int)row * N + n[e], (int)row * 2 * H + n[e], dx[e], st[e], xe[e], fl[e]);
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+      fl[e] = nfl[e];
   }
 }
 
@@ -756,8 +815,12 @@ struct Hii : Passes<N> {
   {
     const long nrows = (long)N * N;
     dim3 grid((unsigned)((nrows + C::TR - 1) / C::TR), (unsigned)(EPI == C2R_STORE ? ngrid : 1));
-    hipLaunchKernelGGL((k_row_c2r<N, C::ROW_NT, C::TR, EPI>), grid, dim3(C::ROW_NT), 0, c->stream, in, ngrid, nrows,
-                       c->tw, fa, ea);
+    if constexpr (EPI == C2R_HII_SPIN)
+      hipLaunchKernelGGL((k_row_c2r_spin<N, C::ROW_NT, C::TR>), grid, dim3(C::ROW_NT), 0, c->stream, in, ngrid, nrows,
+                         c->tw, fa, ea);
+    else
+      hipLaunchKernelGGL((k_row_c2r<N, C::ROW_NT, C::TR, EPI>), grid, dim3(C::ROW_NT), 0, c->stream, in, ngrid, nrows,
+                         c->tw, fa, ea);
     MXR_HIP(hipGetLastError());
     return MXR_OK;
   }

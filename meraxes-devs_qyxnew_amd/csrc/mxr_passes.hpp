@@ -297,13 +297,30 @@ __device__ __forceinline__ void row_c2r_grid(const float2* __restrict__ base, bo
   using SL = Slots<M, T>;
   using LI = LdsRowPad<P>;
   const float2* __restrict__ src = base + (valid ? row : 0) * H;
-  for (int k = t; k < H; k += T) {
-    float2 x = make_float2(0.f, 0.f);
-    if (valid && k < kc)
-      x = src[k];
-    if (k == 0 || k == M)
-      x.y = 0.f;  // c2r semantics: the imaginary parts of the k_z = 0 and N/2 bins do not contribute
-    lds[LI::at(c, k)] = x;
+  // the loads of a batch first, then their LDS stores: the loads are independent, so a batch is
+  // in flight together instead of one round trip per element (two batches bound the registers)
+  constexpr int KI = (H + T - 1) / T;
+  constexpr int KB = (KI + 1) / 2;
+#pragma unroll
+  for (int b0 = 0; b0 < KI; b0 += KB) {
+    float2 xin[KB];
+#pragma unroll
+    for (int i = 0; i < KB; i++) {
+      const int k = t + (b0 + i) * T;
+      float2 x = make_float2(0.f, 0.f);
+      if (b0 + i < KI && valid && k < kc)
+        x = src[k];
+      xin[i] = x;
+    }
+#pragma unroll
+    for (int i = 0; i < KB; i++) {
+      const int k = t + (b0 + i) * T;
+      float2 x = xin[i];
+      if (k == 0 || k == M)
+        x.y = 0.f;  // c2r semantics: the imaginary parts of the k_z = 0 and N/2 bins do not contribute
+      if (b0 + i < KI && k < H)
+        lds[LI::at(c, k)] = x;
+    }
   }
   SyncWave::sync();
   float2 v[SL::NREG];
