@@ -66,12 +66,13 @@ typedef struct mxr_hii_params {
   int Flag_IncludeSpinTemp;     /* x_e_box becomes a 4th filtered grid, Tk_box is used    */
   int Flag_IncludeRecombinations; /* must be 0 (MXR_ERR_UNSUPPORTED otherwise)            */
   int Flag_EvolvingReionRBubbleMax;
-  int Flag_BHFeedback;          /* must be 0 (MXR_ERR_UNSUPPORTED otherwise)              */
+  int Flag_BHFeedback;          /* effective_bhm / effective_bhar become filtered grids    */
   double ReionSfrTimescale;     /* in units of the Hubble time                            */
   double ReionEfficiency;
   double ReionNionPhotPerBary;
   double ReionGammaHaloBias;
   double ReionAlphaUV;
+  double ReionAlphaUVBH;        /* J21 spectral index of the BH sources (Flag_BHFeedback) */
   double ReionRBubbleMax;       /* Mpc/h                                                  */
   double ReionRBubbleMaxRecomb; /* Mpc/h                                                  */
   double ReionRBubbleMin;       /* Mpc/h                                                  */
@@ -102,6 +103,8 @@ typedef struct mxr_hii_grids {
   int storage;                  /* MXR_STORAGE_F32 (0), or MXR_STORAGE_BF16: stars and
                                    weighted_sfr hold bfloat16 bit patterns (uint16, same padded
                                    layout), widened to FP32 on load; fused paths only        */
+  const float* effective_bhm;   /* in  padded real (Flag_BHFeedback only, else NULL)      */
+  const float* effective_bhar;  /* in  padded real (Flag_BHFeedback only, else NULL)      */
 } mxr_hii_grids;
 #define MXR_STORAGE_F32 0
 #define MXR_STORAGE_BF16 1

@@ -161,8 +161,8 @@ int check_hii_params(const mxr_hii_params* p, int N)
     set_error("Unrecognised RtoM filter (%d)", p->ReionRtoMFilterType);
     return MXR_ERR_INVALID_ARG;
   }
-  if (p->Flag_IncludeRecombinations || p->Flag_BHFeedback) {
-    set_error("Flag_IncludeRecombinations / Flag_BHFeedback are not implemented by this library");
+  if (p->Flag_IncludeRecombinations) {
+    set_error("Flag_IncludeRecombinations is not implemented by this library");
     return MXR_ERR_UNSUPPORTED;
   }
   return MXR_OK;
@@ -308,14 +308,14 @@ void mxr_destroy(mxr_ctx* h)
   if (c.stream)
     (void)hipStreamSynchronize(c.stream);
   (void)hipFree(c.tw);
-  for (int g = 0; g < 4; g++) {
+  for (int g = 0; g < 6; g++) {
     (void)hipFree(c.spec[g]);
     (void)hipFree(c.work[g]);
   }
-  for (int g = 0; g < 5; g++) {
+  for (int g = 0; g < 7; g++)
     (void)hipFree(c.in_stage[g]);
+  for (int g = 0; g < 5; g++)
     (void)hipFree(c.out_stage[g]);
-  }
   (void)hipFree(c.ion);
   for (int g = 0; g < 2; g++)
     (void)hipFree(c.ts_src[g]);
@@ -415,6 +415,7 @@ int mxr_autotune(mxr_ctx* h, const mxr_hii_params* p, const mxr_hii_grids* g, co
   }
   RC(check_hii_params(p, h->c.N));
   if (!g->deltax || !g->stars || !g->weighted_sfr || (p->Flag_IncludeSpinTemp && (!g->x_e_box || !g->Tk_box)) ||
+      (p->Flag_BHFeedback && (!g->effective_bhm || !g->effective_bhar)) ||
       (g->mem != MXR_MEM_HOST && g->mem != MXR_MEM_DEVICE) ||
       (g->storage != MXR_STORAGE_F32 && g->storage != MXR_STORAGE_BF16)) {
     set_error("missing input grid, bad memory kind or bad storage type");
@@ -497,6 +498,7 @@ int mxr_find_hii_bubbles(mxr_ctx* h, const mxr_hii_params* params, const mxr_hii
   if (!grids->deltax || !grids->stars || !grids->weighted_sfr || !grids->xH || !grids->z_at_ionization ||
       !grids->J_21_at_ionization || !grids->r_bubble || !grids->temp_kinetic_all_gas ||
       (params->Flag_IncludeSpinTemp && (!grids->x_e_box || !grids->Tk_box)) ||
+      (params->Flag_BHFeedback && (!grids->effective_bhm || !grids->effective_bhar)) ||
       (grids->mem != MXR_MEM_HOST && grids->mem != MXR_MEM_DEVICE) ||
       (grids->storage != MXR_STORAGE_F32 && grids->storage != MXR_STORAGE_BF16)) {
     set_error("missing grid pointer, bad memory kind or bad storage type");
@@ -692,7 +694,8 @@ int mxr_find_hii_bubbles_batched(mxr_batch* b, const mxr_hii_params* params, con
     const mxr_hii_grids& g = grids[i];
     if (g.mem != MXR_MEM_DEVICE || !g.deltax || !g.stars || !g.weighted_sfr || !g.xH || !g.z_at_ionization ||
         !g.J_21_at_ionization || !g.r_bubble || !g.temp_kinetic_all_gas ||
-        (params[i].Flag_IncludeSpinTemp && (!g.x_e_box || !g.Tk_box))) {
+        (params[i].Flag_IncludeSpinTemp && (!g.x_e_box || !g.Tk_box)) ||
+        (params[i].Flag_BHFeedback && (!g.effective_bhm || !g.effective_bhar))) {
       set_error("realisation %d: batched grids must all be device memory", i);
       return MXR_ERR_INVALID_ARG;
     }
@@ -716,7 +719,8 @@ int mxr_find_hii_bubbles_batched(mxr_batch* b, const mxr_hii_params* params, con
   }
   for (int i = 0; i < n; i++) {
     Ctx& c = b->ctx[i]->c;
-    const int rc = ensure_buffers(&c, params[i].Flag_IncludeSpinTemp ? 4 : 3);  // no allocation inside a capture
+    const int rc = ensure_buffers(&c, 3 + (params[i].Flag_IncludeSpinTemp ? 1 : 0) +
+                                            (params[i].Flag_BHFeedback ? 2 : 0));  // no allocation inside a capture
     if (rc)
       return rc;
     c.prof = false;

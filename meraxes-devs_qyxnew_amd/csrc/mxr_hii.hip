@@ -27,16 +27,21 @@
 
 namespace mxr {
 
-enum { C2R_STORE = 0, C2R_HII = 1, C2R_HII_SPIN = 2 };
+// z-pass epilogues: store the real field, or the find_HII update with the x_e grid (SPIN)
+// and / or the BH grids (effective_bhm, effective_bhar)
+enum { C2R_STORE = 0, C2R_HII = 1, C2R_HII_SPIN = 2, C2R_HII_BH = 3, C2R_HII_SPIN_BH = 4 };
+constexpr bool epi_spin(int e) { return e == C2R_HII_SPIN || e == C2R_HII_SPIN_BH; }
+constexpr bool epi_bh(int e) { return e == C2R_HII_BH || e == C2R_HII_SPIN_BH; }
 
 // ---- fused epilogue state -------------------------------------------------------------
 struct HiiEpi {
   double f_coll_prefactor;
   float J21c;     // J_21_aux_constant (float, find_HII_bubbles.c:434-435)
+  float J21c_bh;  // J_21_aux_constant_BH (:436-437)
   float z;        // (float)redshift
   float TK;       // T_RECFAST(z)
   float cT_ad;    // cT_approx(z)
-  int first, last, uvb, spin;
+  int first, last, uvb, spin, bh;
   uint8_t* ion;
   float* xH;
   float* z_in;
@@ -47,7 +52,8 @@ struct HiiEpi {
   const float* sfr;     // unfiltered input (padded)
   int sfr_bf16;         // sfr holds bfloat16 bit patterns
   const float* Tk_box;  // real (spin only)
-  double* partials;     // 6 per block on the last radius
+  const float* bhar;    // unfiltered effective_bhar (padded; BH only)
+  double* partials;     // up to 8 per block
 };
 
 // find_HII_bubbles.c:48-74
@@ -86,14 +92,19 @@ __device__ __forceinline__ float thermal_T(float TK, float cT, float dx)
 // fl(fl(sf / dom) pref) that the last step's partial xH uses.  The quotient is only formed
 // when the division-free comparison sf pref vs nf dom is within 1e-12 of a tie; elsewhere
 // both orderings agree exactly.
-__device__ __forceinline__ bool hii_criterion(const HiiEpi& ea, float df, float& sf, float xf, bool need_lhs,
-                                              double& lhs, double& nf)
+__device__ __forceinline__ bool hii_criterion(const HiiEpi& ea, float df, float& sf, float xf, float& bf,
+                                              bool need_lhs, double& lhs, double& nf)
 {
 #pragma clang fp contract(off)
   df = fmaxf(df, -1.0f + REL_TOL);
   sf = fmaxf(sf, 0.0f);
   if (sf < ABS_TOL)
     sf = 0.0f;
+  if (ea.bh) {
+    bf = fmaxf(bf, 0.0f);
+    if (bf < ABS_TOL)
+      bf = 0.0f;
+  }
   if (ea.spin) {
     xf = fmaxf(xf, 0.0f);
     if (xf < ABS_TOL)
@@ -102,15 +113,26 @@ __device__ __forceinline__ bool hii_criterion(const HiiEpi& ea, float df, float&
   }
   const double density_over_mean = 1.0 + (double)df;
   nf = ea.spin ? 1.0 - (double)xf : 1.0;
-  const double a = (double)sf * ea.f_coll_prefactor;
+  // (bhm + stars) pref vs nf dom: the two quotients' rounding is far inside the 1e-12 band
+  const double a = ((double)sf + (double)(ea.bh ? bf : 0.0f)) * ea.f_coll_prefactor;
   const double b = nf * density_over_mean;
   if (!need_lhs && a > b * (1.0 + 1e-12))
     return true;
   if (!need_lhs && a < b * (1.0 - 1e-12))
     return false;
   const double f_coll_stars = (double)sf / density_over_mean;
-  lhs = (0.0 + f_coll_stars) * ea.f_coll_prefactor;
+  const double f_coll_effective_bhm = ea.bh ? (double)bf / density_over_mean : 0.0;
+  lhs = (f_coll_effective_bhm + f_coll_stars) * ea.f_coll_prefactor;
   return lhs > nf * (1. + 0.0);
+}
+
+// J_21 at a first crossing with UVBFlag 1 (:531-542): the stars term, plus the BH term
+__device__ __forceinline__ float j21_at_crossing(const HiiEpi& ea, float sf, float bf)
+{
+  float j = __fmul_rn(sf, ea.J21c);
+  if (ea.bh)
+    j = __fadd_rn(j, __fmul_rn(bf, ea.J21c_bh));
+  return j;
 }
 
 // The fused update of two adjacent cells ir, ir+1 (ip, ip+1 in the padded layout).
@@ -123,7 +145,7 @@ __device__ __forceinline__ bool hii_criterion(const HiiEpi& ea, float df, float&
 // `was` holds the two cells' flags, loaded by the caller ahead of the transform (ignored on the
 // first radius, where no flag is read).
 __device__ __forceinline__ void hii_pair(const HiiEpi& ea, int ir, int ip, float2 df, float2 sf, float2 xf,
-                                         uchar2 was)
+                                         float2 bf, uchar2 was)
 {
 #pragma clang fp contract(off)
   if (ea.first)
@@ -131,6 +153,7 @@ __device__ __forceinline__ void hii_pair(const HiiEpi& ea, int ir, int ip, float
   const float dfs[2] = {df.x, df.y};
   float sfs[2] = {sf.x, sf.y};
   const float xfs[2] = {xf.x, xf.y};
+  float bfs[2] = {bf.x, bf.y};
   const bool w[2] = {was.x != 0, was.y != 0};
   bool ion[2], cross[2];
   float xh[2];
@@ -141,7 +164,7 @@ __device__ __forceinline__ void hii_pair(const HiiEpi& ea, int ir, int ip, float
     xh[e] = w[e] ? 0.0f : 1.0f;
     cross[e] = false;
     if (!w[e]) {
-      const bool i = hii_criterion(ea, dfs[e], sfs[e], xfs[e], ea.last != 0, lhs, nf);
+      const bool i = hii_criterion(ea, dfs[e], sfs[e], xfs[e], bfs[e], ea.last != 0, lhs, nf);
       ion[e] = i;
       if (i) {
         xh[e] = 0.0f;
@@ -162,7 +185,7 @@ __device__ __forceinline__ void hii_pair(const HiiEpi& ea, int ir, int ip, float
       if (cross[e] && zs[e] < 0) {
         ea.z_in[ir + e] = ea.z;
         if (ea.uvb == 1)
-          ea.J21[ir + e] = __fmul_rn(sfs[e], ea.J21c);
+          ea.J21[ir + e] = j21_at_crossing(ea, sfs[e], bfs[e]);
       }
   }
   if (ea.last) {
@@ -194,7 +217,10 @@ __device__ __forceinline__ void row_c2r_body(CGridPtrs in, int ngrid, long nrows
   constexpr int M = N / 2, H = N / 2 + 1, T = NT / TR, P = M + M / 8 + 1;
   using SL = Slots<M, T>;
   constexpr int NS = SL::QL * SL::RL;
-  constexpr int NK = (EPI == C2R_STORE) ? 1 : (EPI == C2R_HII ? 2 : 3);  // grids kept in registers
+  constexpr bool SPIN = epi_spin(EPI), BH = epi_bh(EPI);
+  // grids kept in registers: deltax, stars, [x_e], [effective_bhm]
+  constexpr int NK = (EPI == C2R_STORE) ? 1 : 2 + (SPIN ? 1 : 0) + (BH ? 1 : 0);
+  constexpr int IXE = 3, IBHM = 3 + (SPIN ? 1 : 0), IBHAR = 4 + (SPIN ? 1 : 0);  // positions in `in`
   __shared__ float2 lds[TR * P];
   __shared__ float2 tw[N];
   const int c = threadIdx.x / T, t = threadIdx.x % T;
@@ -252,8 +278,21 @@ __device__ __forceinline__ void row_c2r_body(CGridPtrs in, int ngrid, long nrows
       wy = wy < ABS_TOL ? 0.0f : wy;
       asm volatile("" ::"v"(wx), "v"(wy));  // the clamped field is computed, then dropped
     }
-    if constexpr (EPI == C2R_HII_SPIN)
-      row_c2r_grid<N, T, TR>(in.p[3], valid, row, lds, tw, kc, c, t, res[2]);  // x_e
+    if constexpr (BH) {
+      float2 wbh[NS];
+      row_c2r_grid<N, T, TR>(in.p[IBHAR], valid, row, lds, tw, kc, c, t, wbh);  // effective_bhar
+#pragma unroll
+      for (int s = 0; s < NS; s++) {  // clamped like the reference; Gamma12 alone reads it
+        float wx = fmaxf(wbh[s].x, 0.0f), wy = fmaxf(wbh[s].y, 0.0f);
+        wx = wx < ABS_TOL ? 0.0f : wx;
+        wy = wy < ABS_TOL ? 0.0f : wy;
+        asm volatile("" ::"v"(wx), "v"(wy));
+      }
+    }
+    if constexpr (SPIN)
+      row_c2r_grid<N, T, TR>(in.p[IXE], valid, row, lds, tw, kc, c, t, res[2]);  // x_e
+    if constexpr (BH)
+      row_c2r_grid<N, T, TR>(in.p[IBHM], valid, row, lds, tw, kc, c, t, res[NK - 1]);  // effective_bhm
     row_c2r_grid<N, T, TR>(in.p[1], valid, row, lds, tw, kc, c, t, res[1]);  // stars
   }
   row_c2r_grid<N, T, TR>(pick(in, g0), valid, row, lds, tw, kc, c, t, res[0]);  // deltax / the grid
@@ -283,33 +322,34 @@ __device__ __forceinline__ void row_c2r_body(CGridPtrs in, int ngrid, long nrows
         for (int r = 0; r < SL::RL; r++) {
           const int n = j + r * SL::BL;
           const int s = q * SL::RL + r;
-          const float2 xe = (EPI == C2R_HII_SPIN) ? res[NK - 1][s] : make_float2(0.f, 0.f);
+          const float2 xe = SPIN ? res[2][s] : make_float2(0.f, 0.f);
+          const float2 bf = BH ? res[NK - 1][s] : make_float2(0.f, 0.f);
           const int ir = (int)row * N + 2 * n;  // N^3 < 2^31 for N <= 1024
           const int ip = (int)row * 2 * H + 2 * n;
           const unsigned f = flags[s / 2] >> (16 * (s % 2));
-          hii_pair(ea, ir, ip, res[0][s], res[1][s], xe, make_uchar2(f & 0xffu, (f >> 8) & 0xffu));
+          hii_pair(ea, ir, ip, res[0][s], res[1][s], xe, bf, make_uchar2(f & 0xffu, (f >> 8) & 0xffu));
         }
     }
   }
 }
 
 // Three waves per SIMD for the store and find_HII epilogues (their registers fit without
-// spilling); the x_e epilogue keeps its registers.
+// spilling); the epilogues with x_e or the BH grids keep their registers.
 template <int N, int NT, int TR, int EPI>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3)))
 k_row_c2r(CGridPtrs in, int ngrid, long nrows, const float2* __restrict__ twg, FiltArgs fa, HiiEpi ea)
 {
   row_c2r_body<N, NT, TR, EPI>(in, ngrid, nrows, twg, fa, ea);
 }
-template <int N, int NT, int TR>
+template <int N, int NT, int TR, int EPI>
 __global__ void __launch_bounds__(NT) k_row_c2r_spin(CGridPtrs in, int ngrid, long nrows,
                                                      const float2* __restrict__ twg, FiltArgs fa, HiiEpi ea)
 {
-  row_c2r_body<N, NT, TR, C2R_HII_SPIN>(in, ngrid, nrows, twg, fa, ea);
+  row_c2r_body<N, NT, TR, EPI>(in, ngrid, nrows, twg, fa, ea);
 }
 
 // One cell of the fused update (the per-cell form of hii_pair, same order of operations).
-__device__ __forceinline__ void hii_cell(const HiiEpi& ea, int ir, int ip, float df, float sf, float xf,
+__device__ __forceinline__ void hii_cell(const HiiEpi& ea, int ir, int ip, float df, float sf, float xf, float bf,
                                          unsigned char flag)
 {
 #pragma clang fp contract(off)
@@ -318,7 +358,7 @@ __device__ __forceinline__ void hii_cell(const HiiEpi& ea, int ir, int ip, float
   float xh = was ? 0.0f : 1.0f;
   if (!was) {
     double lhs = 0.0, nf = 1.0;
-    ion = hii_criterion(ea, df, sf, xf, ea.last != 0, lhs, nf);
+    ion = hii_criterion(ea, df, sf, xf, bf, ea.last != 0, lhs, nf);
     if (ion) {
       xh = 0.0f;
     } else if (ea.last) {
@@ -332,7 +372,7 @@ __device__ __forceinline__ void hii_cell(const HiiEpi& ea, int ir, int ip, float
   if (cross && ea.z_in[ir] < 0) {
     ea.z_in[ir] = ea.z;
     if (ea.uvb == 1)
-      ea.J21[ir] = __fmul_rn(sf, ea.J21c);
+      ea.J21[ir] = j21_at_crossing(ea, sf, bf);
   }
   if (ea.last) {
     ea.xH[ir] = xh;
@@ -414,21 +454,36 @@ __global__ void __launch_bounds__(64) k_row_direct(CGridPtrs in, long nrows, int
 #pragma unroll
       for (int e = 0; e < 4; e++)
         nfl[e] = ea.ion[(int)(row + 1) * N + n[e]];
-    float sfr[4], xe[4] = {0.f, 0.f, 0.f, 0.f}, st[4], dx[4];
-    direct_row<KMAX>(in.p[2], row, H, kc, C, S, lane0, sfr);  // weighted_sfr: transformed, clamped, dropped
+    constexpr bool SPIN = epi_spin(EPI), BH = epi_bh(EPI);
+    constexpr int IXE = 3, IBHM = 3 + (SPIN ? 1 : 0), IBHAR = 4 + (SPIN ? 1 : 0);
+    float sfr[4], xe[4] = {0.f, 0.f, 0.f, 0.f}, bf[4] = {0.f, 0.f, 0.f, 0.f}, st[4], dx[4];
+    // weighted_sfr and effective_bhar: transformed, clamped, dropped (Gamma12 alone reads them)
+    direct_row<KMAX>(in.p[2], row, H, kc, C, S, lane0, sfr);
 #pragma unroll
     for (int e = 0; e < 4; e++) {
       float w = fmaxf(sfr[e], 0.0f);
       w = w < ABS_TOL ? 0.0f : w;
       asm volatile("" ::"v"(w));
     }
-    if constexpr (EPI == C2R_HII_SPIN)
-      direct_row<KMAX>(in.p[3], row, H, kc, C, S, lane0, xe);
+    if constexpr (BH) {
+      float bar[4];
+      direct_row<KMAX>(in.p[IBHAR], row, H, kc, C, S, lane0, bar);
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        float w = fmaxf(bar[e], 0.0f);
+        w = w < ABS_TOL ? 0.0f : w;
+        asm volatile("" ::"v"(w));
+      }
+    }
+    if constexpr (SPIN)
+      direct_row<KMAX>(in.p[IXE], row, H, kc, C, S, lane0, xe);
+    if constexpr (BH)
+      direct_row<KMAX>(in.p[IBHM], row, H, kc, C, S, lane0, bf);
     direct_row<KMAX>(in.p[1], row, H, kc, C, S, lane0, st);
     direct_row<KMAX>(in.p[0], row, H, kc, C, S, lane0, dx);
 #pragma unroll
     for (int e = 0; e < 4; e++)
-      hii_cell(ea, (int)row * N + n[e], (int)row * 2 * H + n[e], dx[e], st[e], xe[e], fl[e]);
+      hii_cell(ea, (int)row * N + n[e], (int)row * 2 * H + n[e], dx[e], st[e], xe[e], bf[e], fl[e]);
 #pragma unroll
     for (int e = 0; e < 4; e++)
       fl[e] = nfl[e];
@@ -471,7 +526,7 @@ __global__ void k_filter(float2* __restrict__ box, int local_ix_start, int slab_
   }
 }
 
-// sanity_check_aliasing on one padded real grid: kind 0 deltax, 1 stars/sfr, 2 x_e
+// sanity_check_aliasing on one padded real grid: kind 0 deltax, 1 stars / sfr / BH grids, 2 x_e
 __global__ void k_clamp(float* __restrict__ f, int N, int kind)
 {
   const int H = N / 2 + 1;
@@ -499,6 +554,7 @@ struct PlainArgs {
   const float* dfl;
   const float* sfl;
   const float* xfl;
+  const float* bfl;  // filtered effective_bhm (BH only)
 };
 
 __global__ void k_criterion(PlainArgs pa, HiiEpi ea)
@@ -511,8 +567,10 @@ __global__ void k_criterion(PlainArgs pa, HiiEpi ea)
     const double density_over_mean = 1.0 + (double)pa.dfl[ip];
     const float sf = pa.sfl[ip];
     const double f_coll_stars = (double)sf / density_over_mean;
+    const float bf = ea.bh ? pa.bfl[ip] : 0.0f;
+    const double f_coll_effective_bhm = ea.bh ? (double)bf / density_over_mean : 0.0;
     const double neutral_fraction = ea.spin ? 1.0 - (double)pa.xfl[ip] : 1.0;
-    const double lhs = (0.0 + f_coll_stars) * ea.f_coll_prefactor;
+    const double lhs = (f_coll_effective_bhm + f_coll_stars) * ea.f_coll_prefactor;
     float xh = ea.xH[ir];
     if (lhs > neutral_fraction * (1. + 0.0)) {
       xh = 0.0f;
@@ -537,7 +595,7 @@ __global__ void k_criterion(PlainArgs pa, HiiEpi ea)
     if ((xh < REL_TOL) && (ea.z_in[ir] < 0)) {
       ea.z_in[ir] = ea.z;
       if (ea.uvb == 1)
-        ea.J21[ir] = __fmul_rn(sf, ea.J21c);
+        ea.J21[ir] = j21_at_crossing(ea, sf, bf);
     }
   }
 }
@@ -549,7 +607,7 @@ __global__ void __launch_bounds__(NT) k_postloop(int N, HiiEpi ea)
 #pragma clang fp contract(off)
   const int H = N / 2 + 1;
   const long n = (long)N * N * N;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   for (long ir = (long)blockIdx.x * NT + threadIdx.x; ir < n; ir += (long)gridDim.x * NT) {
     const long ip = (ir / N) * 2 * H + ir % N;
     const float xh = ea.xH[ir];
@@ -569,8 +627,10 @@ __global__ void __launch_bounds__(NT) k_postloop(int N, HiiEpi ea)
     acc[3] += (double)(ea.sfr_bf16 ? bf16_to_f32(reinterpret_cast<const unsigned short*>(ea.sfr)[ip]) : ea.sfr[ip]);
     acc[4] += (double)temp;
     acc[5] += (double)temp * dom;
+    if (ea.bh)
+      acc[6] += (double)ea.bhar[ip];
   }
-  block_sum<NT, 6>(acc, ea.partials + (long)blockIdx.x * 6);
+  block_sum<NT, 7>(acc, ea.partials + (long)blockIdx.x * 7);
 }
 
 // fixed-order final sum of per-block partials (deterministic)
@@ -815,9 +875,9 @@ struct Hii : Passes<N> {
   {
     const long nrows = (long)N * N;
     dim3 grid((unsigned)((nrows + C::TR - 1) / C::TR), (unsigned)(EPI == C2R_STORE ? ngrid : 1));
-    if constexpr (EPI == C2R_HII_SPIN)
-      hipLaunchKernelGGL((k_row_c2r_spin<N, C::ROW_NT, C::TR>), grid, dim3(C::ROW_NT), 0, c->stream, in, ngrid, nrows,
-                         c->tw, fa, ea);
+    if constexpr (epi_spin(EPI) || epi_bh(EPI))
+      hipLaunchKernelGGL((k_row_c2r_spin<N, C::ROW_NT, C::TR, EPI>), grid, dim3(C::ROW_NT), 0, c->stream, in, ngrid,
+                         nrows, c->tw, fa, ea);
     else
       hipLaunchKernelGGL((k_row_c2r<N, C::ROW_NT, C::TR, EPI>), grid, dim3(C::ROW_NT), 0, c->stream, in, ngrid, nrows,
                          c->tw, fa, ea);
@@ -906,18 +966,27 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
     set_error("BF16 storage is implemented on the fused paths only");
     return MXR_ERR_UNSUPPORTED;
   }
-  const int ngrid = spin ? 4 : 3;
+  const int bh = p->Flag_BHFeedback ? 1 : 0;
+  // grids: deltax, stars, weighted_sfr, [x_e], [effective_bhm, effective_bhar]
+  const int ngrid = 3 + spin + 2 * bh;
+  const int ixe = 3, ibhm = 3 + spin, ibhar = 4 + spin;
   RC(ensure_buffers(c, ngrid));
 
   // ---- inputs and state on the device
-  const float* din[4] = {g->deltax, g->stars, g->weighted_sfr, spin ? g->x_e_box : nullptr};
+  const float* din[MAX_GRIDS] = {g->deltax, g->stars, g->weighted_sfr, nullptr, nullptr, nullptr};
+  if (spin)
+    din[ixe] = g->x_e_box;
+  if (bh) {
+    din[ibhm] = g->effective_bhm;
+    din[ibhar] = g->effective_bhar;
+  }
   const float* dTk = spin ? g->Tk_box : nullptr;
   float* dst[5] = {g->xH, g->z_at_ionization, g->J_21_at_ionization, g->r_bubble, g->temp_kinetic_all_gas};
   if (g->mem == MXR_MEM_HOST) {
     for (int i = 0; i < ngrid; i++)
       RC(stage_in(c, i, din[i], (bf16 && (i == 1 || i == 2)) ? Nc : 2 * Nc, &din[i]));  // bf16: half the bytes
     if (spin)
-      RC(stage_in(c, 4, dTk, nreal, &dTk));
+      RC(stage_in(c, 6, dTk, nreal, &dTk));
     for (int i = 0; i < 5; i++)
       RC(stage_out(c, i, dst[i], nreal, &dst[i]));
   }
@@ -945,6 +1014,8 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
   ea.cT_ad = cT_approx((float)redshift);
   ea.uvb = p->ReionUVBFlag;
   ea.spin = spin;
+  ea.bh = bh;
+  ea.bhar = bh ? din[ibhar] : nullptr;
   ea.ion = c->ion;
   ea.xH = dst[0];
   ea.z_in = dst[1];
@@ -1029,6 +1100,9 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
     Gamma_R_prefactor *= (UM / UT) / PROTONMASS * p->ReionNionPhotPerBary;
     ea.J21c = (float)(Gamma_R_prefactor * PLANCK * 1e21 / (4.0 * M_PI) * p->ReionGammaHaloBias / sfr_timescale *
                       p->ReionAlphaUV);
+    const double Gamma_R_prefactor_BH = Gamma_R_prefactor;  // :429-433
+    ea.J21c_bh = (float)(Gamma_R_prefactor_BH * PLANCK * 1e21 / (4.0 * M_PI) * p->ReionGammaHaloBias /
+                         sfr_timescale * p->ReionAlphaUVBH);
     ea.f_coll_prefactor = p->ReionEfficiency * (4.0 / 3.0) * M_PI / pixel_volume / M_mean * (R * R * R);
     ea.first = (ri == 0);
     ea.last = last;
@@ -1053,10 +1127,14 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
            [&] { return K::template col<+1, COL_YPRUNE>(c, cwork, work, ngrid, K::ygeo(), fa, 1.0, c->col_tc_y); }));
       RC(T(K_INV_Z, G * 8.0 * (double)N * N * km.k1 + state_b, [&] {
         if (direct)
-          return spin ? K::template c2r_direct<C2R_HII_SPIN>(c, cwork, kc, c->zdirect_rpb, ea)
-                      : K::template c2r_direct<C2R_HII>(c, cwork, kc, c->zdirect_rpb, ea);
-        return spin ? K::template c2r<C2R_HII_SPIN>(c, cwork, ngrid, fa, ea)
-                    : K::template c2r<C2R_HII>(c, cwork, ngrid, fa, ea);
+          return bh ? (spin ? K::template c2r_direct<C2R_HII_SPIN_BH>(c, cwork, kc, c->zdirect_rpb, ea)
+                            : K::template c2r_direct<C2R_HII_BH>(c, cwork, kc, c->zdirect_rpb, ea))
+                    : (spin ? K::template c2r_direct<C2R_HII_SPIN>(c, cwork, kc, c->zdirect_rpb, ea)
+                            : K::template c2r_direct<C2R_HII>(c, cwork, kc, c->zdirect_rpb, ea));
+        return bh ? (spin ? K::template c2r<C2R_HII_SPIN_BH>(c, cwork, ngrid, fa, ea)
+                          : K::template c2r<C2R_HII_BH>(c, cwork, ngrid, fa, ea))
+                  : (spin ? K::template c2r<C2R_HII_SPIN>(c, cwork, ngrid, fa, ea)
+                          : K::template c2r<C2R_HII>(c, cwork, ngrid, fa, ea));
       }));
     } else {
       for (int i = 0; i < ngrid; i++) {
@@ -1081,12 +1159,13 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
         RC(T(K_INV_Z, 2 * cplx, [&] { return K::template c2r<C2R_STORE>(c, ci, 1, none, ea); }));
         RC(T(K_CLAMP, 8.0 * nreal, [&] {
           hipLaunchKernelGGL(k_clamp, dim3(ew_blocks(nreal)), dim3(256), 0, c->stream, (float*)c->work[i], N,
-                             i == 0 ? 0 : (i == 3 ? 2 : 1));
+                             i == 0 ? 0 : ((spin && i == ixe) ? 2 : 1));
           MXR_HIP(hipGetLastError());
           return MXR_OK;
         }));
       }
-      PlainArgs pa{N, (const float*)c->work[0], (const float*)c->work[1], spin ? (const float*)c->work[3] : nullptr};
+      PlainArgs pa{N, (const float*)c->work[0], (const float*)c->work[1],
+                   spin ? (const float*)c->work[ixe] : nullptr, bh ? (const float*)c->work[ibhm] : nullptr};
       RC(T(K_CRIT, (double)nreal * (4 + 4 + (spin ? 4 : 0) + 4 + 4 + 4), [&] {
         hipLaunchKernelGGL(k_criterion, dim3(ew_blocks(nreal)), dim3(256), 0, c->stream, pa, ea);
         MXR_HIP(hipGetLastError());
@@ -1100,16 +1179,17 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
   const int nblk = (int)ew_blocks(nreal);
   RC(T(K_POST, (double)nreal * (4 + 4 + 4 + 4 + 4 + 4 + (spin ? 4 : 0)), [&] {
     hipLaunchKernelGGL((k_postloop<256>), dim3(nblk), dim3(256), 0, c->stream, N, ea);
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, c->stream, c->partials, nblk, 6, c->sums);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, c->stream, c->partials, nblk, 7, c->sums);
     MXR_HIP(hipGetLastError());
     return MXR_OK;
   }));
   if (!c->enqueue_only)
     MXR_HIP(hipEventRecord(c->ev1, c->stream));
-  MXR_HIP(hipMemcpyAsync(c->sums_host, c->sums, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
+  MXR_HIP(hipMemcpyAsync(c->sums_host, c->sums, sizeof(double) * 7, hipMemcpyDeviceToHost, c->stream));
   c->fin_total_n_cells = total_n_cells;
   c->fin_sfr_conv = UM / UT * SEC_PER_YEAR / SOLAR_MASS;
   c->fin_n_radii = nR;
+  c->fin_bh = bh;
   if (c->enqueue_only)  // batched: the caller synchronises and calls hii_globals
     return MXR_OK;
   if (g->mem == MXR_MEM_HOST) {
@@ -1136,6 +1216,7 @@ void hii_globals(const Ctx* c, mxr_hii_globals* out)
   out->volume_weighted_global_weighted_sfr = s[3] / c->fin_total_n_cells * c->fin_sfr_conv;
   out->volume_weighted_global_temp_kinetic_all_gas = s[4] / c->fin_total_n_cells;
   out->mass_weighted_global_temp_kinetic_all_gas = s[5] / s[2];
+  out->volume_weighted_global_effective_bhar = c->fin_bh ? s[6] / c->fin_total_n_cells * c->fin_sfr_conv : 0.0;
   out->n_radii = c->fin_n_radii;
 }
 

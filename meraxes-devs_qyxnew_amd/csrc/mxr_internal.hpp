@@ -96,9 +96,10 @@ struct Ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float2* tw = nullptr;            // tw[m] = exp(-2 pi i m / N), m < N (FP64-evaluated, FP32-stored)
-  float2* spec[4] = {};            // unfiltered half spectra (deltax, stars, weighted_sfr, x_e)
-  float2* work[4] = {};            // filtered spectra / FFT work (same size)
-  float* in_stage[5] = {};         // device copies of host inputs: 4 padded grids + Tk_box
+  float2* spec[6] = {};            // unfiltered half spectra (deltax, stars, weighted_sfr,
+                                   // [x_e], [effective_bhm, effective_bhar])
+  float2* work[6] = {};            // filtered spectra / FFT work (same size)
+  float* in_stage[7] = {};         // device copies of host inputs: 6 padded grids + Tk_box
   float* out_stage[5] = {};        // device copies of host state grids
   uint8_t* ion = nullptr;          // per-cell "ionised during this call" flags
   double* partials = nullptr;      // per-block FP64 partial sums
@@ -124,6 +125,7 @@ struct Ctx {
   int col_tc_x = 16, col_tc_y = 16; // columns per workgroup of the per-radius x and y passes
   double fin_total_n_cells = 0, fin_sfr_conv = 0;
   int fin_n_radii = 0;
+  int fin_bh = 0;
 };
 
 inline long n_complex(int N) { return (long)N * N * (N / 2 + 1); }

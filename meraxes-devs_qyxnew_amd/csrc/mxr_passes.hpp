@@ -11,14 +11,16 @@
 
 namespace mxr {
 
+// up to 6 grids: deltax, stars, weighted_sfr, [x_e], [effective_bhm, effective_bhar]
+constexpr int MAX_GRIDS = 6;
 struct GridPtrs {
-  float2* p[4];
+  float2* p[MAX_GRIDS];
 };
 struct CGridPtrs {
-  const float2* p[4];
+  const float2* p[MAX_GRIDS];
 };
 struct CRealPtrs {
-  const float* p[4];
+  const float* p[MAX_GRIDS];
   unsigned bf16;  // bit g: p[g] holds bfloat16 bit patterns (uint16), widened on load
 };
 
@@ -48,7 +50,7 @@ enum { COL_PLAIN = 0, COL_SCALE = 1, COL_FILTER = 2, COL_YPRUNE = 3, COL_FSCALE 
 template <class P>
 __device__ __forceinline__ auto pick(const P& ps, int g)
 {
-  return g == 0 ? ps.p[0] : g == 1 ? ps.p[1] : g == 2 ? ps.p[2] : ps.p[3];
+  return g == 0 ? ps.p[0] : g == 1 ? ps.p[1] : g == 2 ? ps.p[2] : g == 3 ? ps.p[3] : g == 4 ? ps.p[4] : ps.p[5];
 }
 // ---- the reference's window arithmetic (float, unfused) --------------------------------
 __device__ __forceinline__ float kmag_ref(int nx, int ny, int nz, int N, float dk)

@@ -412,9 +412,13 @@ int autotune(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_tuning
   int rc = MXR_OK;
   // inputs on the device
   if (g->mem == MXR_MEM_HOST) {
-    const float* src[5] = {g->deltax, g->stars, g->weighted_sfr, spin ? g->x_e_box : nullptr, spin ? g->Tk_box : nullptr};
-    const float** dst[5] = {&s.deltax, &s.stars, &s.weighted_sfr, &s.x_e_box, &s.Tk_box};
-    for (int i = 0; i < 5 && rc == MXR_OK; i++) {
+    const bool bh = p->Flag_BHFeedback != 0;
+    const float* src[7] = {g->deltax,  g->stars, g->weighted_sfr, spin ? g->x_e_box : nullptr,
+                           spin ? g->Tk_box : nullptr, bh ? g->effective_bhm : nullptr,
+                           bh ? g->effective_bhar : nullptr};
+    const float** dst[7] = {&s.deltax, &s.stars, &s.weighted_sfr, &s.x_e_box, &s.Tk_box, &s.effective_bhm,
+                            &s.effective_bhar};
+    for (int i = 0; i < 7 && rc == MXR_OK; i++) {
       if (!src[i])
         continue;
       const size_t bytes = i == 4 ? sizeof(float) * nreal

@@ -66,6 +66,7 @@ class HiiParams(ctypes.Structure):
         ("ReionNionPhotPerBary", ctypes.c_double),
         ("ReionGammaHaloBias", ctypes.c_double),
         ("ReionAlphaUV", ctypes.c_double),
+        ("ReionAlphaUVBH", ctypes.c_double),
         ("ReionRBubbleMax", ctypes.c_double),
         ("ReionRBubbleMaxRecomb", ctypes.c_double),
         ("ReionRBubbleMin", ctypes.c_double),
@@ -93,6 +94,8 @@ class HiiGrids(ctypes.Structure):
         ("r_bubble", ctypes.c_void_p),
         ("temp_kinetic_all_gas", ctypes.c_void_p),
         ("storage", ctypes.c_int),
+        ("effective_bhm", ctypes.c_void_p),
+        ("effective_bhar", ctypes.c_void_p),
     ]
 
 
@@ -291,7 +294,7 @@ GENESIS = dict(Hubble_h=0.6751, OmegaM=0.3121, OmegaLambda=0.6879, OmegaK=0.0, B
 
 
 def hii_params(N: int, box: float, filter_type: int = 1, zeta: float = 1.0, redshift: float = 8.0,
-               spin_temp: bool = False, tables_dir: str | None = None, **over) -> HiiParams:
+               spin_temp: bool = False, tables_dir: str | None = None, bh: bool = False, **over) -> HiiParams:
     p = HiiParams()
     p.ReionGridDim = N
     p.BoxSize = box
@@ -304,12 +307,13 @@ def hii_params(N: int, box: float, filter_type: int = 1, zeta: float = 1.0, reds
     p.Flag_IncludeSpinTemp = int(spin_temp)
     p.Flag_IncludeRecombinations = 0
     p.Flag_EvolvingReionRBubbleMax = 0
-    p.Flag_BHFeedback = 0
+    p.Flag_BHFeedback = int(bh)
     p.ReionSfrTimescale = 0.5
     p.ReionEfficiency = zeta
     p.ReionNionPhotPerBary = 4000.0
     p.ReionGammaHaloBias = 2.0
     p.ReionAlphaUV = 2.0
+    p.ReionAlphaUVBH = 2.0  # input/params/defaults.par:250
     p.ReionRBubbleMax = 50.0
     p.ReionRBubbleMaxRecomb = 50.0
     p.ReionRBubbleMin = 0.4068
@@ -487,7 +491,7 @@ def _mem_of(arrays):
     return MXR_MEM_HOST if devs.pop() else MXR_MEM_DEVICE
 
 
-HII_IN = ("deltax", "stars", "weighted_sfr", "x_e_box", "Tk_box")
+HII_IN = ("deltax", "stars", "weighted_sfr", "x_e_box", "Tk_box", "effective_bhm", "effective_bhar")
 HII_OUT = ("xH", "z_at_ionization", "J_21_at_ionization", "r_bubble", "temp_kinetic_all_gas")
 
 

@@ -114,6 +114,23 @@ def realisation(N: int, box: float, seed: int, device="cpu", stars_to_matter: fl
     return out
 
 
+def bh_grids(stars: "torch.Tensor", seed: int, bh_fraction: float = 0.05, params=None):
+    """effective_bhm / effective_bhar grids beside a (padded) stars grid: a fraction of the stellar
+    mass with cell-to-cell lognormal scatter (sigma 0.5 dex in e-folds), accreting over 50 Myr."""
+    from mxr import GENESIS  # noqa: E402
+
+    p = dict(GENESIS)
+    if params:
+        p.update(params)
+    UT, _ = _units(p)
+    gen = torch.Generator(device=stars.device)
+    gen.manual_seed(int(seed) * 104729 + 3)
+    scatter = torch.exp(0.5 * torch.randn(stars.shape, generator=gen, device=stars.device, dtype=torch.float64))
+    bhm = (bh_fraction * stars.double() * scatter).float()
+    t50 = 50.0 / (UT / SEC_PER_MEGAYEAR)
+    return dict(effective_bhm=bhm, effective_bhar=(bhm.double() / t50).float())
+
+
 def numpy_realisation(N: int, box: float, seed: int, **kw):
     return {k: v.cpu().numpy() for k, v in realisation(N, box, seed, device="cpu", **kw).items()}
 

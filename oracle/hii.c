@@ -366,6 +366,8 @@ static int check_args(const mxr_hii_params* p, const mxr_hii_grids* g)
     return MXR_ERR_INVALID_ARG;
   if (p->Flag_IncludeSpinTemp && (!g->x_e_box || !g->Tk_box))
     return MXR_ERR_INVALID_ARG;
+  if (p->Flag_BHFeedback && (!g->effective_bhm || !g->effective_bhar))
+    return MXR_ERR_INVALID_ARG;
   return 0;
 }
 

@@ -4,7 +4,7 @@
  * TEST INFRASTRUCTURE ONLY.
  *
  * Follows src/core/find_HII_bubbles.c:86-710 line by line for the configurations the
- * library supports (no recombinations, no BH feedback, no mini-haloes).
+ * library supports (no recombinations, no mini-haloes).
  */
 #define CAT_(a, b) a##b
 #define CAT(a, b) CAT_(a, b)
@@ -18,7 +18,7 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
   const long Nc = (long)N * N * H;
   const long n_real = (long)N * N * N;
 
-  if (p->Flag_IncludeRecombinations || p->Flag_BHFeedback)
+  if (p->Flag_IncludeRecombinations)
     return MXR_ERR_UNSUPPORTED;
   if (p->ReionFilterType < 0 || p->ReionFilterType > 2)
     return MXR_ERR_FILTER;
@@ -47,8 +47,17 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
     cell_length_factor = 1.0;
 
   const int spin = p->Flag_IncludeSpinTemp;
-  const int ngrid = spin ? 4 : 3;
-  const float* src[4] = {g->deltax, g->stars, g->weighted_sfr, g->x_e_box};
+  const int bh = p->Flag_BHFeedback;
+  /* grids: deltax, stars, weighted_sfr, [x_e], [effective_bhm, effective_bhar] */
+  const int ngrid = 3 + (spin ? 1 : 0) + (bh ? 2 : 0);
+  const int IXE = 3, IBHM = 3 + (spin ? 1 : 0), IBHAR = 4 + (spin ? 1 : 0);
+  const float* src[6] = {g->deltax, g->stars, g->weighted_sfr, NULL, NULL, NULL};
+  if (spin)
+    src[IXE] = g->x_e_box;
+  if (bh) {
+    src[IBHM] = g->effective_bhm;
+    src[IBHAR] = g->effective_bhar;
+  }
   float* xH = g->xH;
   float* r_bubble = g->r_bubble;
   float* z_in = g->z_at_ionization;
@@ -71,8 +80,8 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
       margin[ii] = -HUGE_VAL;
 
   /* :163-229 forward transforms (out of place, padded real input) */
-  REAL* unfiltered[4] = {0};
-  REAL* filtered[4] = {0};
+  REAL* unfiltered[6] = {0};
+  REAL* filtered[6] = {0};
   REAL* rin = (REAL*)malloc(sizeof(REAL) * 2 * Nc);
   for (int gi = 0; gi < ngrid; gi++) {
     unfiltered[gi] = (REAL*)malloc(sizeof(REAL) * 2 * Nc);
@@ -142,11 +151,17 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
               filtered[gi][ip] = 0;
           }
           if (spin) {
-            filtered[3][ip] = FMAXR(filtered[3][ip], (REAL)0.0);
-            if (filtered[3][ip] < (REAL)ABS_TOL)
-              filtered[3][ip] = 0;
-            filtered[3][ip] = FMINR(filtered[3][ip], (REAL)0.999);
+            filtered[IXE][ip] = FMAXR(filtered[IXE][ip], (REAL)0.0);
+            if (filtered[IXE][ip] < (REAL)ABS_TOL)
+              filtered[IXE][ip] = 0;
+            filtered[IXE][ip] = FMINR(filtered[IXE][ip], (REAL)0.999);
           }
+          if (bh)
+            for (int gi = IBHM; gi <= IBHAR; gi++) {
+              filtered[gi][ip] = FMAXR(filtered[gi][ip], (REAL)0.0);
+              if (filtered[gi][ip] < (REAL)ABS_TOL)
+                filtered[gi][ip] = 0;
+            }
         }
 
     /* :427-442 per-radius prefactors */
@@ -155,8 +170,11 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
     double Gamma_R_prefactor =
       (1.0 + redshift) * (1.0 + redshift) * R * UnitLength_in_cm / pixel_volume * pow(UnitLength_in_cm, -3.);
     Gamma_R_prefactor *= (UnitMass_in_g / UnitTime_in_s) / PROTONMASS * ReionNionPhotPerBary;
+    const double Gamma_R_prefactor_BH = Gamma_R_prefactor;
     const float J_21_aux_constant = (float)(Gamma_R_prefactor * PLANCK * 1e21 / (4.0 * M_PI) * ReionGammaHaloBias /
                                             sfr_timescale * p->ReionAlphaUV);
+    const float J_21_aux_constant_BH = (float)(Gamma_R_prefactor_BH * PLANCK * 1e21 / (4.0 * M_PI) *
+                                               ReionGammaHaloBias / sfr_timescale * p->ReionAlphaUVBH);
     const double f_coll_prefactor = ReionEfficiency * (4.0 / 3.0) * M_PI / pixel_volume / M_mean * R_cubed;
 
     /* :451-544 per-cell criterion and state */
@@ -168,11 +186,11 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
           const long ip = iz + 2 * H * (iy + (long)N * ix);
           const double density_over_mean = 1.0 + (double)filtered[0][ip];
           const double f_coll_stars = (double)filtered[1][ip] / density_over_mean;
-          const double f_coll_effective_bhm = 0.0;
+          const double f_coll_effective_bhm = bh ? (double)filtered[IBHM][ip] / density_over_mean : 0.0;
           const double rec = 0.0;
           double neutral_fraction;
           if (spin)
-            neutral_fraction = 1.0 - (double)filtered[3][ip];
+            neutral_fraction = 1.0 - (double)filtered[IXE][ip];
           else
             neutral_fraction = 1.0;
           if (margin) {
@@ -192,8 +210,12 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
           }
           if ((xH[ir] < REL_TOL) && (z_in[ir] < 0)) {
             z_in[ir] = (float)redshift;
-            if (flag_ReionUVBFlag == 1)
+            if (flag_ReionUVBFlag == 1) {
               J_21_at_ionization[ir] = (float)((REAL)filtered[1][ip] * (REAL)J_21_aux_constant);
+              if (bh)
+                J_21_at_ionization[ir] = (float)((REAL)J_21_at_ionization[ir] +
+                                                 (REAL)filtered[IBHM][ip] * (REAL)J_21_aux_constant_BH);
+            }
           }
         }
     R /= ReionDeltaRFactor;
@@ -214,6 +236,7 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
   /* :549-645 reductions and temperatures */
   double volume_weighted_global_xH = 0.0, volume_weighted_global_r_bubble = 0.0;
   double volume_weighted_global_weighted_sfr = 0.0, volume_weighted_global_temp_kinetic_all_gas = 0.0;
+  double volume_weighted_global_effective_bhar = 0.0;
   double mass_weight = 0.0, mass_weighted_global_xH = 0.0, mass_weighted_global_r_bubble = 0.0;
   double mass_weighted_global_temp_kinetic_all_gas = 0.0;
   for (int ix = 0; ix < N; ix++)
@@ -225,6 +248,8 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
         volume_weighted_global_xH += cell_xH;
         volume_weighted_global_r_bubble += (double)r_bubble[ir];
         volume_weighted_global_weighted_sfr += (double)weighted_sfr[ip];
+        if (bh)
+          volume_weighted_global_effective_bhar += (double)g->effective_bhar[ip];
         const double density_over_mean = 1.0 + (double)deltax[ip];
         mass_weighted_global_xH += cell_xH * density_over_mean;
         mass_weighted_global_r_bubble += (double)r_bubble[ir] * density_over_mean;
@@ -253,6 +278,8 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
     out->volume_weighted_global_r_bubble = volume_weighted_global_r_bubble / total_n_cells;
     out->volume_weighted_global_weighted_sfr = volume_weighted_global_weighted_sfr / total_n_cells *
                                                (UnitMass_in_g / UnitTime_in_s * SEC_PER_YEAR / SOLAR_MASS);
+    out->volume_weighted_global_effective_bhar = volume_weighted_global_effective_bhar / total_n_cells *
+                                                 (UnitMass_in_g / UnitTime_in_s * SEC_PER_YEAR / SOLAR_MASS);
     out->volume_weighted_global_temp_kinetic_all_gas = volume_weighted_global_temp_kinetic_all_gas / total_n_cells;
     out->mass_weighted_global_xH = mass_weighted_global_xH / mass_weight;
     out->mass_weighted_global_r_bubble = mass_weighted_global_r_bubble / mass_weight;

@@ -324,3 +324,72 @@ def test_error_vs_fp64_oracle(ctxs, ftype):
     assert np.all(np.abs(margin[mg]) < 1e-3)
     assert abs(gg.volume_weighted_global_xH - g64.volume_weighted_global_xH) <= \
         2 * abs(g32.volume_weighted_global_xH - g64.volume_weighted_global_xH) + 1e-6
+
+
+# ---- BH feedback: effective_bhm / effective_bhar as 5th and 6th filtered grids ----
+
+def _with_bh(inp, seed=5, spin_N=None):
+    import synth
+    import torch
+
+    out = dict(inp)
+    out.update({k: v.numpy() for k, v in synth.bh_grids(torch.from_numpy(inp["stars"]), seed).items()})
+    if spin_N:
+        N = spin_N
+        r = np.random.default_rng(3)
+        H = N // 2 + 1
+        xe = np.zeros((N, N, 2 * H), np.float32)
+        xe[:, :, :N] = r.uniform(0.0, 0.3, (N, N, N)).astype(np.float32)
+        out["x_e_box"] = xe.reshape(-1)
+        out["Tk_box"] = r.uniform(5.0, 50.0, N ** 3).astype(np.float32)
+    return out
+
+
+@pytest.mark.parametrize("ftype,spin", [(1, False), (0, False), (1, True)])
+def test_bh_parity_vs_oracle(ctxs, ftype, spin):
+    N, box = 64, 67.8
+    print(f"BH parity filter={ftype} spin={spin}", flush=True)
+    inp = _with_bh(realisation(N, box, ftype), spin_N=N if spin else None)
+    p = mxr.hii_params(N, box, filter_type=ftype, spin_temp=spin, bh=True)
+    sg, sr = _state(N), _state(N)
+    gg = _run_gpu(ctxs(N), p, inp, sg)
+    rg, margin = O.find_hii_bubbles(p, inp, sr, margin=True)
+    st = compare(N, _as_dict(sg), _as_dict(sr), gg, rg, margin)
+    print(st)
+    check(st, p999_tol=1e-4 if ftype == 1 else 5e-4)
+    assert gg.volume_weighted_global_effective_bhar == pytest.approx(rg.volume_weighted_global_effective_bhar,
+                                                                     rel=1e-12)
+    assert gg.volume_weighted_global_effective_bhar > 0
+
+
+@pytest.mark.parametrize("spin", [False, True])
+def test_bh_paths_bit_identical(ctxs, spin):
+    N, box = 64, 67.8
+    inp = _with_bh(realisation(N, box, 1), spin_N=N if spin else None)
+    p = mxr.hii_params(N, box, filter_type=1, spin_temp=spin, bh=True)
+    outs = {}
+    for path in (mxr.MXR_PATH_FUSED, mxr.MXR_PATH_FUSED_DENSE, mxr.MXR_PATH_PLAIN):
+        st = _state(N)
+        g = _run_gpu(ctxs(N, path), p, inp, st)
+        outs[path] = (_as_dict(st), g)
+    ref, rg = outs[mxr.MXR_PATH_FUSED]
+    for path in (mxr.MXR_PATH_FUSED_DENSE, mxr.MXR_PATH_PLAIN):
+        o, g = outs[path]
+        for k in ref:
+            assert int((o[k] != ref[k]).sum()) == 0, f"path {path} field {k}"
+        assert g.volume_weighted_global_effective_bhar == rg.volume_weighted_global_effective_bhar
+    ctxs(N, mxr.MXR_PATH_FUSED)
+
+
+def test_bh_parity_256_direct(ctxs):
+    """256^3 with BH grids: the direct-summation z pass on the radii that keep few k_z."""
+    N, box = 256, 200.0
+    print("BH parity N=256", flush=True)
+    inp = _with_bh(realisation(N, box, 1))
+    p = mxr.hii_params(N, box, filter_type=1, bh=True)
+    sg, sr = _state(N), _state(N)
+    gg = _run_gpu(ctxs(N), p, inp, sg)
+    rg, margin = O.find_hii_bubbles(p, inp, sr, margin=True)
+    st = compare(N, _as_dict(sg), _as_dict(sr), gg, rg, margin)
+    print(st)
+    check(st)

@@ -149,3 +149,83 @@ def test_unsupported_flags_are_errors():
     with pytest.raises(mxr.MxrError) as e:
         O.find_hii_bubbles(p, uniform_inputs(N, 1.0), mxr.HiiState.host(N))
     assert e.value.code == mxr.MXR_ERR_UNSUPPORTED
+
+
+# ---- BH feedback (Flag_BHFeedback): effective_bhm and effective_bhar as filtered grids ----
+
+def _bh_inputs(N, stars, bhm, bhar=0.0):
+    d = uniform_inputs(N, stars)
+    H = N // 2 + 1
+    for k, v in (("effective_bhm", bhm), ("effective_bhar", bhar)):
+        a = np.zeros((N, N, 2 * H), np.float32)
+        a[:, :, :N] = v
+        d[k] = a.reshape(-1)
+    return d
+
+
+@pytest.mark.parametrize("ftype", [0, 1])
+def test_bh_uniform_field_criterion_and_partial_xH(ftype):
+    """BH mass enters the criterion beside the stars: a uniform box ionises iff
+    zeta (stars + bhm) / (V_pix Omega_m rho_crit) > 1, with BH sources alone as well, and the
+    last step's partial xH is 1 - zeta (stars + bhm) / (V_pix Omega_m rho_crit)."""
+    N, box = 16, 20.0
+    p = mxr.hii_params(N, box, filter_type=ftype, zeta=1.3, bh=True)
+    m = matter_per_cell(p) / p.ReionEfficiency
+    for stars, bhm, expect_ion in ((0.0, 1.02 * m, True), (0.6 * m, 0.45 * m, True), (0.4 * m, 0.3 * m, False)):
+        st = mxr.HiiState.host(N)
+        O.find_hii_bubbles(p, _bh_inputs(N, stars, bhm), st)
+        if expect_ion:
+            assert np.all(st.xH == 0)
+        else:
+            np.testing.assert_allclose(st.xH, 1 - (stars + bhm) / m, rtol=3e-5)
+            assert np.all(st.z_at_ionization == -1)
+
+
+def test_bh_J21_adds_the_bh_term_with_its_own_index():
+    """J21 at the first crossing = stars_f J_aux(alpha_UV) + bhm_f J_aux(alpha_UV,BH): doubling
+    alpha_UV,BH doubles the BH part only."""
+    N, box = 16, 20.0
+    m = matter_per_cell(mxr.hii_params(N, box))
+    s, b = 0.9 * m, 0.6 * m
+    res = {}
+    for a_bh in (2.0, 4.0):
+        p = mxr.hii_params(N, box, filter_type=1, bh=True, ReionAlphaUVBH=a_bh)
+        st = mxr.HiiState.host(N)
+        O.find_hii_bubbles(p, _bh_inputs(N, s, b), st)
+        assert np.all(st.xH == 0)
+        res[a_bh] = float(st.J_21_at_ionization[0])
+    p0 = mxr.hii_params(N, box, filter_type=1)
+    st = mxr.HiiState.host(N)
+    O.find_hii_bubbles(p0, uniform_inputs(N, s + b), st)  # alpha_UV = 2: the same per-mass constant
+    j_all = float(st.J_21_at_ionization[0])
+    assert res[2.0] == pytest.approx(j_all, rel=1e-6)
+    j_star = j_all * s / (s + b)
+    assert (res[4.0] - j_star) == pytest.approx(2 * (res[2.0] - j_star), rel=1e-5)
+
+
+def test_bh_global_effective_bhar_and_split_equivalence():
+    """volume-weighted effective_bhar is the unfiltered grid's mean in Msun/yr; moving part of
+    the stellar mass into effective_bhm (same alpha_UV) leaves the flags unchanged up to the
+    threshold cells and xH within rounding."""
+    N, box = 32, 40.0
+    inp = realisation(N, box, 1)
+    p = mxr.hii_params(N, box, filter_type=1)
+    pb = mxr.hii_params(N, box, filter_type=1, bh=True)
+    a, b = mxr.HiiState.host(N), mxr.HiiState.host(N)
+    ga, margin = O.find_hii_bubbles(p, inp, a, margin=True)
+    split = dict(inp)
+    split["stars"] = (inp["stars"] * np.float32(0.75)).astype(np.float32)
+    split["effective_bhm"] = (inp["stars"] - split["stars"]).astype(np.float32)
+    split["effective_bhar"] = (inp["weighted_sfr"] * np.float32(0.3)).astype(np.float32)
+    gb = O.find_hii_bubbles(pb, split, b)
+    mism = (a.xH == 0) != (b.xH == 0)
+    assert mism.sum() <= max(2, 5e-4 * N ** 3)
+    assert np.all(np.abs(margin[mism]) < 1e-3)
+    ok = ~mism
+    assert np.abs(a.xH[ok] - b.xH[ok]).max() < 1e-4
+    UT = p.UnitLength_in_cm / p.UnitVelocity_in_cm_per_s
+    H = N // 2 + 1
+    bhar = split["effective_bhar"].reshape(N, N, 2 * H)[:, :, :N].astype(np.float64).mean()
+    want = bhar * p.UnitMass_in_g / UT * 3.155e7 / 1.989e33
+    assert gb.volume_weighted_global_effective_bhar == pytest.approx(want, rel=1e-12)
+    assert ga.volume_weighted_global_effective_bhar == 0.0
