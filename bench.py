@@ -169,6 +169,52 @@ def ts_leg(args, torch):
     return out
 
 
+def ps_leg(args, torch):
+    """The 21-cm power spectrum (mxr_power_spectrum) of a device-resident brightness box."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import mxr
+    import oracle_lib as O
+
+    N, box, h = args.ps_N, args.box, 0.678
+    g = torch.Generator(device="cuda").manual_seed(args.seed)
+    dT = (20.0 * (1.0 + 0.3 * torch.randn(N ** 3, device="cuda", generator=g))).float().contiguous()
+    ctx = mxr.Context(N)
+    for _ in range(max(args.warmup, 1)):
+        ctx.power_spectrum(dT, box, h)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.ps_steps):
+        PS_k, PS_data, _ = ctx.power_spectrum(dT, box, h)
+    ms = 1e3 * (time.perf_counter() - t0) / args.ps_steps
+    # per cell: mean read 4, deldel read 4 + write 4 (padded), r2c read 4 + write 4 (half
+    # spectrum), y and x passes read + write 8 each, binning read 4: 40 B (padding aside)
+    model = 40.0 * N ** 3
+    out = {"metric": f"cells/s (Compute_PS, {N}^3)", "value": N ** 3 / (ms / 1e3), "unit": "cells/s",
+           "ms_per_call": round(ms, 3), "steps": args.ps_steps, "bins": len(PS_k),
+           "config": {"workload": f"Compute_PS N={N} box={box} cMpc/h h={h}, Gaussian brightness box "
+                                  f"(mean 20 mK, 30% rms)", "grid_dim": N},
+           "call_roofline": {"bound": "hbm", "model_bytes_per_call": model, "achieved": model / (ms / 1e3) / 1e9,
+                             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": model / (ms / 1e3) / HBM_PEAK}}
+    ctx.close()
+    del dT
+    torch.cuda.empty_cache()
+    if args.ps_cpu_N > 0:
+        n = args.ps_cpu_N
+        rng = np.random.default_rng(args.seed)
+        x = (20.0 * (1.0 + 0.3 * rng.standard_normal(n ** 3))).astype(np.float32)
+        t0 = time.perf_counter()
+        O.power_spectrum(x, n, box * n / N, h)
+        s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n ** 3 / s, "unit": "cells/s", "cores": int(O.lib().oracle_num_threads()),
+                               "kind": "port",
+                               "sample": f"oracle Compute_PS at {n}^3 (same cell size; OpenMP FFT, serial binning), "
+                                         f"{s:.2f} s"}
+    log("bench: ps " + json.dumps(out))
+    return out
+
+
 # stars-to-matter ratios of the seeded realisations used by the parity tests at these N
 BATCH_S2M = {96: 0.01253, 128: 0.19495, 160: 0.02558}
 
@@ -303,6 +349,9 @@ def main():
     ap.add_argument("--ts-steps", type=int, default=5)
     ap.add_argument("--ts-stars-to-matter", type=float, default=0.020055)
     ap.add_argument("--ts-cpu-N", type=int, default=64, help="grid of the spin-temperature CPU sample (0: skip)")
+    ap.add_argument("--ps-N", type=int, default=512, help="power-spectrum grid (0: skip that leg)")
+    ap.add_argument("--ps-steps", type=int, default=10)
+    ap.add_argument("--ps-cpu-N", type=int, default=512, help="grid of the power-spectrum CPU sample (0: skip)")
     ap.add_argument("--batch", type=int, default=64, help="realisations in the batch leg (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--autotune", choices=("auto", "always", "never"), default="auto",
@@ -433,6 +482,7 @@ def main():
 
     ts = ts_leg(args, torch) if args.ts_N > 0 else None
     batch = batch_leg(args, torch) if args.batch > 0 else None
+    ps = ps_leg(args, torch) if args.ps_N > 0 else None
 
     value = N ** 3 * n_radii / (ms_step / 1e3)
     out = {
@@ -466,6 +516,7 @@ def main():
         "xH_global": fused_xh,
         "spin_temperature": ts,
         "batch": batch,
+        "power_spectrum": ps,
     }
     print(json.dumps(out), flush=True)
 

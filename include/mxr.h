@@ -282,6 +282,27 @@ typedef struct mxr_tb_grids {
 int mxr_brightness_temperature(mxr_ctx* ctx, const mxr_tb_params* params, const mxr_tb_grids* grids,
                                double* volume_ave_Tb);
 
+/* ---- 21-cm power spectrum -------------------------------------------------------------------- */
+/* Number of k bins: ceilings delta_k 1.35^i below delta_k ReionGridDim, delta_k = (float)(2 pi
+ * / (BoxSize / Hubble_h)).  Replaces Initialise_PowerSpectrum's PS_Length
+ * (src/core/ComputePowerSpectrum.c:18-41).                                              */
+int mxr_ps_length(int ReionGridDim, double BoxSize, double Hubble_h);
+
+/* Dimensional power spectrum (mK^2) of a real grid delta_T (host or device, `mem`): the FP64
+ * mean <dT>, dd = (dT/<dT> - 1) V / N^3 <dT> on the padded grid, an unnormalised r2c, then
+ * per bin the averages of k and of k^3 |dd(k)|^2 / (2 pi^2 V) over the half spectrum, and
+ * PS_error = PS_data / sqrt(count).  PS_Length must equal mxr_ps_length(); the three outputs
+ * are host arrays of PS_Length floats.  Works for any field (delta_TII too).
+ * Replaces Compute_PS (src/core/ComputePowerSpectrum.c:44-255, declared
+ * src/core/ComputePowerSpectrum.h:10).                                                 */
+int mxr_power_spectrum(mxr_ctx* ctx, const float* delta_T, int mem, double BoxSize, double Hubble_h,
+                       int PS_Length, float* PS_k, float* PS_data, float* PS_error);
+
+/* The per-bin mode counts behind PS_error (in_bin_ct, src/core/ComputePowerSpectrum.c:140,
+ * 176), PS_Length doubles into `counts`.  The reference keeps them internal; they are exposed
+ * for callers that combine spectra of several boxes and to verify the binning.              */
+int mxr_ps_counts(mxr_ctx* ctx, double BoxSize, double Hubble_h, int PS_Length, double* counts);
+
 /* ---- spin temperature ----------------------------------------------------------------------- */
 #define MXR_TS_MAX_STEPS 64  /* TsNumFilterSteps upper bound                                  */
 #define MXR_TS_MAX_OPS 16    /* source-grid operations before one shell's forward FFT          */

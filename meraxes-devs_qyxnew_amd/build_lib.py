@@ -20,9 +20,11 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # Packed FP32 instructions are disabled for all device code; FP contraction is left on for
 # the FFT arithmetic and switched off, per function, where the reference's rounding matters.
+# FP32 division and sqrtf are correctly rounded (the power spectrum's |k| binning relies on it).
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Wno-unused-result",
 ]
 

@@ -327,6 +327,7 @@ void mxr_destroy(mxr_ctx* h)
   if (c.ev_red)
     (void)hipEventDestroy(c.ev_red);
   (void)hipFree(c.partials);
+  (void)hipFree(c.ps_part);
   (void)hipFree(c.sums);
   if (c.sums_host)
     (void)hipHostFree(c.sums_host);
@@ -961,6 +962,53 @@ int mxr_brightness_temperature(mxr_ctx* h, const mxr_tb_params* p, const mxr_tb_
     return MXR_ERR_HIP;
   }
   return run_brightness_temperature(&h->c, p, g, ave);
+}
+
+int mxr_ps_length(int ReionGridDim, double BoxSize, double Hubble_h)
+{
+  if (ReionGridDim <= 0 || !(BoxSize > 0) || !(Hubble_h > 0)) {
+    set_error("bad ps_length arguments");
+    return MXR_ERR_INVALID_ARG;
+  }
+  return ps_length(ReionGridDim, BoxSize, Hubble_h);
+}
+
+int mxr_power_spectrum(mxr_ctx* h, const float* delta_T, int mem, double BoxSize, double Hubble_h, int PS_Length,
+                       float* PS_k, float* PS_data, float* PS_error)
+{
+  if (!h || !delta_T || !PS_k || !PS_data || !PS_error || !(BoxSize > 0) || !(Hubble_h > 0) ||
+      (mem != MXR_MEM_HOST && mem != MXR_MEM_DEVICE)) {
+    set_error("bad power_spectrum arguments");
+    return MXR_ERR_INVALID_ARG;
+  }
+  const int nb = ps_length(h->c.N, BoxSize, Hubble_h);
+  if (PS_Length != nb) {
+    set_error("PS_Length %d differs from the %d bins of this grid and box", PS_Length, nb);
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (hipSetDevice(h->c.device) != hipSuccess) {
+    set_error("hipSetDevice(%d) failed", h->c.device);
+    return MXR_ERR_HIP;
+  }
+  return run_power_spectrum(&h->c, delta_T, mem, BoxSize, Hubble_h, PS_k, PS_data, PS_error, nb);
+}
+
+int mxr_ps_counts(mxr_ctx* h, double BoxSize, double Hubble_h, int PS_Length, double* counts)
+{
+  if (!h || !counts || !(BoxSize > 0) || !(Hubble_h > 0)) {
+    set_error("bad ps_counts arguments");
+    return MXR_ERR_INVALID_ARG;
+  }
+  const int nb = ps_length(h->c.N, BoxSize, Hubble_h);
+  if (PS_Length != nb) {
+    set_error("PS_Length %d differs from the %d bins of this grid and box", PS_Length, nb);
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (hipSetDevice(h->c.device) != hipSuccess) {
+    set_error("hipSetDevice(%d) failed", h->c.device);
+    return MXR_ERR_HIP;
+  }
+  return run_ps_counts(&h->c, BoxSize, Hubble_h, nb, counts);
 }
 
 }  // extern "C"

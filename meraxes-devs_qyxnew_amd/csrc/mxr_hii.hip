@@ -856,7 +856,7 @@ __global__ void __launch_bounds__(64) k_tb_rsd(TbRsdArgs a)
     dT[k] = los[k];
 }
 
-// FP64 block sums of a real grid
+// FP64 block sums of a real grid (scalar loads)
 template <int NT>
 __global__ void __launch_bounds__(NT) k_sum_f(const float* __restrict__ x, long n, double* partials)
 {
@@ -864,6 +864,41 @@ __global__ void __launch_bounds__(NT) k_sum_f(const float* __restrict__ x, long 
   for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT)
     acc[0] += (double)x[i];
   block_sum<NT, 1>(acc, partials + blockIdx.x);
+}
+
+// the same with 16-byte loads, four in flight per thread (x 16-byte aligned, n % 4 == 0)
+template <int NT>
+__global__ void __launch_bounds__(NT) k_sum_f4(const float4* __restrict__ x, long n4, double* partials)
+{
+  constexpr int U = 4;
+  double acc[1] = {0.0};
+  const long stride = (long)gridDim.x * NT;
+  long i = (long)blockIdx.x * NT + threadIdx.x;
+  for (; i + (U - 1) * stride < n4; i += U * stride) {
+    float4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      v[u] = x[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      acc[0] += ((double)v[u].x + (double)v[u].y) + ((double)v[u].z + (double)v[u].w);
+  }
+  for (; i < n4; i += stride) {
+    const float4 v = x[i];
+    acc[0] += ((double)v.x + (double)v.y) + ((double)v.z + (double)v.w);
+  }
+  block_sum<NT, 1>(acc, partials + blockIdx.x);
+}
+
+int launch_sum_f(Ctx* c, const float* x, long n, int nblk)
+{
+  if (n % 4 == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0)
+    hipLaunchKernelGGL((k_sum_f4<256>), dim3(nblk), dim3(256), 0, c->stream, reinterpret_cast<const float4*>(x), n / 4,
+                       c->partials);
+  else
+    hipLaunchKernelGGL((k_sum_f<256>), dim3(nblk), dim3(256), 0, c->stream, x, n, c->partials);
+  MXR_HIP(hipGetLastError());
+  return MXR_OK;
 }
 
 template <int N>
@@ -1391,8 +1426,7 @@ int run_brightness_temperature(Ctx* c, const mxr_tb_params* p, const mxr_tb_grid
       hipLaunchKernelGGL(k_tb_rsd, dim3((unsigned)((rows + 63) / 64)), dim3(64), 0, c->stream, ra);
       MXR_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL((k_sum_f<256>), dim3(nblk), dim3(256), 0, c->stream, dT, nreal, c->partials);
-    MXR_HIP(hipGetLastError());
+    RC(launch_sum_f(c, dT, nreal, nblk));
   }
   hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, c->stream, c->partials, nblk, 1, c->sums);
   MXR_HIP(hipGetLastError());

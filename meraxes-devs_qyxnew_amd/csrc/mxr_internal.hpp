@@ -77,6 +77,8 @@ struct KStat {
 };
 
 // One grid dimension's worth of persistent device state.
+constexpr int MXR_PS_MAXB = 40;  // power-spectrum bins: 1 + log(N)/log(1.35), 21 at N = 512
+
 struct Ctx {
   // profiling: an event pair per timed launch, resolved after the call's final sync
   bool prof = false;
@@ -123,6 +125,13 @@ struct Ctx {
   int zdirect_rpb = 4;             // rows per wavefront of the direct z pass
   int live_tiles = 1;              // x/y pass grids over the tiles that can hold kept columns
   int col_tc_x = 16, col_tc_y = 16; // columns per workgroup of the per-radius x and y passes
+  // power spectrum: per-block bin partials, and the data-independent per-bin mode counts and
+  // |k| sums of the last (grid, delta_k) seen
+  double* ps_part = nullptr;
+  size_t ps_part_cap = 0;          // doubles
+  float ps_geom_dk = 0;
+  int ps_geom_nb = 0;
+  double ps_geom[2 * MXR_PS_MAXB] = {};
   double fin_total_n_cells = 0, fin_sfr_conv = 0;
   int fin_n_radii = 0;
   int fin_bh = 0;
@@ -145,10 +154,17 @@ int run_filter(Ctx* c, float2* box, int local_ix_start, int slab_nx, int grid_di
 int run_velocity_gradient(Ctx* c, float2* box, int slab_nx, int grid_dim, float dk);
 int run_brightness_temperature(Ctx* c, const mxr_tb_params* p, const mxr_tb_grids* g, double* ave);
 int launch_final_sum(Ctx* c, const double* partials, int nblk, int nv, double* out);  // fixed-order FP64 sum
+int launch_sum_f(Ctx* c, const float* x, long n, int nblk);  // FP64 block sums of x into c->partials
 void hii_globals(const Ctx* c, mxr_hii_globals* out);  // after a synchronised enqueue_only call
 int run_compute_ts(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_globals* out,
                    mxr_ts_tables* tables);
 int ensure_buffers(Ctx* c, int ngrid);
+
+// power spectrum (mxr_ps.hip)
+int ps_length(int N, double BoxSize, double Hubble_h);
+int run_power_spectrum(Ctx* c, const float* dT, int mem, double BoxSize, double Hubble_h, float* PS_k,
+                       float* PS_data, float* PS_error, int nb);
+int run_ps_counts(Ctx* c, double BoxSize, double Hubble_h, int nb, double* counts);
 int prof_begin(Ctx* c);                              // returns the event index (or -1)
 int prof_end(Ctx* c, int ev, int cls, double bytes); // records the launch
 int prof_resolve(Ctx* c);                            // after a stream sync

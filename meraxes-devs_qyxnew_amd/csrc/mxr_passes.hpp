@@ -22,7 +22,31 @@ struct CGridPtrs {
 struct CRealPtrs {
   const float* p[MAX_GRIDS];
   unsigned bf16;  // bit g: p[g] holds bfloat16 bit patterns (uint16), widened on load
+  // deldel loads (power spectrum, one grid): when dd_sum is set, p[0] is a real-layout grid
+  // v and each value loads as dd = (float)((v / ave - 1) volume / total), then
+  // (float)(dd ave), with ave = *dd_sum / total (ComputePowerSpectrum.c:97-104)
+  const double* dd_sum;
+  double dd_volume, dd_total, dd_rtotal;  // dd_rtotal = 1 / dd_total rounded to nearest
+  int dd_total_pow2;                      // dd_total is a power of two: dd_rtotal is exact
 };
+
+// x / d rounded to nearest from r = RN(1/d): q = RN(x r) is within an ulp of x / d, the residual
+// x - q d is exact through the FMA, and RN(q + residual r) is RN(x / d) (Markstein's theorem)
+__device__ __forceinline__ double div_rcp(double x, double d, double r)
+{
+  const double q = x * r;
+  return __fma_rn(__fma_rn(-q, d, x), r, q);
+}
+
+// pow2: total is a power of two, so multiplying by its reciprocal is the exact division
+__device__ __forceinline__ float deldel(float v, double ave, double rave, double volume, double total,
+                                        double rtotal, bool pow2)
+{
+#pragma clang fp contract(off)
+  const double t = (div_rcp((double)v, ave, rave) - 1) * volume;
+  const float x = (float)(pow2 ? t * rtotal : div_rcp(t, total, rtotal));
+  return (float)((double)x * ave);
+}
 
 __device__ __forceinline__ float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
@@ -209,25 +233,50 @@ __global__ void __launch_bounds__(NT) k_row_r2c(CRealPtrs in, GridPtrs out, long
   const bool valid = row < nrows;
   for (int m = threadIdx.x; m < N; m += NT)
     tw[m] = twg[m];
-  const float* __restrict__ src = pick(in, g) + (valid ? row : 0) * 2 * H;
+  // one uniform choice of the load form, then every load issued (from a clamped in-row index)
+  // before any is used; slots past the row's data are zeroed afterwards
+  const long rb = valid ? row : 0;
+  auto nc = [&](int q, int r) { return min(t + T * q + r * SL::B0, M - 1); };
   float2 v[SL::NREG];
+  if (in.dd_sum) {
+    const double ave = *in.dd_sum / in.dd_total, rave = 1.0 / ave;
+    const float* __restrict__ src = pick(in, g) + rb * N;
+#pragma unroll
+    for (int q = 0; q < SL::Q0; q++)
+#pragma unroll
+      for (int r = 0; r < SL::R0; r++)
+        v[q * SL::R0 + r] = *reinterpret_cast<const float2*>(src + 2 * nc(q, r));
+#pragma unroll
+    for (int i = 0; i < SL::Q0 * SL::R0; i++)
+      v[i] = make_float2(deldel(v[i].x, ave, rave, in.dd_volume, in.dd_total, in.dd_rtotal, in.dd_total_pow2),
+                         deldel(v[i].y, ave, rave, in.dd_volume, in.dd_total, in.dd_rtotal, in.dd_total_pow2));
+  } else if ((in.bf16 >> g) & 1u) {
+    // a row of 2H bfloat16 values is H 32-bit words
+    const unsigned* __restrict__ src = reinterpret_cast<const unsigned*>(pick(in, g)) + rb * H;
+    unsigned u[SL::Q0 * SL::R0];
+#pragma unroll
+    for (int q = 0; q < SL::Q0; q++)
+#pragma unroll
+      for (int r = 0; r < SL::R0; r++)
+        u[q * SL::R0 + r] = src[nc(q, r)];
+#pragma unroll
+    for (int i = 0; i < SL::Q0 * SL::R0; i++)
+      v[i] = make_float2(bf16_to_f32((unsigned short)(u[i] & 0xffffu)), bf16_to_f32((unsigned short)(u[i] >> 16)));
+  } else {
+    const float* __restrict__ src = pick(in, g) + rb * 2 * H;
+#pragma unroll
+    for (int q = 0; q < SL::Q0; q++)
+#pragma unroll
+      for (int r = 0; r < SL::R0; r++)
+        v[q * SL::R0 + r] = *reinterpret_cast<const float2*>(src + 2 * nc(q, r));
+  }
 #pragma unroll
   for (int q = 0; q < SL::Q0; q++) {
-    const int j = t + T * q;
+    const bool ok = valid && (SL::B0 % T == 0 || t + T * q < SL::B0);
 #pragma unroll
-    for (int r = 0; r < SL::R0; r++) {
-      const int n = j + r * SL::B0;
-      float2 x = make_float2(0.f, 0.f);
-      if (valid && (SL::B0 % T == 0 || j < SL::B0))
-        x = (in.bf16 >> g) & 1u
-              ? [&] {
-                  const unsigned v = *reinterpret_cast<const unsigned*>(
-                    reinterpret_cast<const unsigned short*>(pick(in, g)) + (valid ? row : 0) * 2 * H + 2 * n);
-                  return make_float2(bf16_to_f32((unsigned short)(v & 0xffffu)), bf16_to_f32((unsigned short)(v >> 16)));
-                }()
-              : *reinterpret_cast<const float2*>(src + 2 * n);
-      v[q * SL::R0 + r] = x;
-    }
+    for (int r = 0; r < SL::R0; r++)
+      if (!ok)
+        v[q * SL::R0 + r] = make_float2(0.f, 0.f);
   }
   __syncthreads();  // twiddles visible (single-stage transforms use them below)
   tile_fft<M, T, -1, N, LI, SyncWave>(v, lds, tw, c, t);

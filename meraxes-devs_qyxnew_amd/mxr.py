@@ -394,6 +394,20 @@ class MxrError(RuntimeError):
 _lib = None
 
 
+def _bind_torch_runtime():
+    """Import torch (when installed) before libmxr loads.
+
+    torch ships its own HIP and HSA runtime libraries under the same sonames as the system's
+    ones that libmxr links.  Loaded after torch, libmxr binds to the copies already in the
+    process, so the process holds one runtime and device pointers pass freely between torch
+    and libmxr.  Loaded the other way round, the process would hold two runtimes, and only the
+    first to initialise would see the GPU.  Without torch this does nothing."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Load libmxr.so (raises if it was not built: there is no fallback)."""
     global _lib
@@ -401,6 +415,7 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise MxrError(MXR_ERR_NO_DEVICE, "libmxr.so", f"not built at {LIB_PATH}")
+    _bind_torch_runtime()
     L = ctypes.CDLL(LIB_PATH)
     vp = ctypes.c_void_p
     L.mxr_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int]
@@ -432,6 +447,14 @@ def lib():
     L.mxr_brightness_temperature.argtypes = [vp, ctypes.POINTER(TbParams), ctypes.POINTER(TbGrids),
                                              ctypes.POINTER(ctypes.c_double)]
     L.mxr_brightness_temperature.restype = ctypes.c_int
+    L.mxr_ps_length.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    L.mxr_ps_length.restype = ctypes.c_int
+    fpp = ctypes.POINTER(ctypes.c_float)
+    L.mxr_power_spectrum.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, fpp, fpp,
+                                     fpp]
+    L.mxr_power_spectrum.restype = ctypes.c_int
+    L.mxr_ps_counts.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.mxr_ps_counts.restype = ctypes.c_int
     L.mxr_compute_ts.argtypes = [vp, ctypes.POINTER(TsParams), ctypes.POINTER(TsGrids), ctypes.POINTER(TsGlobals),
                                  ctypes.POINTER(TsTables)]
     L.mxr_compute_ts.restype = ctypes.c_int
@@ -631,6 +654,27 @@ class Context:
         check(lib().mxr_velocity_gradient(self._h, _ptr(box), mem, slab_nx, grid_dim, box_size),
               "mxr_velocity_gradient")
 
+    def power_spectrum(self, delta_T, box_size: float, hubble_h: float):
+        """mxr_power_spectrum of a real grid (numpy or torch).  Returns (PS_k, PS_data, PS_error) as
+        float32 arrays of ps_length() bins."""
+        mem = _mem_of([delta_T])
+        if mem == MXR_MEM_DEVICE:
+            _sync_torch()
+        nb = ps_length(self.N, box_size, hubble_h)
+        out = [np.zeros(nb, np.float32) for _ in range(3)]
+        fpp = ctypes.POINTER(ctypes.c_float)
+        check(lib().mxr_power_spectrum(self._h, _ptr(delta_T), mem, box_size, hubble_h, nb,
+                                       *[o.ctypes.data_as(fpp) for o in out]), "mxr_power_spectrum")
+        return tuple(out)
+
+    def ps_counts(self, box_size: float, hubble_h: float) -> np.ndarray:
+        """mxr_ps_counts: modes per power-spectrum bin (float64)."""
+        nb = ps_length(self.N, box_size, hubble_h)
+        out = np.zeros(nb, np.float64)
+        check(lib().mxr_ps_counts(self._h, box_size, hubble_h, nb, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+              "mxr_ps_counts")
+        return out
+
     def compute_ts(self, params: TsParams, deltax, stars, sfr_histories, xray_luminosity_histories, n_histories: int,
                    x_e_box_prev, x_e_box, Tk_box, TS_box, stored_fcoll: np.ndarray, want_tables: bool = True):
         """One ComputeTs step.  Returns (TsGlobals, TsTables or None); stored_fcoll (host float64, one
@@ -687,6 +731,14 @@ def _with_dummy_state(arrays: dict) -> dict:
     for k in ("xH", "z_at_ionization", "J_21_at_ionization", "r_bubble", "temp_kinetic_all_gas"):
         a.setdefault(k, ref)
     return a
+
+
+def ps_length(grid_dim: int, box_size: float, hubble_h: float) -> int:
+    """mxr_ps_length: the number of k bins of the power spectrum."""
+    n = lib().mxr_ps_length(grid_dim, box_size, hubble_h)
+    if n < 0:
+        check(n, "mxr_ps_length")
+    return n
 
 
 def wisdom_lookup(path: str, arch: str, cus: int, grid_dim: int, filter_type: int, spin: int = 0):

@@ -612,3 +612,102 @@ int oracle_brightness_temperature(const mxr_tb_params* p, const mxr_tb_grids* g,
     *volume_ave_Tb = Ave_Tb;
   return 0;
 }
+
+/* ---------------------------------------------------------------- power spectrum */
+/* Initialise_PowerSpectrum (ComputePowerSpectrum.c:18-42): bins of k_ceil = delta_k 1.35^i in
+ * float while k_ceil < k_max = delta_k N, with delta_k = (float)(2 pi / (BoxSize / h)). */
+int oracle_ps_length(int N, double BoxSize, double Hubble_h)
+{
+  const double box_size = BoxSize / Hubble_h;
+  const float k_factor = 1.35;
+  const float delta_k = (float)(2. * M_PI / box_size);
+  const float k_max = delta_k * N;
+  int num_bins = 0;
+  float k_ceil = delta_k;
+  while (k_ceil < k_max) {
+    num_bins++;
+    k_ceil *= k_factor;
+  }
+  return num_bins;
+}
+
+/* Compute_PS (ComputePowerSpectrum.c:44-255, no mini-haloes): delta_T (real layout) ->
+ * PS_k, PS_data, PS_error (PS_Length each). */
+int oracle_power_spectrum(int N, double BoxSize, double Hubble_h, const float* delta_T, float* PS_k, float* PS_data,
+                          float* PS_error, int PS_Length)
+{
+  if (N < 2 || !delta_T || !PS_k || !PS_data || !PS_error || PS_Length != oracle_ps_length(N, BoxSize, Hubble_h))
+    return MXR_ERR_INVALID_ARG;
+  const long H = N / 2 + 1;
+  const double box_size = BoxSize / Hubble_h;
+  const float volume = powf((float)(float)box_size, 3);
+  const double total_n_cells = pow((double)N, 3);
+  double ave = 0.0;
+  for (long i = 0; i < (long)N * N * N; i++)
+    ave += delta_T[i];
+  ave /= total_n_cells;
+  float* dd = (float*)calloc((size_t)2 * N * N * H, sizeof(float));
+  float* spec = (float*)malloc(sizeof(float) * 2 * (size_t)N * N * H);
+  if (!dd || !spec) {
+    free(dd);
+    free(spec);
+    return MXR_ERR_OOM;
+  }
+  for (int ii = 0; ii < N; ii++)
+    for (int jj = 0; jj < N; jj++)
+      for (int kk = 0; kk < N; kk++) {
+        const long ir = kk + (long)N * (jj + (long)N * ii), ip = kk + 2 * H * (jj + (long)N * ii);
+        dd[ip] = (float)((delta_T[ir] / ave - 1) * volume / (float)total_n_cells);
+        dd[ip] *= ave;
+      }
+  oracle_r2c3d_f32(N, dd, spec);
+  free(dd);
+  const float k_factor = 1.35;
+  const float delta_k = (float)(2. * M_PI / box_size);
+  const float k_first_bin_ceil = delta_k;
+  const float k_max = delta_k * N;
+  double* p_box = (double*)calloc((size_t)PS_Length, sizeof(double));
+  double* k_ave = (double*)calloc((size_t)PS_Length, sizeof(double));
+  unsigned long long* in_bin_ct = (unsigned long long*)calloc((size_t)PS_Length, sizeof(unsigned long long));
+  const int HII_middle = N / 2;
+  for (int n_x = 0; n_x < N; n_x++) {
+    float k_x = n_x * delta_k;
+    if (n_x > HII_middle)
+      k_x = (n_x - N) * delta_k;
+    for (int n_y = 0; n_y < N; n_y++) {
+      float k_y = n_y * delta_k;
+      if (n_y > HII_middle)
+        k_y = (n_y - N) * delta_k;
+      for (int n_z = 0; n_z <= HII_middle; n_z++) {
+        const float k_z = n_z * delta_k;
+        const float k_mag = (float)sqrt(k_x * k_x + k_y * k_y + k_z * k_z);
+        int ct = 0;
+        float k_floor = 0;
+        float k_ceil = k_first_bin_ceil;
+        while (k_ceil < k_max) {
+          if ((k_mag >= k_floor) && (k_mag < k_ceil)) {
+            const long idx = n_z + H * (n_y + (long)N * n_x);
+            const double re = spec[2 * idx], im = spec[2 * idx + 1];
+            in_bin_ct[ct]++;
+            p_box[ct] += pow(k_mag, 3) * pow(hypot(re, im), 2.) / (2.0 * M_PI * M_PI * volume);
+            k_ave[ct] += k_mag;
+            break;
+          }
+          ct++;
+          k_floor = k_ceil;
+          k_ceil *= k_factor;
+        }
+      }
+    }
+  }
+  free(spec);
+  for (int ii = 0; ii < PS_Length; ii++) {
+    PS_k[ii] = (float)(k_ave[ii] / (double)in_bin_ct[ii]);
+    PS_data[ii] = (float)(p_box[ii] / (double)in_bin_ct[ii]);
+    PS_error[ii] = (float)((p_box[ii] / (double)in_bin_ct[ii]) / sqrt((double)in_bin_ct[ii]));
+  }
+  free(p_box);
+  free(k_ave);
+  free(in_bin_ct);
+  return 0;
+}

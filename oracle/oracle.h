@@ -59,6 +59,9 @@ int oracle_hii_sample_f32(const mxr_hii_params* p, const mxr_hii_grids* g, int n
 
 int oracle_brightness_temperature(const mxr_tb_params* p, const mxr_tb_grids* g, double* volume_ave_Tb);
 int oracle_velocity_gradient(float* box, int slab_nx, int grid_dim, double BoxSize);
+int oracle_ps_length(int N, double BoxSize, double Hubble_h);
+int oracle_power_spectrum(int N, double BoxSize, double Hubble_h, const float* delta_T, float* PS_k, float* PS_data,
+                          float* PS_error, int PS_Length);
 
 /* One spin-temperature step on host grids, with the grid-independent tables `tb` the library
  * reported for the same call (mxr_ts_tables).  red0 (optional, 3 doubles) receives the R_ct = 0

@@ -68,6 +68,10 @@ def lib():
     L.oracle_velocity_gradient.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]
     L.oracle_velocity_gradient.restype = ctypes.c_int
     L.oracle_num_threads.restype = ctypes.c_int
+    L.oracle_ps_length.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    L.oracle_ps_length.restype = ctypes.c_int
+    L.oracle_power_spectrum.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, fp, fp, fp, fp, ctypes.c_int]
+    L.oracle_power_spectrum.restype = ctypes.c_int
     for name in ("oracle_ts_step_f32", "oracle_ts_step_f64"):
         getattr(L, name).argtypes = [ctypes.POINTER(mxr.TsParams), ctypes.POINTER(mxr.TsTables),
                                      ctypes.POINTER(mxr.TsGrids), ctypes.POINTER(mxr.TsGlobals), fp, fp]
@@ -178,6 +182,22 @@ def velocity_gradient(box: np.ndarray, N: int, box_size: float) -> np.ndarray:
     if rc:
         raise mxr.MxrError(rc, "oracle_velocity_gradient")
     return out
+
+
+def ps_length(N: int, box_size: float, hubble_h: float) -> int:
+    return lib().oracle_ps_length(N, box_size, hubble_h)
+
+
+def power_spectrum(delta_T: np.ndarray, N: int, box_size: float, hubble_h: float):
+    """The oracle's Compute_PS of a real grid: (PS_k, PS_data, PS_error) float32 arrays."""
+    nb = ps_length(N, box_size, hubble_h)
+    dT = np.ascontiguousarray(delta_T, np.float32).reshape(-1)
+    assert dT.size == N ** 3
+    out = [np.zeros(nb, np.float32) for _ in range(3)]
+    rc = lib().oracle_power_spectrum(N, box_size, hubble_h, _p(dT), *[_p(o) for o in out], nb)
+    if rc:
+        raise mxr.MxrError(rc, "oracle_power_spectrum")
+    return tuple(out)
 
 
 def ts_step(params, tables, deltax, stars, sfr_histories, xray_histories, n_hist, x_e_box_prev, Tk_box,

@@ -52,6 +52,11 @@ def test_radius_schedule_matches_oracle(N, box, n):
     assert r[0] == 50.0 or r[0] == pytest.approx(0.620350491 * box)
 
 
+@pytest.mark.parametrize("N,box,h", [(16, 20.0, 0.678), (64, 67.8, 0.678), (512, 400.0, 0.678), (160, 150.0, 0.7)])
+def test_ps_length_matches_oracle(N, box, h):
+    assert mxr.ps_length(N, box, h) == O.ps_length(N, box, h)
+
+
 def test_rtom_matches_oracle():
     p = mxr.hii_params(64, 67.8)
     for R in (0.5, 3.0, 42.0):
@@ -68,3 +73,7 @@ def test_null_arguments_are_errors():
     assert L.mxr_get_tuning(None, ctypes.byref(t)) == mxr.MXR_ERR_INVALID_ARG
     assert L.mxr_set_tuning(None, ctypes.byref(t)) == mxr.MXR_ERR_INVALID_ARG
     assert L.mxr_profile_enable(None, 1) == mxr.MXR_ERR_INVALID_ARG
+    assert L.mxr_power_spectrum(None, None, 0, 400.0, 0.678, 21, None, None, None) == mxr.MXR_ERR_INVALID_ARG
+    assert L.mxr_ps_counts(None, 400.0, 0.678, 21, None) == mxr.MXR_ERR_INVALID_ARG
+    assert L.mxr_ps_length(0, 400.0, 0.678) == mxr.MXR_ERR_INVALID_ARG
+    assert L.mxr_ps_length(512, -1.0, 0.678) == mxr.MXR_ERR_INVALID_ARG
