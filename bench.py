@@ -169,6 +169,83 @@ def ts_leg(args, torch):
     return out
 
 
+def baryon_leg(args, torch):
+    """construct_baryon_grids (mxr_construct_baryon_grids) on a device-resident catalogue and grids."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import mxr
+    import oracle_lib as O
+    import synth
+
+    N, box, ngal, nstore = args.baryon_N, args.box, args.baryon_gals, args.baryon_nstore
+    p = mxr.baryon_params(N, box, spin_temp=True, bh=True, agn_xray=mxr.MXR_AGN_XRAY_BOTH, nstore=nstore,
+                          BlackHoleMassLimitReion=1e-5)
+    log(f"bench: baryon grids N={N} galaxies={ngal} histories={nstore}")
+    host = synth.galaxies(ngal, box, seed=args.seed, n_halos=ngal // 8)
+    gals = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in host.items()}
+    live = int((host["Type"] <= 2).sum())
+    del host
+    npad = N * N * 2 * (N // 2 + 1)
+    grids = {k: torch.zeros(npad * (nstore if "histories" in k else 1), device="cuda") for k in mxr.BARYON_GRIDS}
+    ctx = mxr.Context(N)
+    for _ in range(max(args.warmup, 1)):
+        ctx.construct_baryon_grids(p, gals, grids)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.baryon_steps):
+        ctx.construct_baryon_grids(p, gals, grids)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.baryon_steps
+    # the same with ring histories (mxr_baryon_grids.history_head): no slot moves
+    head = np.zeros(1, np.int32)
+    ctx.construct_baryon_grids(p, gals, grids, history_head=head)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.baryon_steps):
+        ctx.construct_baryon_grids(p, gals, grids, history_head=head)
+    torch.cuda.synchronize()
+    ms_ring = 1e3 * (time.perf_counter() - t0) / args.baryon_steps
+    # HBM model per call: the grid side (8 grids cleared and written, 4 histories shifted by a
+    # slot: read nstore - 1 and write nstore grids) plus the galaxy side (keys 24 B, an 8-bit
+    # radix sort of key + index over 4 passes 128 B, packing the fields 72 + 64 B, the deposit's
+    # keys, indices and one 64-byte record 72 B)
+    grid_bytes = npad * 4.0 * (8 + 4 * (2 * nstore - 1))
+    gal_bytes = ngal * (24.0 + 128.0 + 136.0 + 72.0)
+    model = grid_bytes + gal_bytes
+    ring_bytes = npad * 4.0 * (8 + 4) + gal_bytes
+    out = {"metric": f"galaxies/s (construct_baryon_grids, {N}^3, {nstore} history slots)",
+           "value": ngal / (ms / 1e3), "unit": "galaxies/s", "ms_per_call": round(ms, 3), "steps": args.baryon_steps,
+           "config": {"workload": f"construct_baryon_grids N={N} box={box} cMpc/h, {ngal} galaxies ({live} live, "
+                                  f"clustered), spin temperature, BH feedback, hard+soft AGN X-ray, "
+                                  f"NstoreSnapshots_Heating={nstore}", "grid_dim": N, "galaxies": ngal},
+           "call_roofline": {"bound": "hbm", "model_bytes_per_call": model, "grid_bytes": grid_bytes,
+                             "galaxy_bytes": gal_bytes, "achieved": model / (ms / 1e3) / 1e9,
+                             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": model / (ms / 1e3) / HBM_PEAK},
+           "ring_histories": {"ms_per_call": round(ms_ring, 3), "value": ngal / (ms_ring / 1e3),
+                              "model_bytes_per_call": ring_bytes,
+                              "frac": ring_bytes / (ms_ring / 1e3) / HBM_PEAK,
+                              "note": "history_head set: the new grid takes the oldest slot, nothing moves"}}
+    ctx.close()
+    del gals, grids
+    torch.cuda.empty_cache()
+    if args.baryon_cpu_N > 0:
+        n = args.baryon_cpu_N
+        ng = ngal * n ** 3 // N ** 3
+        hg = synth.galaxies(ng, box * n / N, seed=args.seed, n_halos=max(1, ng // 8))
+        hgrids = synth.baryon_grids(n, nstore, seed=1)
+        pc = mxr.baryon_params(n, box * n / N, spin_temp=True, bh=True, agn_xray=mxr.MXR_AGN_XRAY_BOTH,
+                               nstore=nstore, BlackHoleMassLimitReion=1e-5)
+        t0 = time.perf_counter()
+        O.construct_baryon_grids(pc, hg, hgrids)
+        s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": ng / s, "unit": "galaxies/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle construct_baryon_grids at {n}^3 with {ng} galaxies (same density "
+                                         f"and flags, {nstore} slots), {s:.2f} s, serial"}
+    log("bench: baryon " + json.dumps(out))
+    return out
+
+
 def ps_leg(args, torch):
     """The 21-cm power spectrum (mxr_power_spectrum) of a device-resident brightness box."""
     import numpy as np
@@ -349,6 +426,11 @@ def main():
     ap.add_argument("--ts-steps", type=int, default=5)
     ap.add_argument("--ts-stars-to-matter", type=float, default=0.020055)
     ap.add_argument("--ts-cpu-N", type=int, default=64, help="grid of the spin-temperature CPU sample (0: skip)")
+    ap.add_argument("--baryon-N", type=int, default=512, help="galaxies-to-grids grid (0: skip that leg)")
+    ap.add_argument("--baryon-gals", type=int, default=16_000_000)
+    ap.add_argument("--baryon-nstore", type=int, default=14, help="history slots (41 snapshots to z = 8)")
+    ap.add_argument("--baryon-steps", type=int, default=5)
+    ap.add_argument("--baryon-cpu-N", type=int, default=256, help="grid of the galaxies-to-grids CPU sample (0: skip)")
     ap.add_argument("--ps-N", type=int, default=512, help="power-spectrum grid (0: skip that leg)")
     ap.add_argument("--ps-steps", type=int, default=10)
     ap.add_argument("--ps-cpu-N", type=int, default=512, help="grid of the power-spectrum CPU sample (0: skip)")
@@ -483,6 +565,7 @@ def main():
     ts = ts_leg(args, torch) if args.ts_N > 0 else None
     batch = batch_leg(args, torch) if args.batch > 0 else None
     ps = ps_leg(args, torch) if args.ps_N > 0 else None
+    baryon = baryon_leg(args, torch) if args.baryon_N > 0 else None
 
     value = N ** 3 * n_radii / (ms_step / 1e3)
     out = {
@@ -517,6 +600,7 @@ def main():
         "spin_temperature": ts,
         "batch": batch,
         "power_spectrum": ps,
+        "baryon_grids": baryon,
     }
     print(json.dumps(out), flush=True)
 

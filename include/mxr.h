@@ -303,6 +303,94 @@ int mxr_power_spectrum(mxr_ctx* ctx, const float* delta_T, int mem, double BoxSi
  * for callers that combine spectra of several boxes and to verify the binning.              */
 int mxr_ps_counts(mxr_ctx* ctx, double BoxSize, double Hubble_h, int PS_Length, double* counts);
 
+/* ---- galaxies to grids (construct_baryon_grids) -------------------------------------------- */
+/* Flag_IncludeAGNXray (run_params_t.physics): which AGN X-ray emissivity grids are built     */
+#define MXR_AGN_XRAY_NONE 0
+#define MXR_AGN_XRAY_BOTH 1
+#define MXR_AGN_XRAY_HARD 2
+#define MXR_AGN_XRAY_SOFT 3
+
+typedef struct mxr_baryon_params {
+  int ReionGridDim;
+  double BoxSize;                  /* Mpc/h, the side pos_to_ngp divides positions by       */
+  int Flag_IncludeSpinTemp;        /* sfr, X-ray and AGN grids and their histories            */
+  int Flag_BHFeedback;             /* effective_bhm and effective_bhar grids                  */
+  int Flag_InstantaneousSFR;       /* sfr grids from Sfr (1), or GrossStellarMass / t_sfr (0)  */
+  int Flag_IncludeAGNXray;         /* MXR_AGN_XRAY_*                                          */
+  double BlackHoleMassLimitReion;  /* BH grids count galaxies with BlackHoleMass >= this      */
+  double sfr_timescale;            /* ReionSfrTimescale * hubble_time(snapshot), internal     */
+  int NstoreSnapshots_Heating;     /* padded grids in each *_histories array, slot 0 newest   */
+  /* galaxy X-ray luminosity (calculate_galaxy_xray_luminosity, reionization.c:1982-2023)    */
+  int Flag_XrayMetallicity;
+  double LXrayGal;
+  double XrayLogLNorm;
+  double XrayMetallicitySlope;
+  double XrayMetallicitySolar;
+  double XrayOxygenAbundanceFactor;
+  double NuXrayThreshold;
+  double NuXraySoftCut;
+  double SpecIndexXrayGal;
+  double UnitMass_in_g;            /* run_globals.units                                        */
+  double UnitTime_in_s;
+} mxr_baryon_params;
+
+/* Galaxy properties as structure of arrays (galaxy_t fields, src/meraxes.h), n entries each,
+ * in the order the reference accumulates them (its galaxy_to_slab_map order).  Arrays a
+ * call's flags do not read may be NULL.                                                     */
+typedef struct mxr_galaxies {
+  int mem;                         /* MXR_MEM_HOST or MXR_MEM_DEVICE                          */
+  long n;
+  const float* Pos;                /* [n][3], Mpc/h                                            */
+  const int* Type;                 /* > 2: dead, skipped                                       */
+  const double* FescWeightedGSM;
+  const double* FescWeightedSfr;
+  const double* Sfr;
+  const double* GrossStellarMass;
+  const double* ColdGas;           /* Flag_XrayMetallicity                                     */
+  const double* MetalsColdGas;
+  const double* BlackHoleMass;     /* Flag_BHFeedback or AGN X-ray grids                      */
+  const double* EffectiveBHM;
+  const double* EffectiveBHAR;
+  const double* BHXrayEmissivity_hard;
+  const double* BHXrayEmissivity_soft;
+} mxr_galaxies;
+
+/* Output grids in the padded real layout (reion_grids_t names); a *_histories array holds
+ * NstoreSnapshots_Heating padded grids.  Grids a call's flags do not build may be NULL.    */
+typedef struct mxr_baryon_grids {
+  int mem;
+  float* stars;
+  float* weighted_sfr;
+  float* effective_bhm;
+  float* effective_bhar;
+  float* sfr;
+  float* sfr_histories;
+  float* xray_luminosity;
+  float* xray_luminosity_histories;
+  float* BHXrayEmissivity_hard;
+  float* bh_xray_histories_hard;
+  float* BHXrayEmissivity_soft;
+  float* bh_xray_histories_soft;
+  int* history_head;    /* NULL: the reference's layout, each history shifted by one slot (a
+                           move of NstoreSnapshots_Heating - 1 grids per history).  Otherwise the
+                           histories are rings (history s in slot (*history_head + s) % n, as
+                           mxr_ts_grids.history_head reads them): the new grid goes into slot
+                           (*history_head + n - 1) % n, which becomes *history_head, and nothing
+                           moves.  The ring's padding entries are unspecified.               */
+} mxr_baryon_grids;
+
+/* Nearest-grid-point deposit of the galaxies into the grids, for one rank holding the whole
+ * grid, without stochasticity or mini-haloes: every cell sums its galaxies in the given order
+ * with the reference's float/double arithmetic, negative sums clamp to zero, sfr becomes a
+ * rate, and each history array shifts by one slot before slot 0 takes the new grid.
+ * *n_below_bh_limit (may be NULL) gets the live galaxies under BlackHoleMassLimitReion
+ * (Flag_BHFeedback).  A position outside [0, BoxSize) that pos_to_ngp would map below 0 is
+ * MXR_ERR_INVALID_ARG (the reference asserts).
+ * Replaces construct_baryon_grids (src/core/reionization.c:2025-2591, declared
+ * reionization.h:73), with pos_to_ngp (misc_tools.c:207-217).                            */
+int mxr_construct_baryon_grids(mxr_ctx* ctx, const mxr_baryon_params* params, const mxr_galaxies* galaxies,
+                               const mxr_baryon_grids* grids, long* n_below_bh_limit);
+
 /* ---- spin temperature ----------------------------------------------------------------------- */
 #define MXR_TS_MAX_STEPS 64  /* TsNumFilterSteps upper bound                                  */
 #define MXR_TS_MAX_OPS 16    /* source-grid operations before one shell's forward FFT          */
@@ -357,6 +445,9 @@ typedef struct mxr_ts_grids {
   float* TS_box;                  /* out real                                               */
   double* stored_fcoll;           /* host, in/out: collapse fraction per snapshot (the entry
                                      [snapshot] is written, [snapshot - 1] read)            */
+  int history_head;               /* slot of the newest history grid: history s lives in slot
+                                     (history_head + s) % n_histories.  0 is the reference's
+                                     layout; a ring kept by mxr_construct_baryon_grids moves it */
 } mxr_ts_grids;
 
 /* Grid-independent per-shell tables and the history plan ComputeTs builds on the host

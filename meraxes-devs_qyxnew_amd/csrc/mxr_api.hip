@@ -328,6 +328,7 @@ void mxr_destroy(mxr_ctx* h)
     (void)hipEventDestroy(c.ev_red);
   (void)hipFree(c.partials);
   (void)hipFree(c.ps_part);
+  (void)hipFree(c.bg_ws);
   (void)hipFree(c.sums);
   if (c.sums_host)
     (void)hipHostFree(c.sums_host);
@@ -991,6 +992,52 @@ int mxr_power_spectrum(mxr_ctx* h, const float* delta_T, int mem, double BoxSize
     return MXR_ERR_HIP;
   }
   return run_power_spectrum(&h->c, delta_T, mem, BoxSize, Hubble_h, PS_k, PS_data, PS_error, nb);
+}
+
+int mxr_construct_baryon_grids(mxr_ctx* h, const mxr_baryon_params* p, const mxr_galaxies* gal,
+                               const mxr_baryon_grids* g, long* n_below_bh_limit)
+{
+  if (!h || !p || !gal || !g || p->ReionGridDim != h->c.N || !(p->BoxSize > 0) || gal->n < 0 ||
+      gal->n >= 2147483647L || (gal->mem != MXR_MEM_HOST && gal->mem != MXR_MEM_DEVICE) ||
+      (g->mem != MXR_MEM_HOST && g->mem != MXR_MEM_DEVICE)) {
+    set_error("bad construct_baryon_grids arguments");
+    return MXR_ERR_INVALID_ARG;
+  }
+  const int spin = p->Flag_IncludeSpinTemp, bh = p->Flag_BHFeedback, agn = p->Flag_IncludeAGNXray;
+  if (agn < MXR_AGN_XRAY_NONE || agn > MXR_AGN_XRAY_SOFT || (spin && p->NstoreSnapshots_Heating < 1) ||
+      (spin && !p->Flag_InstantaneousSFR && !(p->sfr_timescale > 0))) {
+    set_error("construct_baryon_grids: Flag_IncludeAGNXray %d, NstoreSnapshots_Heating %d, sfr_timescale %g", agn,
+              p->NstoreSnapshots_Heating, p->sfr_timescale);
+    return MXR_ERR_INVALID_ARG;
+  }
+  const bool hard = spin && (agn == MXR_AGN_XRAY_BOTH || agn == MXR_AGN_XRAY_HARD);
+  const bool soft = spin && (agn == MXR_AGN_XRAY_BOTH || agn == MXR_AGN_XRAY_SOFT);
+  const bool want_sfr = spin && p->Flag_InstantaneousSFR, want_gsm = spin && !p->Flag_InstantaneousSFR;
+  const bool gal_ok = gal->n == 0 ||
+                      (gal->Pos && gal->Type && gal->FescWeightedGSM && gal->FescWeightedSfr &&
+                       (!want_sfr || gal->Sfr) && (!want_gsm || gal->GrossStellarMass) &&
+                       (!(spin && p->Flag_XrayMetallicity) || (gal->ColdGas && gal->MetalsColdGas)) &&
+                       (!(bh || hard || soft) || gal->BlackHoleMass) &&
+                       (!bh || (gal->EffectiveBHM && gal->EffectiveBHAR)) && (!hard || gal->BHXrayEmissivity_hard) &&
+                       (!soft || gal->BHXrayEmissivity_soft));
+  const bool grid_ok = g->stars && g->weighted_sfr && (!bh || (g->effective_bhm && g->effective_bhar)) &&
+                       (!spin || (g->sfr && g->sfr_histories && g->xray_luminosity && g->xray_luminosity_histories)) &&
+                       (!hard || (g->BHXrayEmissivity_hard && g->bh_xray_histories_hard)) &&
+                       (!soft || (g->BHXrayEmissivity_soft && g->bh_xray_histories_soft));
+  if (!gal_ok || !grid_ok) {
+    set_error("construct_baryon_grids: a galaxy array or grid these flags need is NULL");
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (spin && g->history_head && (*g->history_head < 0 || *g->history_head >= p->NstoreSnapshots_Heating)) {
+    set_error("construct_baryon_grids: history_head %d is not a slot of %d", *g->history_head,
+              p->NstoreSnapshots_Heating);
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (hipSetDevice(h->c.device) != hipSuccess) {
+    set_error("hipSetDevice(%d) failed", h->c.device);
+    return MXR_ERR_HIP;
+  }
+  return run_construct_baryon_grids(&h->c, p, gal, g, n_below_bh_limit);
 }
 
 int mxr_ps_counts(mxr_ctx* h, double BoxSize, double Hubble_h, int PS_Length, double* counts)

@@ -132,6 +132,9 @@ struct Ctx {
   float ps_geom_dk = 0;
   int ps_geom_nb = 0;
   double ps_geom[2 * MXR_PS_MAXB] = {};
+  // galaxies to grids: sort keys and values, the sort's storage, staged galaxy arrays
+  void* bg_ws = nullptr;
+  size_t bg_ws_cap = 0;            // bytes
   double fin_total_n_cells = 0, fin_sfr_conv = 0;
   int fin_n_radii = 0;
   int fin_bh = 0;
@@ -165,6 +168,10 @@ int ps_length(int N, double BoxSize, double Hubble_h);
 int run_power_spectrum(Ctx* c, const float* dT, int mem, double BoxSize, double Hubble_h, float* PS_k,
                        float* PS_data, float* PS_error, int nb);
 int run_ps_counts(Ctx* c, double BoxSize, double Hubble_h, int nb, double* counts);
+
+// galaxies to grids (mxr_baryon.hip)
+int run_construct_baryon_grids(Ctx* c, const mxr_baryon_params* p, const mxr_galaxies* gal,
+                               const mxr_baryon_grids* grids, long* n_below);
 int prof_begin(Ctx* c);                              // returns the event index (or -1)
 int prof_end(Ctx* c, int ev, int cls, double bytes); // records the launch
 int prof_resolve(Ctx* c);                            // after a stream sync

@@ -38,11 +38,13 @@ struct TsOpsArgs {
   TsOp ops[TS_MAX_OPS];
 };
 
-__global__ void k_ts_source(TsOpsArgs oa, const float* __restrict__ hs, const float* __restrict__ hx, long hstride,
+// hs0, hx0: the current snapshot's grids (history 0, wherever a ring keeps it)
+__global__ void k_ts_source(TsOpsArgs oa, const float* __restrict__ hs, const float* __restrict__ hx,
+                            const float* __restrict__ hs0, const float* __restrict__ hx0, long hstride,
                             float* __restrict__ ss, float* __restrict__ sx, long n)
 {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    float v[2] = {oa.init ? hs[i] : ss[i], oa.init ? hx[i] : sx[i]};
+    float v[2] = {oa.init ? hs0[i] : ss[i], oa.init ? hx0[i] : sx[i]};
     for (int k = 0; k < oa.nops; k++) {
       const TsOp o = oa.ops[k];
 #pragma unroll
@@ -450,6 +452,10 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
                 g->n_histories);
       return MXR_ERR_INVALID_ARG;
     }
+    if (g->history_head < 0 || g->history_head >= g->n_histories) {
+      set_error("ComputeTs: history_head %d is not a slot of %d", g->history_head, g->n_histories);
+      return MXR_ERR_INVALID_ARG;
+    }
   }
   RC(ts_ensure(c, pl.homogeneous ? 0 : F));
   TsDev* hd = reinterpret_cast<TsDev*>(c->ts_dev_host);
@@ -474,6 +480,7 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
   float* xe_out = g->x_e_box;
   const float* hs = g->sfr_histories;
   const float* hx = g->xray_luminosity_histories;
+  int hist_head = pl.homogeneous ? 0 : g->history_head;  // ring slot of history 0 on the device
   const int nstage = pl.homogeneous ? 0 : pl.hist_max + 1 > 1 ? pl.hist_max + 1 : 1;
   if (g->mem == MXR_MEM_HOST) {
     auto stage = [&](int slot, const float* host, long nfl, bool in) -> int {
@@ -507,14 +514,22 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
         MXR_HIP(hipMalloc(&c->ts_hist, sizeof(float) * need));
         c->ts_hist_cap = need;
       }
-      MXR_HIP(hipMemcpyAsync(c->ts_hist, hs, sizeof(float) * nstage * npad, hipMemcpyHostToDevice, c->stream));
-      MXR_HIP(hipMemcpyAsync(c->ts_hist + (size_t)nstage * npad, hx, sizeof(float) * nstage * npad,
-                             hipMemcpyHostToDevice, c->stream));
+      // staged in logical order (history s at s), whatever ring slot holds it
+      for (int s = 0; s < nstage; s++) {
+        const size_t src = (size_t)((g->history_head + s) % g->n_histories) * npad;
+        MXR_HIP(hipMemcpyAsync(c->ts_hist + (size_t)s * npad, hs + src, sizeof(float) * npad, hipMemcpyHostToDevice,
+                               c->stream));
+        MXR_HIP(hipMemcpyAsync(c->ts_hist + (size_t)(nstage + s) * npad, hx + src, sizeof(float) * npad,
+                               hipMemcpyHostToDevice, c->stream));
+      }
       hs = c->ts_hist;
       hx = c->ts_hist + (size_t)nstage * npad;
+      hist_head = 0;
     }
   }
 
+  const float* hs0 = hs + (size_t)hist_head * npad;
+  const float* hx0 = hx + (size_t)hist_head * npad;
   const int nblk = (int)ew_blocks(nreal);
   double* sums = c->sums_host;
   MXR_HIP(hipEventRecord(c->ev0, c->stream));
@@ -605,18 +620,20 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
             int nread = 1;
             for (int k = 0; k < pl.nops[r]; k++) {
               oa.ops[k] = pl.ops[r][k];
+              if (oa.ops[k].kind >= TS_OP_LOAD)
+                oa.ops[k].hist = (hist_head + oa.ops[k].hist) % g->n_histories;
               nread += (pl.ops[r][k].kind >= TS_OP_LOAD) ? 1 : 0;
             }
             RC(T(K_TS_SRC, 2.0 * 4.0 * npad * (nread + 1), [&] {
-              hipLaunchKernelGGL(k_ts_source, dim3(ew_blocks(npad)), dim3(256), 0, c->stream, oa, hs, hx, npad,
-                                 c->ts_src[0], c->ts_src[1], npad);
+              hipLaunchKernelGGL(k_ts_source, dim3(ew_blocks(npad)), dim3(256), 0, c->stream, oa, hs, hx, hs0, hx0,
+                                 npad, c->ts_src[0], c->ts_src[1], npad);
               MXR_HIP(hipGetLastError());
               return MXR_OK;
             }));
             src_valid = true;
           }
-          rin.p[0] = src_valid ? c->ts_src[0] : hs;
-          rin.p[1] = src_valid ? c->ts_src[1] : hx;
+          rin.p[0] = src_valid ? c->ts_src[0] : hs0;
+          rin.p[1] = src_valid ? c->ts_src[1] : hx0;
           RC(T(K_TS_FWD, 2.0 * 3 * 2 * cplx, [&] {
             RC(K::r2c(c, rin, spec, 2));
             RC((K::template col<-1, COL_PLAIN>(c, cspec, spec, 2, K::ygeo(), none, 1.0)));

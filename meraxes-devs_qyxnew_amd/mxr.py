@@ -30,6 +30,7 @@ MXR_ERR_NO_DEVICE = -6
 MXR_ERR_FILTER = -7
 MXR_ERR_NO_WISDOM = -8
 
+MXR_AGN_XRAY_NONE, MXR_AGN_XRAY_BOTH, MXR_AGN_XRAY_HARD, MXR_AGN_XRAY_SOFT = 0, 1, 2, 3
 MXR_MEM_HOST = 0
 MXR_MEM_DEVICE = 1
 MXR_PATH_FUSED = 0
@@ -139,6 +140,49 @@ class KernelStat(ctypes.Structure):
                 ("alg_bytes", ctypes.c_double)]
 
 
+class BaryonParams(ctypes.Structure):
+    _fields_ = [
+        ("ReionGridDim", ctypes.c_int),
+        ("BoxSize", ctypes.c_double),
+        ("Flag_IncludeSpinTemp", ctypes.c_int),
+        ("Flag_BHFeedback", ctypes.c_int),
+        ("Flag_InstantaneousSFR", ctypes.c_int),
+        ("Flag_IncludeAGNXray", ctypes.c_int),
+        ("BlackHoleMassLimitReion", ctypes.c_double),
+        ("sfr_timescale", ctypes.c_double),
+        ("NstoreSnapshots_Heating", ctypes.c_int),
+        ("Flag_XrayMetallicity", ctypes.c_int),
+        ("LXrayGal", ctypes.c_double),
+        ("XrayLogLNorm", ctypes.c_double),
+        ("XrayMetallicitySlope", ctypes.c_double),
+        ("XrayMetallicitySolar", ctypes.c_double),
+        ("XrayOxygenAbundanceFactor", ctypes.c_double),
+        ("NuXrayThreshold", ctypes.c_double),
+        ("NuXraySoftCut", ctypes.c_double),
+        ("SpecIndexXrayGal", ctypes.c_double),
+        ("UnitMass_in_g", ctypes.c_double),
+        ("UnitTime_in_s", ctypes.c_double),
+    ]
+
+
+GALAXY_FIELDS = ("Pos", "Type", "FescWeightedGSM", "FescWeightedSfr", "Sfr", "GrossStellarMass", "ColdGas",
+                 "MetalsColdGas", "BlackHoleMass", "EffectiveBHM", "EffectiveBHAR", "BHXrayEmissivity_hard",
+                 "BHXrayEmissivity_soft")
+
+
+class Galaxies(ctypes.Structure):
+    _fields_ = [("mem", ctypes.c_int), ("n", ctypes.c_long)] + [(f, ctypes.c_void_p) for f in GALAXY_FIELDS]
+
+
+BARYON_GRIDS = ("stars", "weighted_sfr", "effective_bhm", "effective_bhar", "sfr", "sfr_histories",
+                "xray_luminosity", "xray_luminosity_histories", "BHXrayEmissivity_hard", "bh_xray_histories_hard",
+                "BHXrayEmissivity_soft", "bh_xray_histories_soft")
+
+
+class BaryonGrids(ctypes.Structure):
+    _fields_ = [("mem", ctypes.c_int)] + [(f, ctypes.c_void_p) for f in BARYON_GRIDS] + [("history_head", ctypes.c_void_p)]
+
+
 class TbParams(ctypes.Structure):
     _fields_ = [
         ("ReionGridDim", ctypes.c_int),
@@ -216,6 +260,7 @@ class TsGrids(ctypes.Structure):
         ("Tk_box", ctypes.c_void_p),
         ("TS_box", ctypes.c_void_p),
         ("stored_fcoll", ctypes.c_void_p),
+        ("history_head", ctypes.c_int),
     ]
 
 
@@ -348,6 +393,37 @@ def tb_params(N: int, redshift: float = 8.0, spin_temp: bool = False, box: float
     return p
 
 
+def baryon_params(N: int, box: float, spin_temp: bool = False, bh: bool = False, agn_xray: int = 0,
+                  instantaneous_sfr: bool = True, xray_metallicity: bool = False, nstore: int = 1,
+                  sfr_timescale: float = 0.05, **over) -> BaryonParams:
+    """construct_baryon_grids parameters with input/params/defaults.par values (LXrayGal 3.16e40,
+    Kyritsis et al. X-ray metallicity terms, NuXray 500/2000 eV, alpha 1, Genesis units)."""
+    p = BaryonParams()
+    p.ReionGridDim = N
+    p.BoxSize = box
+    p.Flag_IncludeSpinTemp = int(spin_temp)
+    p.Flag_BHFeedback = int(bh)
+    p.Flag_InstantaneousSFR = int(instantaneous_sfr)
+    p.Flag_IncludeAGNXray = agn_xray
+    p.BlackHoleMassLimitReion = -1.0
+    p.sfr_timescale = sfr_timescale
+    p.NstoreSnapshots_Heating = nstore
+    p.Flag_XrayMetallicity = int(xray_metallicity)
+    p.LXrayGal = 3.16e40
+    p.XrayLogLNorm = 40.68
+    p.XrayMetallicitySlope = -1.22
+    p.XrayMetallicitySolar = 0.0134
+    p.XrayOxygenAbundanceFactor = 1.0
+    p.NuXrayThreshold = 500.0
+    p.NuXraySoftCut = 2000.0
+    p.SpecIndexXrayGal = 1.0
+    p.UnitMass_in_g = GENESIS["UnitMass_in_g"]
+    p.UnitTime_in_s = GENESIS["UnitLength_in_cm"] / GENESIS["UnitVelocity_in_cm_per_s"]
+    for k, v in over.items():
+        setattr(p, k, v)
+    return p
+
+
 # Spin-temperature defaults: input/params/defaults.par:51-68, 245, 274-282; OmegaR 0 and
 # wLambda -1 as in input/params/simulations/Genesis_*.par.
 TS_DEFAULTS = dict(OmegaR=0.0, wLambda=-1.0, Y_He=0.24, ReionRtoMFilterType=0, TsNumFilterSteps=40,
@@ -455,6 +531,9 @@ def lib():
     L.mxr_power_spectrum.restype = ctypes.c_int
     L.mxr_ps_counts.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.mxr_ps_counts.restype = ctypes.c_int
+    L.mxr_construct_baryon_grids.argtypes = [vp, ctypes.POINTER(BaryonParams), ctypes.POINTER(Galaxies),
+                                             ctypes.POINTER(BaryonGrids), ctypes.POINTER(ctypes.c_long)]
+    L.mxr_construct_baryon_grids.restype = ctypes.c_int
     L.mxr_compute_ts.argtypes = [vp, ctypes.POINTER(TsParams), ctypes.POINTER(TsGrids), ctypes.POINTER(TsGlobals),
                                  ctypes.POINTER(TsTables)]
     L.mxr_compute_ts.restype = ctypes.c_int
@@ -675,10 +754,37 @@ class Context:
               "mxr_ps_counts")
         return out
 
+    def construct_baryon_grids(self, params: BaryonParams, galaxies: dict, grids: dict, history_head=None) -> int:
+        """mxr_construct_baryon_grids.  galaxies maps GALAXY_FIELDS names to arrays (numpy or torch,
+        Pos as (n, 3) float32, Type int32, the rest float64); grids maps BARYON_GRIDS names to
+        padded float32 arrays (histories NstoreSnapshots_Heating grids long), written in place.
+        history_head: None for the reference's shifted histories, or a one-element int32 numpy
+        array holding the ring's head, updated in place.
+        Returns the number of live galaxies below BlackHoleMassLimitReion."""
+        g = Galaxies()
+        g.mem = _mem_of([v for v in galaxies.values() if v is not None])
+        g.n = len(galaxies["Type"])
+        for f in GALAXY_FIELDS:
+            setattr(g, f, _ptr(galaxies.get(f)))
+        o = BaryonGrids()
+        o.mem = _mem_of([v for v in grids.values() if v is not None])
+        for f in BARYON_GRIDS:
+            setattr(o, f, _ptr(grids.get(f)))
+        if history_head is not None:
+            assert isinstance(history_head, np.ndarray) and history_head.dtype == np.int32 and history_head.size == 1
+            o.history_head = history_head.ctypes.data
+        if MXR_MEM_DEVICE in (g.mem, o.mem):
+            _sync_torch()
+        below = ctypes.c_long()
+        check(lib().mxr_construct_baryon_grids(self._h, ctypes.byref(params), ctypes.byref(g), ctypes.byref(o),
+                                               ctypes.byref(below)), "mxr_construct_baryon_grids")
+        return below.value
+
     def compute_ts(self, params: TsParams, deltax, stars, sfr_histories, xray_luminosity_histories, n_histories: int,
-                   x_e_box_prev, x_e_box, Tk_box, TS_box, stored_fcoll: np.ndarray, want_tables: bool = True):
+                   x_e_box_prev, x_e_box, Tk_box, TS_box, stored_fcoll: np.ndarray, want_tables: bool = True,
+                   history_head: int = 0):
         """One ComputeTs step.  Returns (TsGlobals, TsTables or None); stored_fcoll (host float64, one
-        entry per snapshot) is updated in place."""
+        entry per snapshot) is updated in place.  history_head: ring slot of the newest history grid."""
         g = TsGrids()
         g.mem = _mem_of([deltax, stars, sfr_histories, xray_luminosity_histories, x_e_box_prev, x_e_box, Tk_box,
                          TS_box])
@@ -690,6 +796,7 @@ class Context:
         g.x_e_box_prev, g.x_e_box, g.Tk_box, g.TS_box = _ptr(x_e_box_prev), _ptr(x_e_box), _ptr(Tk_box), _ptr(TS_box)
         assert isinstance(stored_fcoll, np.ndarray) and stored_fcoll.dtype == np.float64
         g.stored_fcoll = stored_fcoll.ctypes.data
+        g.history_head = history_head
         out = TsGlobals()
         tab = TsTables() if want_tables else None
         check(lib().mxr_compute_ts(self._h, ctypes.byref(params), ctypes.byref(g), ctypes.byref(out),

@@ -186,3 +186,50 @@ def ts_inputs(N: int, box: float, seed: int, ZZ, snapshot: int, n_hist: int, dev
         xr_h[s * npad:(s + 1) * npad] = (sfr.double() * (to_msun_yr * LXRAY_GAL / 1e40)).float()
         del sfr
     return dict(deltax=cur["deltax"], stars=cur["stars"], sfr_histories=sfr_h, xray_luminosity_histories=xr_h)
+
+
+def galaxies(n: int, box: float, seed: int, n_halos: int | None = None, dead_fraction: float = 0.02,
+             negative_fraction: float = 0.0) -> dict:
+    """A synthetic galaxy catalogue (galaxy_t fields as structure of arrays, numpy, host).
+
+    Galaxies cluster around n_halos centres (a few tens of kpc/h scatter), so many cells hold
+    several galaxies and the order of their float sums matters.  Masses and rates are lognormal
+    in internal units (1e10 Msun/h, Myr-scale times); a few galaxies are dead (Type 3), and
+    negative_fraction of them carry negative stellar quantities (to exercise the clamps)."""
+    rng = np.random.default_rng(seed)
+    n_halos = n_halos or max(1, n // 8)
+    centres = rng.uniform(0.0, box, (n_halos, 3))
+    host = rng.integers(0, n_halos, n)
+    pos = centres[host] + rng.normal(0.0, 0.02 * box / 64, (n, 3))
+    pos = np.mod(pos, box).astype(np.float32)
+    pos[pos >= np.float32(box)] = 0.0
+    t = rng.choice(np.array([0, 1, 2, 3], np.int32), n, p=[0.5, 0.5 - dead_fraction, 0.0, dead_fraction])
+    lg = lambda mu, s: np.exp(rng.normal(mu, s, n))
+    gsm = lg(math.log(1e-3), 1.5)
+    sfr = lg(math.log(1e-2), 1.5)
+    out = dict(
+        Pos=pos, Type=t.astype(np.int32),
+        FescWeightedGSM=0.1 * gsm, FescWeightedSfr=0.1 * sfr, Sfr=sfr, GrossStellarMass=gsm,
+        ColdGas=lg(math.log(1e-2), 1.0), MetalsColdGas=lg(math.log(1e-4), 1.0),
+        BlackHoleMass=lg(math.log(1e-5), 2.0), EffectiveBHM=lg(math.log(1e-6), 2.0),
+        EffectiveBHAR=lg(math.log(1e-8), 2.0), BHXrayEmissivity_hard=lg(math.log(1e-4), 2.0),
+        BHXrayEmissivity_soft=lg(math.log(1e-4), 2.0))
+    if negative_fraction > 0:
+        neg = rng.uniform(size=n) < negative_fraction
+        for k in ("FescWeightedGSM", "FescWeightedSfr", "Sfr", "GrossStellarMass", "EffectiveBHM"):
+            out[k][neg] *= -1.0
+    return out
+
+
+def baryon_grids(N: int, nstore: int, seed: int = 0) -> dict:
+    """Output grids for construct_baryon_grids (padded float32, numpy): histories hold nstore
+    grids of earlier snapshots (random, so that the shift is visible), the rest random too."""
+    rng = np.random.default_rng(seed)
+    npad = N * N * 2 * (N // 2 + 1)
+    g = {}
+    for k in ("stars", "weighted_sfr", "effective_bhm", "effective_bhar", "sfr", "xray_luminosity",
+              "BHXrayEmissivity_hard", "BHXrayEmissivity_soft"):
+        g[k] = rng.uniform(-1, 1, npad).astype(np.float32)
+    for k in ("sfr_histories", "xray_luminosity_histories", "bh_xray_histories_hard", "bh_xray_histories_soft"):
+        g[k] = rng.uniform(-1, 1, npad * nstore).astype(np.float32)
+    return g

@@ -59,6 +59,10 @@ int oracle_hii_sample_f32(const mxr_hii_params* p, const mxr_hii_grids* g, int n
 
 int oracle_brightness_temperature(const mxr_tb_params* p, const mxr_tb_grids* g, double* volume_ave_Tb);
 int oracle_velocity_gradient(float* box, int slab_nx, int grid_dim, double BoxSize);
+/* construct_baryon_grids for one rank (reionization.c:2025-2591), host arrays. */
+int oracle_construct_baryon_grids(const mxr_baryon_params* p, const mxr_galaxies* g, const mxr_baryon_grids* out,
+                                  long* n_below_bh_limit);
+
 int oracle_ps_length(int N, double BoxSize, double Hubble_h);
 int oracle_power_spectrum(int N, double BoxSize, double Hubble_h, const float* delta_T, float* PS_k, float* PS_data,
                           float* PS_error, int PS_Length);

@@ -68,6 +68,9 @@ def lib():
     L.oracle_velocity_gradient.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]
     L.oracle_velocity_gradient.restype = ctypes.c_int
     L.oracle_num_threads.restype = ctypes.c_int
+    L.oracle_construct_baryon_grids.argtypes = [ctypes.POINTER(mxr.BaryonParams), ctypes.POINTER(mxr.Galaxies),
+                                                ctypes.POINTER(mxr.BaryonGrids), ctypes.POINTER(ctypes.c_long)]
+    L.oracle_construct_baryon_grids.restype = ctypes.c_int
     L.oracle_ps_length.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.oracle_ps_length.restype = ctypes.c_int
     L.oracle_power_spectrum.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, fp, fp, fp, fp, ctypes.c_int]
@@ -182,6 +185,26 @@ def velocity_gradient(box: np.ndarray, N: int, box_size: float) -> np.ndarray:
     if rc:
         raise mxr.MxrError(rc, "oracle_velocity_gradient")
     return out
+
+
+def construct_baryon_grids(params, galaxies: dict, grids: dict) -> int:
+    """The oracle's construct_baryon_grids on host arrays; grids are written in place."""
+    g = mxr.Galaxies()
+    g.mem = 0
+    g.n = len(galaxies["Type"])
+    for f in mxr.GALAXY_FIELDS:
+        a = galaxies.get(f)
+        setattr(g, f, _p(a) if a is not None else None)
+    o = mxr.BaryonGrids()
+    o.mem = 0
+    for f in mxr.BARYON_GRIDS:
+        a = grids.get(f)
+        setattr(o, f, _p(a) if a is not None else None)
+    below = ctypes.c_long()
+    rc = lib().oracle_construct_baryon_grids(ctypes.byref(params), ctypes.byref(g), ctypes.byref(o), ctypes.byref(below))
+    if rc:
+        raise mxr.MxrError(rc, "oracle_construct_baryon_grids")
+    return below.value
 
 
 def ps_length(N: int, box_size: float, hubble_h: float) -> int:
