@@ -169,6 +169,53 @@ def ts_leg(args, torch):
     return out
 
 
+def grid_leg(args, torch):
+    """Input-grid preparation (mxr_prepare_grid): a device-resident n_hi^3 density grid smoothed
+    and subsampled to N^3 and normalised."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import mxr
+    import oracle_lib as O
+
+    n_hi, N, box = args.grid_hi, args.grid_hi // 2, args.box
+    g = torch.Generator(device="cuda").manual_seed(args.seed)
+    hi = torch.rand(n_hi ** 3, device="cuda", generator=g).mul_(16.0).contiguous()
+    p = mxr.grid_params(box, npart=n_hi ** 3, part_mass=1.0)
+    out = torch.zeros(N * N * 2 * (N // 2 + 1), device="cuda")
+    ctx = mxr.Context(N)
+    ctx.prepare_grid(hi, n_hi, mxr.MXR_GRID_DENSITY, box, p, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.grid_steps):
+        ctx.prepare_grid(hi, n_hi, mxr.MXR_GRID_DENSITY, box, p, out)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.grid_steps
+    f = (n_hi // 2 + 1) / n_hi
+    # per hi-res cell: pad read 4 + write 4, r2c 4 + 8 f, y and x passes 16 f each, filter 16 f,
+    # inverse x and y 16 f each, c2r 8 f + 4; the subsample reads and writes 1/8 of that in N^3
+    model = n_hi ** 3 * (8 + 4 + 8 * f + 5 * 16 * f + 8 * f + 4) + N ** 3 * 8
+    out_j = {"metric": f"input cells/s (prepare_grid {n_hi}^3 -> {N}^3, smoothing)", "value": n_hi ** 3 / (ms / 1e3),
+             "unit": "cells/s", "ms_per_call": round(ms, 3), "steps": args.grid_steps,
+             "config": {"workload": f"smooth_grid + subsample_grid + overdensity, {n_hi}^3 -> {N}^3, box {box} cMpc/h",
+                        "n_hi": n_hi, "grid_dim": N},
+             "call_roofline": {"bound": "hbm", "model_bytes_per_call": model, "achieved": model / (ms / 1e3) / 1e9,
+                               "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": model / (ms / 1e3) / HBM_PEAK}}
+    ctx.close()
+    del hi, out
+    torch.cuda.empty_cache()
+    if args.grid_cpu_hi > 0:
+        nh = args.grid_cpu_hi
+        x = np.random.default_rng(args.seed).uniform(0, 16, nh ** 3).astype(np.float32)
+        t0 = time.perf_counter()
+        O.prepare_grid(nh // 2, x, nh, mxr.MXR_GRID_DENSITY, box * nh / n_hi, p)
+        s = time.perf_counter() - t0
+        out_j["cpu_baseline"] = {"value": nh ** 3 / s, "unit": "cells/s", "cores": int(O.lib().oracle_num_threads()),
+                                 "kind": "port", "sample": f"oracle prepare_grid {nh}^3 -> {nh // 2}^3, {s:.2f} s"}
+    log("bench: grid " + json.dumps(out_j))
+    return out_j
+
+
 def baryon_leg(args, torch):
     """construct_baryon_grids (mxr_construct_baryon_grids) on a device-resident catalogue and grids."""
     import numpy as np
@@ -426,6 +473,9 @@ def main():
     ap.add_argument("--ts-steps", type=int, default=5)
     ap.add_argument("--ts-stars-to-matter", type=float, default=0.020055)
     ap.add_argument("--ts-cpu-N", type=int, default=64, help="grid of the spin-temperature CPU sample (0: skip)")
+    ap.add_argument("--grid-hi", type=int, default=512, help="input-grid preparation from this grid (0: skip)")
+    ap.add_argument("--grid-steps", type=int, default=5)
+    ap.add_argument("--grid-cpu-hi", type=int, default=512, help="grid of the input-grid CPU sample (0: skip)")
     ap.add_argument("--baryon-N", type=int, default=512, help="galaxies-to-grids grid (0: skip that leg)")
     ap.add_argument("--baryon-gals", type=int, default=16_000_000)
     ap.add_argument("--baryon-nstore", type=int, default=14, help="history slots (41 snapshots to z = 8)")
@@ -566,6 +616,7 @@ def main():
     batch = batch_leg(args, torch) if args.batch > 0 else None
     ps = ps_leg(args, torch) if args.ps_N > 0 else None
     baryon = baryon_leg(args, torch) if args.baryon_N > 0 else None
+    grid = grid_leg(args, torch) if args.grid_hi > 0 else None
 
     value = N ** 3 * n_radii / (ms_step / 1e3)
     out = {
@@ -601,6 +652,7 @@ def main():
         "batch": batch,
         "power_spectrum": ps,
         "baryon_grids": baryon,
+        "input_grid": grid,
     }
     print(json.dumps(out), flush=True)
 

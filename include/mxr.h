@@ -303,6 +303,53 @@ int mxr_power_spectrum(mxr_ctx* ctx, const float* delta_T, int mem, double BoxSi
  * for callers that combine spectra of several boxes and to verify the binning.              */
 int mxr_ps_counts(mxr_ctx* ctx, double BoxSize, double Hubble_h, int PS_Length, double* counts);
 
+/* ---- input density and velocity grids (read_grids*.c) --------------------------------------- */
+/* enum grid_prop (src/core/read_grids.h) */
+#define MXR_GRID_DENSITY 0
+#define MXR_GRID_X_VELOCITY 1
+#define MXR_GRID_Y_VELOCITY 2
+#define MXR_GRID_Z_VELOCITY 3
+
+/* Header of a gbpTrees grid file (read_header, src/core/read_grids-gbptrees.c:38-86): three
+ * ints n_cell, three doubles box_size (cMpc/h), ints n_grids (4) and ma_scheme, then per grid a
+ * 32-byte identifier and n_cell^3 floats (x slowest).                                      */
+typedef struct mxr_grid_file_info {
+  int n_cell[3];
+  double box_size[3];
+  int n_grids;
+  int ma_scheme;
+} mxr_grid_file_info;
+
+int mxr_grid_file_read_info(const char* fname, mxr_grid_file_info* info);
+
+typedef struct mxr_grid_params {
+  double BoxSize;                  /* run_globals.params.BoxSize (Mpc/h): the smoothing scale  */
+  double Hubble_h;
+  long long NPart;                 /* density only                                            */
+  double PartMass;                 /* density only                                            */
+  int TreesID;                     /* MXR_TREES_*: selects the density normalisation and floor */
+} mxr_grid_params;
+
+/* One rank's read_grid post-processing on a grid already in memory: hi is the file's grid at
+ * n_hi^3 in the real layout (host or device, mem_in); n_hi a multiple of the context's
+ * dimension N and, when larger, itself a supported dimension.  Larger grids are smoothed
+ * (r2c, /n_hi^3, real-space top-hat of radius BoxSize/N/2, c2r) and every (n_hi/N)-th point
+ * kept; then, for MXR_GRID_DENSITY, the overdensity max(rho mean_inv - 1, floor) with the
+ * file's box size file_box.  slab: the padded real N^3 grid (mem_out); its padding entries
+ * are zero.  For callers with their own reader (the VELOCIraptor HDF5 grids).
+ * Replaces smooth_grid and subsample_grid (src/core/read_grids.c:128-287) and the density
+ * step of read_grid__gbptrees (read_grids-gbptrees.c:201-218) and read_grid__velociraptor
+ * (read_grids-velociraptor.c:213-230).                                                    */
+int mxr_prepare_grid(mxr_ctx* ctx, const float* hi, int n_hi, int mem_in, int property, double file_box,
+                     const mxr_grid_params* params, float* slab, int mem_out);
+
+/* read_grid__gbptrees (src/core/read_grids-gbptrees.c:88-231) for one rank: reads grid
+ * `property` of the file and prepares it as mxr_prepare_grid does (TreesID is taken as
+ * MXR_TREES_GBPTREES).  MXR_ERR_INVALID_ARG for an unreadable file, n_grids != 4, a
+ * non-cubic grid or one not a multiple of N (the reference aborts).                       */
+int mxr_read_grid_gbptrees(mxr_ctx* ctx, const char* fname, int property, const mxr_grid_params* params, float* slab,
+                           int mem_out);
+
 /* ---- galaxies to grids (construct_baryon_grids) -------------------------------------------- */
 /* Flag_IncludeAGNXray (run_params_t.physics): which AGN X-ray emissivity grids are built     */
 #define MXR_AGN_XRAY_NONE 0

@@ -329,6 +329,8 @@ void mxr_destroy(mxr_ctx* h)
   (void)hipFree(c.partials);
   (void)hipFree(c.ps_part);
   (void)hipFree(c.bg_ws);
+  if (c.hires)
+    mxr_destroy(c.hires);
   (void)hipFree(c.sums);
   if (c.sums_host)
     (void)hipHostFree(c.sums_host);
@@ -992,6 +994,89 @@ int mxr_power_spectrum(mxr_ctx* h, const float* delta_T, int mem, double BoxSize
     return MXR_ERR_HIP;
   }
   return run_power_spectrum(&h->c, delta_T, mem, BoxSize, Hubble_h, PS_k, PS_data, PS_error, nb);
+}
+
+int mxr_grid_file_read_info(const char* fname, mxr_grid_file_info* info)
+{
+  if (!fname || !info) {
+    set_error("bad grid_file_read_info arguments");
+    return MXR_ERR_INVALID_ARG;
+  }
+  return read_grid_file_info(fname, info, nullptr);
+}
+
+static int grid_args_ok(mxr_ctx* h, int property, const mxr_grid_params* p, const float* slab, int mem_out)
+{
+  if (!h || !p || !slab || property < MXR_GRID_DENSITY || property > MXR_GRID_Z_VELOCITY || !(p->BoxSize > 0) ||
+      (mem_out != MXR_MEM_HOST && mem_out != MXR_MEM_DEVICE) ||
+      (property == MXR_GRID_DENSITY && (p->NPart <= 0 || !(p->PartMass > 0) || !(p->Hubble_h > 0)))) {
+    set_error("bad grid arguments");
+    return 0;
+  }
+  return 1;
+}
+
+}  // extern "C" (reopened below)
+
+// the output grid on the device (staged for a host grid), then the work, then the copy back
+template <class F>
+static int with_device_slab(Ctx& c, float* slab, int mem_out, F&& work)
+{
+  if (mem_out == MXR_MEM_DEVICE)
+    return work(slab);
+  const size_t bytes = sizeof(float) * 2 * n_complex(c.N);
+  float* d = nullptr;
+  MXR_HIP(hipMalloc(&d, bytes));
+  int rc = work(d);
+  if (rc == MXR_OK) {
+    hipError_t e = hipMemcpyAsync(slab, d, bytes, hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess)
+      e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess)
+      rc = hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+extern "C" {
+
+int mxr_prepare_grid(mxr_ctx* h, const float* hi, int n_hi, int mem_in, int property, double file_box,
+                     const mxr_grid_params* p, float* slab, int mem_out)
+{
+  if (!grid_args_ok(h, property, p, slab, mem_out) || !hi || (mem_in != MXR_MEM_HOST && mem_in != MXR_MEM_DEVICE) ||
+      !(file_box > 0)) {
+    set_error("bad prepare_grid arguments");
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (n_hi < h->c.N || n_hi % h->c.N != 0) {
+    set_error("n_cell %d is not a multiple of ReionGridDim %d (at least as large)", n_hi, h->c.N);
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (n_hi != h->c.N && !supported_dim(n_hi)) {
+    set_error("smoothing a %d^3 grid is not supported (need one of 16,32,64,96,128,160,256,512)", n_hi);
+    return MXR_ERR_UNSUPPORTED;
+  }
+  if (hipSetDevice(h->c.device) != hipSuccess) {
+    set_error("hipSetDevice(%d) failed", h->c.device);
+    return MXR_ERR_HIP;
+  }
+  return with_device_slab(h->c, slab, mem_out, [&](float* d) {
+    return run_prepare_grid(&h->c, hi, n_hi, mem_in, property, file_box, p, d);
+  });
+}
+
+int mxr_read_grid_gbptrees(mxr_ctx* h, const char* fname, int property, const mxr_grid_params* p, float* slab,
+                           int mem_out)
+{
+  if (!fname || !grid_args_ok(h, property, p, slab, mem_out))
+    return MXR_ERR_INVALID_ARG;
+  if (hipSetDevice(h->c.device) != hipSuccess) {
+    set_error("hipSetDevice(%d) failed", h->c.device);
+    return MXR_ERR_HIP;
+  }
+  return with_device_slab(h->c, slab, mem_out,
+                          [&](float* d) { return run_read_grid_gbptrees(&h->c, fname, property, p, d); });
 }
 
 int mxr_construct_baryon_grids(mxr_ctx* h, const mxr_baryon_params* p, const mxr_galaxies* gal,

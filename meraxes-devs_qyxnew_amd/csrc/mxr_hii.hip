@@ -1290,6 +1290,22 @@ int run_filter(Ctx* c, float2* box, int ix0, int nx, int N, float dk, float R, i
   return MXR_OK;
 }
 
+int run_c2r_store(Ctx* c, float2* spec)
+{
+  CGridPtrs cs{};
+  cs.p[0] = spec;
+  FiltArgs none{};
+  none.N = c->N;
+  HiiEpi ea{};
+  switch (c->N) {
+#define X(n)                                                                                                         \
+  case n: return Hii<n>::template c2r<C2R_STORE>(c, cs, 1, none, ea);
+    MXR_DIMS(X)
+#undef X
+    default: set_error("unsupported grid dimension %d", c->N); return MXR_ERR_INVALID_ARG;
+  }
+}
+
 int run_velocity_gradient(Ctx* c, float2* box, int slab_nx, int grid_dim, float dk)
 {
   const int H = grid_dim / 2 + 1;

@@ -132,6 +132,8 @@ struct Ctx {
   float ps_geom_dk = 0;
   int ps_geom_nb = 0;
   double ps_geom[2 * MXR_PS_MAXB] = {};
+  // a context of an input grid's dimension, for smoothing it (created on first use)
+  mxr_ctx* hires = nullptr;
   // galaxies to grids: sort keys and values, the sort's storage, staged galaxy arrays
   void* bg_ws = nullptr;
   size_t bg_ws_cap = 0;            // bytes
@@ -168,6 +170,13 @@ int ps_length(int N, double BoxSize, double Hubble_h);
 int run_power_spectrum(Ctx* c, const float* dT, int mem, double BoxSize, double Hubble_h, float* PS_k,
                        float* PS_data, float* PS_error, int nb);
 int run_ps_counts(Ctx* c, double BoxSize, double Hubble_h, int nb, double* counts);
+
+// input grids (mxr_grid.hip)
+int run_c2r_store(Ctx* c, float2* spec);  // in-place c2r of one half spectrum (mxr_hii.hip)
+int read_grid_file_info(const char* fname, mxr_grid_file_info* info, long* data_offset_of_grid0);
+int run_prepare_grid(Ctx* c, const float* hi, int n_hi, int mem_in, int property, double file_box,
+                     const mxr_grid_params* p, float* out);
+int run_read_grid_gbptrees(Ctx* c, const char* fname, int property, const mxr_grid_params* p, float* out);
 
 // galaxies to grids (mxr_baryon.hip)
 int run_construct_baryon_grids(Ctx* c, const mxr_baryon_params* p, const mxr_galaxies* gal,

@@ -140,6 +140,36 @@ class KernelStat(ctypes.Structure):
                 ("alg_bytes", ctypes.c_double)]
 
 
+MXR_GRID_DENSITY, MXR_GRID_X_VELOCITY, MXR_GRID_Y_VELOCITY, MXR_GRID_Z_VELOCITY = 0, 1, 2, 3
+
+
+class GridFileInfo(ctypes.Structure):
+    _fields_ = [("n_cell", ctypes.c_int * 3), ("box_size", ctypes.c_double * 3), ("n_grids", ctypes.c_int),
+                ("ma_scheme", ctypes.c_int)]
+
+
+class GridParams(ctypes.Structure):
+    _fields_ = [("BoxSize", ctypes.c_double), ("Hubble_h", ctypes.c_double), ("NPart", ctypes.c_longlong),
+                ("PartMass", ctypes.c_double), ("TreesID", ctypes.c_int)]
+
+
+def grid_params(box: float, npart: int = 2160 ** 3, part_mass: float = 1.0, hubble_h: float | None = None,
+                trees_id: int = 1) -> GridParams:
+    p = GridParams()
+    p.BoxSize = box
+    p.Hubble_h = GENESIS["Hubble_h"] if hubble_h is None else hubble_h
+    p.NPart = npart
+    p.PartMass = part_mass
+    p.TreesID = trees_id
+    return p
+
+
+def grid_file_info(fname: str) -> GridFileInfo:
+    info = GridFileInfo()
+    check(lib().mxr_grid_file_read_info(fname.encode(), ctypes.byref(info)), "mxr_grid_file_read_info")
+    return info
+
+
 class BaryonParams(ctypes.Structure):
     _fields_ = [
         ("ReionGridDim", ctypes.c_int),
@@ -531,6 +561,13 @@ def lib():
     L.mxr_power_spectrum.restype = ctypes.c_int
     L.mxr_ps_counts.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.mxr_ps_counts.restype = ctypes.c_int
+    L.mxr_grid_file_read_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(GridFileInfo)]
+    L.mxr_grid_file_read_info.restype = ctypes.c_int
+    L.mxr_prepare_grid.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                   ctypes.POINTER(GridParams), vp, ctypes.c_int]
+    L.mxr_prepare_grid.restype = ctypes.c_int
+    L.mxr_read_grid_gbptrees.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(GridParams), vp, ctypes.c_int]
+    L.mxr_read_grid_gbptrees.restype = ctypes.c_int
     L.mxr_construct_baryon_grids.argtypes = [vp, ctypes.POINTER(BaryonParams), ctypes.POINTER(Galaxies),
                                              ctypes.POINTER(BaryonGrids), ctypes.POINTER(ctypes.c_long)]
     L.mxr_construct_baryon_grids.restype = ctypes.c_int
@@ -753,6 +790,23 @@ class Context:
         check(lib().mxr_ps_counts(self._h, box_size, hubble_h, nb, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
               "mxr_ps_counts")
         return out
+
+    def prepare_grid(self, hi, n_hi: int, prop: int, file_box: float, params: GridParams, slab):
+        """mxr_prepare_grid: hi is the n_hi^3 real-layout grid, slab the padded N^3 output (numpy or
+        torch each, written in place)."""
+        mi, mo = _mem_of([hi]), _mem_of([slab])
+        if MXR_MEM_DEVICE in (mi, mo):
+            _sync_torch()
+        check(lib().mxr_prepare_grid(self._h, _ptr(hi), n_hi, mi, prop, file_box, ctypes.byref(params), _ptr(slab), mo),
+              "mxr_prepare_grid")
+
+    def read_grid_gbptrees(self, fname: str, prop: int, params: GridParams, slab):
+        """mxr_read_grid_gbptrees into the padded N^3 slab (numpy or torch)."""
+        mo = _mem_of([slab])
+        if mo == MXR_MEM_DEVICE:
+            _sync_torch()
+        check(lib().mxr_read_grid_gbptrees(self._h, fname.encode(), prop, ctypes.byref(params), _ptr(slab), mo),
+              "mxr_read_grid_gbptrees")
 
     def construct_baryon_grids(self, params: BaryonParams, galaxies: dict, grids: dict, history_head=None) -> int:
         """mxr_construct_baryon_grids.  galaxies maps GALAXY_FIELDS names to arrays (numpy or torch,

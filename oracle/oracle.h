@@ -59,6 +59,11 @@ int oracle_hii_sample_f32(const mxr_hii_params* p, const mxr_hii_grids* g, int n
 
 int oracle_brightness_temperature(const mxr_tb_params* p, const mxr_tb_grids* g, double* volume_ave_Tb);
 int oracle_velocity_gradient(float* box, int slab_nx, int grid_dim, double BoxSize);
+/* Input grids: smooth_grid + subsample_grid + the density step (one rank); out is padded N^3. */
+int oracle_prepare_grid(int N, const float* hi, int n_hi, int property, double file_box, const mxr_grid_params* p,
+                        float* out);
+int oracle_read_grid_gbptrees(const char* fname, int property, int N, const mxr_grid_params* p, float* out);
+
 /* construct_baryon_grids for one rank (reionization.c:2025-2591), host arrays. */
 int oracle_construct_baryon_grids(const mxr_baryon_params* p, const mxr_galaxies* g, const mxr_baryon_grids* out,
                                   long* n_below_bh_limit);

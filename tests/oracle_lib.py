@@ -68,6 +68,12 @@ def lib():
     L.oracle_velocity_gradient.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]
     L.oracle_velocity_gradient.restype = ctypes.c_int
     L.oracle_num_threads.restype = ctypes.c_int
+    L.oracle_prepare_grid.argtypes = [ctypes.c_int, fp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                      ctypes.POINTER(mxr.GridParams), fp]
+    L.oracle_prepare_grid.restype = ctypes.c_int
+    L.oracle_read_grid_gbptrees.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(mxr.GridParams),
+                                            fp]
+    L.oracle_read_grid_gbptrees.restype = ctypes.c_int
     L.oracle_construct_baryon_grids.argtypes = [ctypes.POINTER(mxr.BaryonParams), ctypes.POINTER(mxr.Galaxies),
                                                 ctypes.POINTER(mxr.BaryonGrids), ctypes.POINTER(ctypes.c_long)]
     L.oracle_construct_baryon_grids.restype = ctypes.c_int
@@ -184,6 +190,23 @@ def velocity_gradient(box: np.ndarray, N: int, box_size: float) -> np.ndarray:
     rc = lib().oracle_velocity_gradient(out.ctypes.data, N, N, box_size)
     if rc:
         raise mxr.MxrError(rc, "oracle_velocity_gradient")
+    return out
+
+
+def prepare_grid(N: int, hi: np.ndarray, n_hi: int, prop: int, file_box: float, params) -> np.ndarray:
+    out = np.zeros(N * N * 2 * (N // 2 + 1), np.float32)
+    rc = lib().oracle_prepare_grid(N, _p(np.ascontiguousarray(hi, np.float32)), n_hi, prop, file_box,
+                                   ctypes.byref(params), _p(out))
+    if rc:
+        raise mxr.MxrError(rc, "oracle_prepare_grid")
+    return out
+
+
+def read_grid_gbptrees(fname: str, prop: int, N: int, params) -> np.ndarray:
+    out = np.zeros(N * N * 2 * (N // 2 + 1), np.float32)
+    rc = lib().oracle_read_grid_gbptrees(fname.encode(), prop, N, ctypes.byref(params), _p(out))
+    if rc:
+        raise mxr.MxrError(rc, "oracle_read_grid_gbptrees")
     return out
 
 
