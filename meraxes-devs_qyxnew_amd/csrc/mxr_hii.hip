@@ -41,6 +41,9 @@ struct HiiEpi {
   float z;        // (float)redshift
   float TK;       // T_RECFAST(z)
   float cT_ad;    // cT_approx(z)
+  // the call's constant powers in the fully ionised temperature, from the host's libm as the
+  // reference computes them: ((1+z)/7.1)^2.5, T_RE^1.7, (1e4 (1+z)/4)^1.7
+  double fi_z7, fi_tre, fi_he;
   int first, last, uvb, spin, bh;
   uint8_t* ion;
   float* xH;
@@ -57,7 +60,8 @@ struct HiiEpi {
 };
 
 // find_HII_bubbles.c:48-74
-__device__ __noinline__ float fully_ionized_T(float z_re, float z, float delta)
+__device__ __noinline__ float fully_ionized_T(float z_re, float z, float delta, double fi_z7, double fi_tre,
+                                              double fi_he)
 {
 #pragma clang fp contract(off)
   float result, delta_re;
@@ -71,11 +75,10 @@ __device__ __noinline__ float fully_ionized_T(float z_re, float z, float delta)
       delta = (float)(-1. + (double)ABS_TOL);
     result = (float)pow((1. + (double)delta) / (1. + (double)delta_re), 1.1333);
     result = (float)((double)result * pow((1. + (double)z) / (1. + (double)z_re), 3.4));
-    result = (float)((double)result *
-                     (double)expf((float)(pow((1. + (double)z) / 7.1, 2.5) - pow((1. + (double)z_re) / 7.1, 2.5))));
+    result = (float)((double)result * (double)expf((float)(fi_z7 - pow((1. + (double)z_re) / 7.1, 2.5))));
   }
-  result = (float)((double)result * pow(T_RE, 1.7));
-  result = (float)((double)result + pow(1e4 * ((1. + (double)z) / 4.), 1.7) * (double)(1 + delta));
+  result = (float)((double)result * fi_tre);
+  result = (float)((double)result + fi_he * (double)(1 + delta));
   result = (float)pow((double)result, 0.5882);
   return result;
 }
@@ -616,7 +619,7 @@ __global__ void __launch_bounds__(NT) k_postloop(int N, HiiEpi ea)
     const float zin = ea.z_in[ir];
     float temp = ea.temp[ir];
     if (zin > 0 && xh < REL_TOL)
-      temp = fully_ionized_T(zin, ea.z, dx_in);
+      temp = fully_ionized_T(zin, ea.z, dx_in, ea.fi_z7, ea.fi_tre, ea.fi_he);
     const float floor_T = ea.spin ? ea.Tk_box[ir] : thermal_T(ea.TK, ea.cT_ad, dx_in);
     if (temp < floor_T)
       temp = floor_T;
@@ -1047,6 +1050,9 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
   ea.z = (float)redshift;
   ea.TK = (float)TKd;
   ea.cT_ad = cT_approx((float)redshift);
+  ea.fi_z7 = std::pow((1. + (double)ea.z) / 7.1, 2.5);
+  ea.fi_tre = std::pow(T_RE, 1.7);
+  ea.fi_he = std::pow(1e4 * ((1. + (double)ea.z) / 4.), 1.7);
   ea.uvb = p->ReionUVBFlag;
   ea.spin = spin;
   ea.bh = bh;

@@ -77,13 +77,25 @@ __device__ __forceinline__ auto pick(const P& ps, int g)
   return g == 0 ? ps.p[0] : g == 1 ? ps.p[1] : g == 2 ? ps.p[2] : g == 3 ? ps.p[3] : g == 4 ? ps.p[4] : ps.p[5];
 }
 // ---- the reference's window arithmetic (float, unfused) --------------------------------
+// correctly rounded float square root of x = 0 or a normal x > 0 (sqrtf's result): the hardware
+// root, within an ulp, then the residuals of its two neighbours decide, as in the usual correctly
+// rounded expansion (HIP's __fsqrt_rn is the hardware root itself)
+__device__ __forceinline__ float sqrt_rn(float x)
+{
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float dn = __int_as_float(__float_as_int(s) - 1), up = __int_as_float(__float_as_int(s) + 1);
+  float r = __builtin_fmaf(-dn, s, x) <= 0.f ? dn : s;
+  r = __builtin_fmaf(-up, s, x) > 0.f ? up : r;
+  return r;
+}
+
 __device__ __forceinline__ float kmag_ref(int nx, int ny, int nz, int N, float dk)
 {
   const int mid = N / 2;
   const float kx = __fmul_rn((float)(nx > mid ? nx - N : nx), dk);
   const float ky = __fmul_rn((float)(ny > mid ? ny - N : ny), dk);
   const float kz = __fmul_rn((float)nz, dk);
-  return __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz, kz)));
+  return sqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz, kz)));
 }
 
 // sharp-k (type 1): kR *= 0.413566994 (double product, float result); zero if kR > 1
@@ -92,6 +104,25 @@ __device__ __forceinline__ bool sharpk_keep(int nx, int ny, int nz, const FiltAr
   const float kR = __fmul_rn(kmag_ref(nx, ny, nz, fa.N, fa.dk), fa.R);
   const float s = (float)((double)kR * 0.413566994);
   return !(s > 1.0f);
+}
+
+// Along one axis of a column the sharp-k test depends on |n| only (the index n and N - n give
+// the same |k| bit for bit) and every step of it is monotone in |n|, so it keeps exactly the
+// |n| <= extent: the largest such |n| in 0..N/2 by bisection, or -1 when |n| = 0 is dropped.
+template <class Keep>
+__device__ __forceinline__ int kept_extent(Keep keep, int N)
+{
+  if (!keep(0))
+    return -1;
+  int lo = 0, hi = N / 2;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (keep(mid))
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
 }
 
 // multiplicative window of types 0 (real-space top-hat) and 2 (Gaussian); one out-of-line copy
@@ -163,6 +194,16 @@ __global__ void __launch_bounds__(NT) k_col(CGridPtrs in, GridPtrs out, ColGeom 
   }
 
   const float2* __restrict__ src = pick(in, g);
+  // the sharp-k window's kept extent of |n| along the column, once per column; elements past it
+  // load the column's first element (a cache hit) and are zeroed, so every load issues together
+  int ext = L;
+  if constexpr (MODE == COL_FILTER) {
+    if (colnz && fa.apply && fa.ftype == 1)
+      ext = kept_extent([&](int m) { return sharpk_keep(m, fj, fk, fa); }, L);
+  } else if constexpr (MODE == COL_YPRUNE) {
+    if (colnz && fa.apply && fa.prune)
+      ext = kept_extent([&](int m) { return sharpk_keep(0, m, fk, fa); }, L);
+  }
   float2 v[SL::NREG];
 #pragma unroll
   for (int q = 0; q < SL::Q0; q++) {
@@ -170,28 +211,17 @@ __global__ void __launch_bounds__(NT) k_col(CGridPtrs in, GridPtrs out, ColGeom 
 #pragma unroll
     for (int r = 0; r < SL::R0; r++) {
       const int e = j + r * SL::B0;
-      float2 x = make_float2(0.f, 0.f);
-      if ((SL::B0 % T == 0 || j < SL::B0) && colnz) {
-        if constexpr (MODE == COL_FILTER) {
-          if (!fa.apply) {
-            x = src[base + (long)e * geo.es];
-          } else if (fa.ftype == 1) {
-            if (sharpk_keep(e, fj, fk, fa))
-              x = src[base + (long)e * geo.es];
-          } else {
-            const float w = window_mul(e, fj, fk, fa);
-            x = src[base + (long)e * geo.es];
-            // rounded on its own (no fusion into the first butterfly), as filter() does
-            x = make_float2(__fmul_rn(x.x, w), __fmul_rn(x.y, w));
-          }
-        } else if constexpr (MODE == COL_YPRUNE) {
-          if (!fa.apply || !fa.prune || sharpk_keep(0, e, fk, fa))
-            x = src[base + (long)e * geo.es];
-        } else {
-          x = src[base + (long)e * geo.es];
+      const bool in_col = (SL::B0 % T == 0 || j < SL::B0) && colnz;
+      const bool keep = in_col && (e < L - e ? e : L - e) <= ext;
+      float2 x = src[keep ? base + (long)e * geo.es : base];
+      if constexpr (MODE == COL_FILTER) {
+        if (fa.apply && fa.ftype != 1 && in_col) {
+          const float w = window_mul(e, fj, fk, fa);
+          // rounded on its own (no fusion into the first butterfly), as filter() does
+          x = make_float2(__fmul_rn(x.x, w), __fmul_rn(x.y, w));
         }
       }
-      v[q * SL::R0 + r] = x;
+      v[q * SL::R0 + r] = keep ? x : make_float2(0.f, 0.f);
     }
   }
   tile_fft<L, T, S, L, LdsCol<TC>>(v, lds, tw, c, t);

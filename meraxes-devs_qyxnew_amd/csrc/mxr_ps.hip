@@ -50,18 +50,6 @@ struct PsArgs {
   double* partials;          // [block][bin][NQ]
 };
 
-// correctly rounded float square root of a normal x > 0: the hardware root (within an ulp),
-// then the residuals of its two neighbours decide, as in the usual correctly rounded expansion
-// (no denormal scaling: |k|^2 >= delta_k^2 here)
-__device__ __forceinline__ float sqrt_rn(float x)
-{
-  const float s = __builtin_amdgcn_sqrtf(x);
-  const float dn = __int_as_float(__float_as_int(s) - 1), up = __int_as_float(__float_as_int(s) + 1);
-  float r = __builtin_fmaf(-dn, s, x) <= 0.f ? dn : s;
-  r = __builtin_fmaf(-up, s, x) > 0.f ? up : r;
-  return r;
-}
-
 __device__ __forceinline__ double wave_sum(double x)
 {
 #pragma unroll
@@ -177,7 +165,7 @@ __global__ void __launch_bounds__(PS_NT) k_ps_bin(PsArgs a)
         // float (24 bits) equals rounding it to float once, as 53 >= 2 x 24 + 2
         const float kz = (float)nz * a.delta_k;
         const float k2 = __fadd_rn(kxy2, __fmul_rn(kz, kz));
-        const float k_mag = k2 > 0.f ? sqrt_rn(k2) : 0.f;
+        const float k_mag = sqrt_rn(k2);
         // bin i: edge[i] <= k < edge[i + 1].  The guess 1 + floor(log_1.35(k / delta_k)) is within
         // one bin of i (the float ceilings drift from delta_k 1.35^j by under 3e-6 relative), so
         // one look at edge[g] and edge[g + 1] decides.
