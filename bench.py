@@ -486,6 +486,8 @@ def main():
     ap.add_argument("--ps-cpu-N", type=int, default=512, help="grid of the power-spectrum CPU sample (0: skip)")
     ap.add_argument("--batch", type=int, default=64, help="realisations in the batch leg (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--only", choices=("ps", "baryon", "grid"), default=None,
+                    help="run just this leg and print its JSON (for profiling one component)")
     ap.add_argument("--autotune", choices=("auto", "always", "never"), default="auto",
                     help="launch shapes: wisdom entry if present else autotune (auto), always tune, or defaults")
     ap.add_argument("--wisdom", default=os.path.join(tempfile.gettempdir(), "mxr-wisdom.json"))
@@ -500,6 +502,11 @@ def main():
 
     import mxr
     import synth
+
+    if args.only:
+        leg = {"ps": ps_leg, "baryon": baryon_leg, "grid": grid_leg}[args.only]
+        print(json.dumps(leg(args, torch)), flush=True)
+        return
 
     N, box = args.N, args.box
     params = mxr.hii_params(N, box, filter_type=1, zeta=1.0)

@@ -143,9 +143,9 @@ static __device__ __noinline__ float window_mul(int nx, int ny, int nz, const Fi
 }
 
 // ---- column pass (x: columns = (j,k) flattened, elements i; y: columns = (i,k), elements j)
-template <int L, int NT, int TC, int S, int MODE>
+template <int L, int NT, int TC, int S, int MODE, int TPB>
 __global__ void __launch_bounds__(NT) k_col(CGridPtrs in, GridPtrs out, ColGeom geo, const float2* __restrict__ twg,
-                                            FiltArgs fa, double div)
+                                            FiltArgs fa, double div, long ntile)
 {
   constexpr int T = NT / TC;
   using SL = Slots<L, T>;
@@ -154,93 +154,112 @@ __global__ void __launch_bounds__(NT) k_col(CGridPtrs in, GridPtrs out, ColGeom 
   const int g = blockIdx.y;
   const int c = threadIdx.x % TC;
   const int t = threadIdx.x / TC;
-  long col = (long)blockIdx.x * TC + c;
-  bool valid = col < geo.ncol;
-  if constexpr (MODE == COL_FILTER || MODE == COL_YPRUNE) {
-    if (fa.live > 0) {
-      // live-tile grid: blocks enumerate (outer index, k_z tile) pairs whose k_z tile starts
-      // below the kept count; for the x pass the outer index is j with |n_y| < live
-      const int H = fa.N / 2 + 1;
-      const int nkt = (fa.live + TC - 1) / TC;
-      const int jj = (int)(blockIdx.x / nkt), kt = (int)(blockIdx.x % nkt);
-      int outer = jj;
-      if constexpr (MODE == COL_FILTER)
-        outer = jj < fa.live ? jj : fa.N - (2 * fa.live - 1 - jj);
-      const int k = kt * TC + c;
-      col = (long)outer * H + k;
-      valid = k < H;
-    }
-  }
-  const long base = valid ? (col / geo.cpp) * geo.ps + (col % geo.cpp) : 0;
   for (int m = threadIdx.x; m < L; m += NT)
     tw[m] = twg[m];
-
-  bool colnz = valid;
-  int fj = 0, fk = 0;
-  if constexpr (MODE == COL_FILTER) {
-    const int H = fa.N / 2 + 1;
-    fj = (int)(col / H);
-    fk = (int)(col % H);
-    if (valid && fa.apply && fa.prune && fa.ftype == 1)
-      colnz = sharpk_keep(0, fj, fk, fa);
-  } else if constexpr (MODE == COL_YPRUNE) {
-    fk = (int)(col % geo.cpp);
-    if (valid && fa.apply && fa.prune)
-      colnz = sharpk_keep(0, 0, fk, fa);
-  }
-  if constexpr (MODE == COL_FILTER || MODE == COL_YPRUNE) {
-    if (!__syncthreads_or(colnz))
-      return;  // every column of the tile is zero for this radius
-  }
-
   const float2* __restrict__ src = pick(in, g);
-  // the sharp-k window's kept extent of |n| along the column, once per column; elements past it
-  // load the column's first element (a cache hit) and are zeroed, so every load issues together
-  int ext = L;
-  if constexpr (MODE == COL_FILTER) {
-    if (colnz && fa.apply && fa.ftype == 1)
-      ext = kept_extent([&](int m) { return sharpk_keep(m, fj, fk, fa); }, L);
-  } else if constexpr (MODE == COL_YPRUNE) {
-    if (colnz && fa.apply && fa.prune)
-      ext = kept_extent([&](int m) { return sharpk_keep(0, m, fk, fa); }, L);
-  }
-  float2 v[SL::NREG];
+
+  // TPB adjacent tiles per block: every tile's loads are issued first, so the later tiles'
+  // memory latency hides behind the earlier tiles' transforms (the block barriers inside the
+  // transform wait for LDS only)
+  long base[TPB];
+  bool colnz[TPB];
+  int fj[TPB], fk[TPB];
+  float2 v[TPB][SL::NREG];
 #pragma unroll
-  for (int q = 0; q < SL::Q0; q++) {
-    const int j = t + T * q;
-#pragma unroll
-    for (int r = 0; r < SL::R0; r++) {
-      const int e = j + r * SL::B0;
-      const bool in_col = (SL::B0 % T == 0 || j < SL::B0) && colnz;
-      const bool keep = in_col && (e < L - e ? e : L - e) <= ext;
-      float2 x = src[keep ? base + (long)e * geo.es : base];
-      if constexpr (MODE == COL_FILTER) {
-        if (fa.apply && fa.ftype != 1 && in_col) {
-          const float w = window_mul(e, fj, fk, fa);
-          // rounded on its own (no fusion into the first butterfly), as filter() does
-          x = make_float2(__fmul_rn(x.x, w), __fmul_rn(x.y, w));
-        }
+  for (int i = 0; i < TPB; i++) {
+    const long tile = (long)blockIdx.x * TPB + i;
+    long col = tile * TC + c;
+    bool valid = tile < ntile && col < geo.ncol;
+    if constexpr (MODE == COL_FILTER || MODE == COL_YPRUNE) {
+      if (fa.live > 0) {
+        // live-tile grid: tiles enumerate (outer index, k_z tile) pairs whose k_z tile starts
+        // below the kept count; for the x pass the outer index is j with |n_y| < live
+        const int H = fa.N / 2 + 1;
+        const int nkt = (fa.live + TC - 1) / TC;
+        const int jj = (int)(tile / nkt), kt = (int)(tile % nkt);
+        int outer = jj;
+        if constexpr (MODE == COL_FILTER)
+          outer = jj < fa.live ? jj : fa.N - (2 * fa.live - 1 - jj);
+        const int k = kt * TC + c;
+        col = (long)outer * H + k;
+        valid = tile < ntile && k < H;
       }
-      v[q * SL::R0 + r] = keep ? x : make_float2(0.f, 0.f);
+    }
+    base[i] = valid ? (col / geo.cpp) * geo.ps + (col % geo.cpp) : 0;
+    colnz[i] = valid;
+    fj[i] = 0;
+    fk[i] = 0;
+    if constexpr (MODE == COL_FILTER) {
+      const int H = fa.N / 2 + 1;
+      fj[i] = (int)(col / H);
+      fk[i] = (int)(col % H);
+      if (valid && fa.apply && fa.prune && fa.ftype == 1)
+        colnz[i] = sharpk_keep(0, fj[i], fk[i], fa);
+    } else if constexpr (MODE == COL_YPRUNE) {
+      fk[i] = (int)(col % geo.cpp);
+      if (valid && fa.apply && fa.prune)
+        colnz[i] = sharpk_keep(0, 0, fk[i], fa);
+    }
+    // the sharp-k window's kept extent of |n| along the column, once per column; elements past
+    // it load the column's first element (a cache hit) and are zeroed, so every load issues
+    // together
+    int ext = L;
+    if constexpr (MODE == COL_FILTER) {
+      if (colnz[i] && fa.apply && fa.ftype == 1) {
+        const int j0 = fj[i], k0 = fk[i];
+        ext = kept_extent([&](int m) { return sharpk_keep(m, j0, k0, fa); }, L);
+      }
+    } else if constexpr (MODE == COL_YPRUNE) {
+      if (colnz[i] && fa.apply && fa.prune) {
+        const int k0 = fk[i];
+        ext = kept_extent([&](int m) { return sharpk_keep(0, m, k0, fa); }, L);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SL::Q0; q++) {
+      const int j = t + T * q;
+#pragma unroll
+      for (int r = 0; r < SL::R0; r++) {
+        const int e = j + r * SL::B0;
+        const bool in_col = (SL::B0 % T == 0 || j < SL::B0) && colnz[i];
+        const bool keep = in_col && (e < L - e ? e : L - e) <= ext;
+        float2 x = src[keep ? base[i] + (long)e * geo.es : base[i]];
+        if constexpr (MODE == COL_FILTER) {
+          if (fa.apply && fa.ftype != 1 && in_col) {
+            const float w = window_mul(e, fj[i], fk[i], fa);
+            // rounded on its own (no fusion into the first butterfly), as filter() does
+            x = make_float2(__fmul_rn(x.x, w), __fmul_rn(x.y, w));
+          }
+        }
+        v[i][q * SL::R0 + r] = keep ? x : make_float2(0.f, 0.f);
+      }
     }
   }
-  tile_fft<L, T, S, L, LdsCol<TC>>(v, lds, tw, c, t);
-  if (!colnz)
-    return;
+
   float2* __restrict__ dst = pick(out, g);
 #pragma unroll
-  for (int q = 0; q < SL::QL; q++) {
-    const int j = t + T * q;
-    if (SL::BL % T == 0 || j < SL::BL) {
+  for (int i = 0; i < TPB; i++) {
+    // block-uniform: a tile whose columns are all zero for this radius is skipped; the barrier
+    // also orders the LDS between consecutive tiles' transforms and publishes the twiddles
+    if (!__syncthreads_or(colnz[i]))
+      continue;
+    tile_fft<L, T, S, L, LdsCol<TC>>(v[i], lds, tw, c, t);
+    if (!colnz[i])
+      continue;
 #pragma unroll
-      for (int r = 0; r < SL::RL; r++) {
-        const int e = j + r * SL::BL;
-        float2 y = v[q * SL::RL + r];
-        if constexpr (MODE == COL_SCALE)
-          y = make_float2((float)((double)y.x / div), (float)((double)y.y / div));
-        if constexpr (MODE == COL_FSCALE)
-          y = make_float2(__fdiv_rn(y.x, (float)div), __fdiv_rn(y.y, (float)div));
-        dst[base + (long)e * geo.es] = y;
+    for (int q = 0; q < SL::QL; q++) {
+      const int j = t + T * q;
+      if (SL::BL % T == 0 || j < SL::BL) {
+#pragma unroll
+        for (int r = 0; r < SL::RL; r++) {
+          const int e = j + r * SL::BL;
+          float2 y = v[i][q * SL::RL + r];
+          if constexpr (MODE == COL_SCALE)
+            y = make_float2((float)((double)y.x / div), (float)((double)y.y / div));
+          if constexpr (MODE == COL_FSCALE)
+            y = make_float2(__fdiv_rn(y.x, (float)div), __fdiv_rn(y.y, (float)div));
+          dst[base[i] + (long)e * geo.es] = y;
+        }
       }
     }
   }
@@ -449,6 +468,7 @@ struct Cfg {
   static constexpr int TC = 16;
   static constexpr int COL_T = (N / 8 >= 16) ? 16 : (N / 8 < 1 ? 1 : N / 8);
   static constexpr int COL_NT = TC * COL_T;
+  static constexpr int COL_TPB = 1;  // column tiles per block (2 overlaps loads with transforms but needs ~280 VGPRs)
   static constexpr int ROW_T = pow2_floor((M / 8 >= 16) ? 16 : (M / 8 < 1 ? 1 : M / 8));
   static constexpr int ROW_NT = 256;
   static constexpr int TR = ROW_NT / ROW_T;
@@ -543,13 +563,15 @@ struct Passes {
   static int col_tc(Ctx* c, CGridPtrs in, GridPtrs out, int ngrid, ColGeom geo, FiltArgs fa, double div)
   {
     constexpr int NT = TC * C::COL_T;
-    long nblk = (geo.ncol + TC - 1) / TC;
+    constexpr int TPB = C::COL_TPB;
+    long ntile = (geo.ncol + TC - 1) / TC;
     if ((MODE == COL_FILTER || MODE == COL_YPRUNE) && fa.live > 0) {
       const long nkt = (fa.live + TC - 1) / TC;
-      nblk = (MODE == COL_FILTER ? 2L * fa.live - 1 : (long)N) * nkt;
+      ntile = (MODE == COL_FILTER ? 2L * fa.live - 1 : (long)N) * nkt;
     }
-    dim3 grid((unsigned)nblk, (unsigned)ngrid);
-    hipLaunchKernelGGL((k_col<N, NT, TC, S, MODE>), grid, dim3(NT), 0, c->stream, in, out, geo, c->tw, fa, div);
+    dim3 grid((unsigned)((ntile + TPB - 1) / TPB), (unsigned)ngrid);
+    hipLaunchKernelGGL((k_col<N, NT, TC, S, MODE, TPB>), grid, dim3(NT), 0, c->stream, in, out, geo, c->tw, fa, div,
+                       ntile);
     MXR_HIP(hipGetLastError());
     return MXR_OK;
   }
