@@ -329,6 +329,7 @@ void mxr_destroy(mxr_ctx* h)
   (void)hipFree(c.partials);
   (void)hipFree(c.ps_part);
   (void)hipFree(c.bg_ws);
+  (void)hipFree(c.wtab);
   if (c.hires)
     mxr_destroy(c.hires);
   (void)hipFree(c.sums);

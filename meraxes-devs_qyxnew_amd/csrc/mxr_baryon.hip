@@ -234,17 +234,42 @@ __global__ void __launch_bounds__(256) k_bg_deposit(BgArgs a, const unsigned* __
     if (key >= a.sentinel || (i > 0 && keys[i - 1] == key))
       continue;
     float st = 0.f, ws = 0.f, bm = 0.f, br = 0.f, xr = 0.f, sf = 0.f, hh = 0.f, hs = 0.f;
-    for (long j = i; j < a.n && keys[j] == key; j++) {
-      const GalRec r = rec[vals[j]];
-      st = (float)((double)st + r.st);
-      ws = (float)((double)ws + r.ws);
-      bm = (float)((double)bm + r.bm);
-      br = (float)((double)br + r.br);
-      below += (unsigned)r.below;
-      xr = xr + r.xr;
-      sf = (float)((double)sf + r.sf);
-      hh = (float)((double)hh + r.hh);
-      hs = (float)((double)hs + r.hs);
+    // the run in chunks of U: each chunk's keys and indices, then its records, are loaded
+    // together (one round trip each); entries past the run load the chunk's first record again
+    // (a cache hit) and add 0.0, which leaves every sum unchanged (no sum is -0.0)
+    constexpr int U = 8;
+    bool more = true;
+    for (long j = i; more; j += U) {
+      unsigned kk[U], vv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const long jj = j + u < a.n ? j + u : a.n - 1;
+        kk[u] = keys[jj];
+        vv[u] = vals[jj];
+      }
+      bool in_run[U];
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        in_run[u] = j + u < a.n && kk[u] == key;
+      GalRec rr[U];
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        rr[u] = rec[in_run[u] ? vv[u] : vv[0]];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const bool m = in_run[u];
+        const GalRec& r = rr[u];
+        st = (float)((double)st + (m ? r.st : 0.0));
+        ws = (float)((double)ws + (m ? r.ws : 0.0));
+        bm = (float)((double)bm + (m ? r.bm : 0.0));
+        br = (float)((double)br + (m ? r.br : 0.0));
+        below += m ? (unsigned)r.below : 0u;
+        xr = xr + (m ? r.xr : 0.0f);
+        sf = (float)((double)sf + (m ? r.sf : 0.0));
+        hh = (float)((double)hh + (m ? r.hh : 0.0));
+        hs = (float)((double)hs + (m ? r.hs : 0.0));
+      }
+      more = in_run[U - 1];
     }
     deposit_cell(a, key, st, ws, bm, br, xr, sf, hh, hs);
   }

@@ -47,27 +47,18 @@ __global__ void k_grid_finish(Finish f)
   }
 }
 
-// the file's real-layout grid into the padded layout r2c reads
-__global__ void k_grid_pad(const float* __restrict__ in, float* __restrict__ out, int n)
-{
-  const int H2 = 2 * (n / 2 + 1);
-  const long nreal = (long)n * n * n;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nreal; i += (long)gridDim.x * blockDim.x)
-    out[(i / n) * H2 + i % n] = in[i];
-}
-
 // smooth_grid (read_grids.c:128-165): r2c, /n^3, filter(R, 0), c2r; the result is the padded
-// real grid in hc->spec[0]
+// real grid in hc->spec[0].  r2c reads the file's real-layout rows directly; the filter is the
+// window table's multiply on the inverse x pass's loads (the same product filter() forms, each
+// mode rounded once), the table built once per (n, R) and kept.
 template <int N>
 int smooth(Ctx* hc, const float* hi_real, double BoxSize, int N_lo)
 {
   using K = Passes<N>;
   RC(ensure_buffers(hc, 1));
-  float* padded = reinterpret_cast<float*>(hc->work[0]);
-  hipLaunchKernelGGL(k_grid_pad, dim3(ew_blocks(n_real(N))), dim3(256), 0, hc->stream, hi_real, padded, N);
-  MXR_HIP(hipGetLastError());
   CRealPtrs rin{};
-  rin.p[0] = padded;
+  rin.p[0] = hi_real;
+  rin.real = 1u;
   GridPtrs spec{};
   spec.p[0] = hc->spec[0];
   CGridPtrs cspec{};
@@ -79,10 +70,14 @@ int smooth(Ctx* hc, const float* hi_real, double BoxSize, int N_lo)
   RC((K::template col<-1, COL_SCALE>(hc, cspec, spec, 1, K::xgeo(), none, std::pow((double)N, 3))));
   // filter(slab, ..., n_cell[0], (float)(BoxSize / ReionGridDim / 2.0), 0): delta_k from the
   // float box size (reionization.c:3376-3377)
-  const float box_size = (float)BoxSize;
-  const float dk = (float)(2.0 * M_PI / box_size);
-  RC(run_filter(hc, hc->spec[0], 0, N, N, dk, (float)(BoxSize / (double)N_lo / 2.0), 0));
-  RC((K::template col<+1, COL_PLAIN>(hc, cspec, spec, 1, K::xgeo(), none, 1.0)));
+  FiltArgs fa{};
+  fa.N = N;
+  fa.dk = (float)(2.0 * M_PI / (float)BoxSize);
+  fa.R = (float)(BoxSize / (double)N_lo / 2.0);
+  fa.ftype = 0;
+  fa.apply = 1;
+  RC(window_table(hc, fa.dk, fa.R, fa.ftype, &fa.wtab));
+  RC((K::template col<+1, COL_FILTER>(hc, cspec, spec, 1, K::xgeo(), fa, 1.0)));
   RC((K::template col<+1, COL_PLAIN>(hc, cspec, spec, 1, K::ygeo(), none, 1.0)));
   return run_c2r_store(hc, hc->spec[0]);
 }
@@ -109,12 +104,13 @@ int run_prepare_grid(Ctx* c, const float* hi, int n_hi, int mem_in, int property
     f.mean_inv = (float)(std::pow(file_box, 3) * p->Hubble_h / ((double)p->NPart * p->PartMass));
     f.floor_v = (float)(-1.0 + (double)(float)1e-5);
   }
-  // the hi-res grid on the device
+  // the hi-res grid on the device, 8-byte aligned (r2c reads it in pairs)
   const float* hd = hi;
   float* staged = nullptr;
-  if (mem_in == MXR_MEM_HOST) {
+  if (mem_in == MXR_MEM_HOST || (reinterpret_cast<uintptr_t>(hi) & 7u) != 0) {
     MXR_HIP(hipMalloc(&staged, sizeof(float) * nreal_hi));
-    hipError_t e = hipMemcpyAsync(staged, hi, sizeof(float) * nreal_hi, hipMemcpyHostToDevice, c->stream);
+    hipError_t e = hipMemcpyAsync(staged, hi, sizeof(float) * nreal_hi,
+                                  mem_in == MXR_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, c->stream);
     if (e == hipSuccess)
       e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {

@@ -1281,6 +1281,41 @@ int run_find_hii_bubbles(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g
   }
 }
 
+// the window of types 0 and 2 by (|n_x|, |n_y|, n_z): the same window_mul evaluation a mode
+// with those magnitudes gets
+__global__ void k_window_table(float* __restrict__ tab, FiltArgs fa)
+{
+  const int H = fa.N / 2 + 1;
+  const long n = (long)H * H * H;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % H), b = (int)((idx / H) % H), a = (int)(idx / ((long)H * H));
+    tab[idx] = window_mul(a, b, c, fa);
+  }
+}
+
+int window_table(Ctx* c, float dk, float R, int ftype, const float** tab)
+{
+  if (!(c->wtab && c->wtab_ftype == ftype && c->wtab_dk == dk && c->wtab_R == R)) {
+    const long H = c->N / 2 + 1, n = H * H * H;
+    c->wtab_ftype = -1;
+    if (!c->wtab)
+      MXR_HIP(hipMalloc(&c->wtab, sizeof(float) * n));
+    FiltArgs fa{};
+    fa.N = c->N;
+    fa.dk = dk;
+    fa.R = R;
+    fa.ftype = ftype;
+    fa.apply = 1;
+    hipLaunchKernelGGL(k_window_table, dim3(ew_blocks(n)), dim3(256), 0, c->stream, c->wtab, fa);
+    MXR_HIP(hipGetLastError());
+    c->wtab_ftype = ftype;
+    c->wtab_dk = dk;
+    c->wtab_R = R;
+  }
+  *tab = c->wtab;
+  return MXR_OK;
+}
+
 int run_filter(Ctx* c, float2* box, int ix0, int nx, int N, float dk, float R, int ftype)
 {
   FiltArgs fa{};

@@ -134,6 +134,10 @@ struct Ctx {
   double ps_geom[2 * MXR_PS_MAXB] = {};
   // a context of an input grid's dimension, for smoothing it (created on first use)
   mxr_ctx* hires = nullptr;
+  // the filter window of types 0 and 2 by (|n_x|, |n_y|, n_z), for the last (dk, R, type) seen
+  float* wtab = nullptr;
+  float wtab_dk = 0, wtab_R = 0;
+  int wtab_ftype = -1;
   // galaxies to grids: sort keys and values, the sort's storage, staged galaxy arrays
   void* bg_ws = nullptr;
   size_t bg_ws_cap = 0;            // bytes
@@ -156,6 +160,9 @@ int check_hii_params(const mxr_hii_params* p, int N);
 bool supported_dim(int N);
 int run_find_hii_bubbles(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_hii_globals* out);
 int run_filter(Ctx* c, float2* box, int local_ix_start, int slab_nx, int grid_dim, float dk, float R, int ftype);
+// device table of the type 0 / 2 window at (dk, R) over (|n_x|, |n_y|, n_z) in 0..N/2, kept in
+// the context and rebuilt (on the context's stream) when (dk, R, type) changes
+int window_table(Ctx* c, float dk, float R, int ftype, const float** tab);
 int run_velocity_gradient(Ctx* c, float2* box, int slab_nx, int grid_dim, float dk);
 int run_brightness_temperature(Ctx* c, const mxr_tb_params* p, const mxr_tb_grids* g, double* ave);
 int launch_final_sum(Ctx* c, const double* partials, int nblk, int nv, double* out);  // fixed-order FP64 sum

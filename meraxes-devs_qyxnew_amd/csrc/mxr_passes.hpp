@@ -22,6 +22,7 @@ struct CGridPtrs {
 struct CRealPtrs {
   const float* p[MAX_GRIDS];
   unsigned bf16;  // bit g: p[g] holds bfloat16 bit patterns (uint16), widened on load
+  unsigned real;  // bit g: p[g] is a real-layout grid (rows of N floats, no padding)
   // deldel loads (power spectrum, one grid): when dd_sum is set, p[0] is a real-layout grid
   // v and each value loads as dd = (float)((v / ave - 1) volume / total), then
   // (float)(dd ave), with ave = *dd_sum / total (ComputePowerSpectrum.c:97-104)
@@ -64,6 +65,10 @@ struct FiltArgs {
   int apply;
   int prune;  // skip the modes the sharp-k window zeroes
   int live;   // > 0: the grid covers only tiles that can hold kept columns, `live` = kept k_z count
+  // types 0 and 2: when set, the window's values by (|n_x|, |n_y|, n_z), each in 0..N/2 (index
+  // (a (N/2+1) + b) (N/2+1) + c), evaluated by window_mul beforehand; |k| depends on the three
+  // magnitudes only, so a table entry is the value window_mul gives at any of its modes
+  const float* wtab;
 };
 
 // COL_SCALE divides by `div` in double (find_HII_bubbles.c:235-253), COL_FSCALE by (float)div
@@ -225,13 +230,43 @@ __global__ void __launch_bounds__(NT) k_col(CGridPtrs in, GridPtrs out, ColGeom 
         const bool keep = in_col && (e < L - e ? e : L - e) <= ext;
         float2 x = src[keep ? base[i] + (long)e * geo.es : base[i]];
         if constexpr (MODE == COL_FILTER) {
-          if (fa.apply && fa.ftype != 1 && in_col) {
+          if (fa.apply && fa.ftype != 1 && !fa.wtab && in_col) {
             const float w = window_mul(e, fj[i], fk[i], fa);
             // rounded on its own (no fusion into the first butterfly), as filter() does
             x = make_float2(__fmul_rn(x.x, w), __fmul_rn(x.y, w));
           }
         }
         v[i][q * SL::R0 + r] = keep ? x : make_float2(0.f, 0.f);
+      }
+    }
+    if constexpr (MODE == COL_FILTER) {
+      // tabulated window (types 0 and 2): every entry's load issued, then the products
+      if (fa.apply && fa.ftype != 1 && fa.wtab) {
+        const int Hn = fa.N / 2 + 1;
+        const long wrow = (long)min(fj[i], fa.N - fj[i]) * Hn + fk[i];
+        float w[SL::NREG];
+#pragma unroll
+        for (int q = 0; q < SL::Q0; q++) {
+          const int j = t + T * q;
+#pragma unroll
+          for (int r = 0; r < SL::R0; r++) {
+            const int e = j + r * SL::B0;
+            const bool in_col = (SL::B0 % T == 0 || j < SL::B0) && colnz[i];
+            w[q * SL::R0 + r] = fa.wtab[in_col ? (long)min(e, fa.N - e) * Hn * Hn + wrow : 0];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < SL::Q0; q++) {
+          const int j = t + T * q;
+#pragma unroll
+          for (int r = 0; r < SL::R0; r++) {
+            const bool in_col = (SL::B0 % T == 0 || j < SL::B0) && colnz[i];
+            float2& x = v[i][q * SL::R0 + r];
+            const float wv = w[q * SL::R0 + r];
+            if (in_col)
+              x = make_float2(__fmul_rn(x.x, wv), __fmul_rn(x.y, wv));
+          }
+        }
       }
     }
   }
@@ -312,7 +347,7 @@ __global__ void __launch_bounds__(NT) k_row_r2c(CRealPtrs in, GridPtrs out, long
     for (int i = 0; i < SL::Q0 * SL::R0; i++)
       v[i] = make_float2(bf16_to_f32((unsigned short)(u[i] & 0xffffu)), bf16_to_f32((unsigned short)(u[i] >> 16)));
   } else {
-    const float* __restrict__ src = pick(in, g) + rb * 2 * H;
+    const float* __restrict__ src = pick(in, g) + rb * (((in.real >> g) & 1u) ? N : 2 * H);
 #pragma unroll
     for (int q = 0; q < SL::Q0; q++)
 #pragma unroll

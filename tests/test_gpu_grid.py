@@ -65,3 +65,33 @@ def test_errors(tmp_path):
     assert e.value.code == mxr.MXR_ERR_UNSUPPORTED
     with pytest.raises(mxr.MxrError):
         ctx.read_grid_gbptrees(str(tmp_path / "missing.dat"), 0, p, out)
+
+
+def test_smoothing_window_cache_and_unaligned_input():
+    """The smoothing filter's window values are cached per (grid, R): a call at another box size
+    rebuilds them, and returning to the first box gives the first result bit for bit, as does a
+    fresh context.  A device input at an odd float offset is staged and gives the same result."""
+    import torch
+    N, n_hi = 32, 64
+    hi = density_grids(n_hi, 5)[0]
+    outs = []
+    ctx = mxr.Context(N)
+    for box in (50.0, 80.0, 50.0):
+        p = mxr.grid_params(box, npart=n_hi ** 3, part_mass=1.0)
+        out = np.zeros(N * N * 2 * (N // 2 + 1), np.float32)
+        ctx.prepare_grid(hi, n_hi, mxr.MXR_GRID_DENSITY, box, p, out)
+        outs.append(out)
+    assert np.array_equal(outs[0], outs[2])
+    assert not np.array_equal(outs[0], outs[1])
+    p = mxr.grid_params(50.0, npart=n_hi ** 3, part_mass=1.0)
+    fresh = np.zeros_like(outs[0])
+    mxr.Context(N).prepare_grid(hi, n_hi, mxr.MXR_GRID_DENSITY, 50.0, p, fresh)
+    assert np.array_equal(fresh, outs[0])
+    buf = torch.zeros(hi.size + 1, device="cuda")
+    buf[1:] = torch.from_numpy(hi.reshape(-1)).cuda()
+    dev = torch.zeros(outs[0].size, device="cuda")
+    ctx.prepare_grid(buf[1:], n_hi, mxr.MXR_GRID_DENSITY, 50.0, p, dev)
+    assert np.array_equal(dev.cpu().numpy(), outs[0])
+    ref = O.prepare_grid(N, hi, n_hi, mxr.MXR_GRID_DENSITY, 80.0, mxr.grid_params(80.0, npart=n_hi ** 3, part_mass=1.0))
+    err = np.abs(_real(outs[1], N).astype(np.float64) - _real(ref, N)).max() / np.abs(_real(ref, N)).max()
+    assert err <= 2e-6
