@@ -4,11 +4,14 @@
 // The reference accumulates each cell's galaxies into a float buffer in galaxy order.  Float
 // sums depend on that order, so instead of atomics: each live galaxy gets its NGP cell as a key
 // (pos_to_ngp, misc_tools.c:207-217), a stable radix sort groups the galaxies by cell keeping
-// their order, and one thread per occupied cell sums its run of galaxies in order with the
-// reference's arithmetic and writes the finalised values.  Cells without galaxies keep the zero
-// the preparation kernel wrote, which is what the reference's finalisation gives them.
+// their order, a run-length encoding lists the occupied cells, and one thread per occupied cell
+// sums its run of galaxies in order with the reference's arithmetic and finalises the cell's
+// values.  One wavefront per grid row then writes the row of every grid once, coalesced: the
+// occupied cells' values and zero elsewhere, which is what the reference's clear and
+// finalisation give a cell without galaxies.  The histories move up by device copies first.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cmath>
 
 #include "mxr_passes.hpp"
@@ -73,54 +76,6 @@ __global__ void k_bg_keys(BgArgs a, unsigned* __restrict__ keys, unsigned* __res
   vals[i] = (unsigned)i;
 }
 
-// reionization.c:2093-2141: clear the grids and shift each history by one slot; slot 0 keeps its
-// padding (the reference's finalisation writes its real cells only), and so do the BH grids
-__global__ void k_bg_prep(BgArgs a)
-{
-  const int N = a.N, H2 = 2 * (N / 2 + 1);
-  const long npad = (long)N * N * H2;
-  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < npad; p += (long)gridDim.x * blockDim.x) {
-    const bool real = (int)(p % H2) < N;
-    a.stars[p] = 0.f;
-    a.wsfr[p] = 0.f;
-    if (a.bh && real) {
-      a.g_bhm[p] = 0.f;
-      a.g_bhar[p] = 0.f;
-    }
-    if (a.spin) {
-      // slot s moves to s + 1, from the top down, 8 slots per step with every load of a step
-      // issued before its stores (the next step's slots lie below the ones just written)
-      auto shift = [&](float* __restrict__ h) {
-        for (int s = a.nstore - 2; s >= 0; s -= 8) {
-          float t[8];
-#pragma unroll
-          for (int k = 0; k < 8; k++)
-            if (k <= s)
-              t[k] = h[(long)(s - k) * npad + p];
-#pragma unroll
-          for (int k = 0; k < 8; k++)
-            if (k <= s)
-              h[(long)(s - k + 1) * npad + p] = t[k];
-        }
-        if (real)
-          h[p] = 0.f;
-      };
-      a.g_sfr[p] = 0.f;
-      shift(a.h_sfr);
-      a.g_x[p] = 0.f;
-      shift(a.h_x);
-      if (a.hard) {
-        a.g_h[p] = 0.f;
-        shift(a.h_h);
-      }
-      if (a.soft) {
-        a.g_s[p] = 0.f;
-        shift(a.h_s);
-      }
-    }
-  }
-}
-
 // calculate_galaxy_xray_luminosity (reionization.c:1982-2023) without scatter
 __device__ __forceinline__ double xray_luminosity(const BgArgs& a, unsigned g)
 {
@@ -175,53 +130,18 @@ __global__ void k_bg_pack(BgArgs a, GalRec* __restrict__ rec)
   rec[i] = r;
 }
 
-// the finalisation of one cell's sums into the padded grids (:2424-2569)
-__device__ __forceinline__ void deposit_cell(const BgArgs& a, unsigned key, float st, float ws, float bm, float br,
-                                             float xr, float sf, float hh, float hs)
-{
-#pragma clang fp contract(off)
-  const int N = a.N;
-  const long ip = (long)(key / N) * (2 * (N / 2 + 1)) + key % N;
-  a.stars[ip] = st < 0 ? 0.f : st;
-  {
-    double v = (double)ws;
-    if (v < 0)
-      v = 0.0;
-    a.wsfr[ip] = (float)v;
-  }
-  if (a.bh) {
-    a.g_bhm[ip] = bm < 0 ? 0.f : bm;
-    a.g_bhar[ip] = br < 0 ? 0.f : br;
-  }
-  if (a.spin) {
-    a.g_x[ip] = xr;
-    a.h_x[ip] = xr;
-    double v = (double)sf;
-    if (v < 0)
-      v = 0.0;
-    if (!a.inst)
-      v /= a.sfr_timescale;
-    a.g_sfr[ip] = (float)v;
-    a.h_sfr[ip] = (float)v;
-    if (a.hard) {
-      const float h = hh < 0 ? 0.f : hh;
-      a.g_h[ip] = h;
-      a.h_h[ip] = h;
-    }
-    if (a.soft) {
-      const float s = hs < 0 ? 0.f : hs;
-      a.g_s[ip] = s;
-      a.h_s[ip] = s;
-    }
-  }
-}
+// one occupied cell's finalised values (:2424-2569): the clamps and the sfr conversion
+struct __align__(32) CellVal {
+  float st, ws, bm, br, xr, sf, hh, hs;
+};
 
-// one thread per run of equal keys: the cell's galaxies in their given order (:2219-2411),
-// then the finalisation of every property (:2424-2569); the grid-stride loop keeps the block
-// count small so that the below-limit count needs one global atomic per block
-__global__ void __launch_bounds__(256) k_bg_deposit(BgArgs a, const unsigned* __restrict__ keys,
-                                                    const unsigned* __restrict__ vals,
-                                                    const GalRec* __restrict__ rec)
+// one thread per occupied cell (run of equal keys): the cell's galaxies in their given order
+// (:2219-2411), then the finalisation of every property
+__global__ void __launch_bounds__(256) k_bg_runs(BgArgs a, const unsigned* __restrict__ run_keys,
+                                                 const unsigned* __restrict__ run_len,
+                                                 const unsigned* __restrict__ run_off, const unsigned* __restrict__ nrun,
+                                                 const unsigned* __restrict__ vals, const GalRec* __restrict__ rec,
+                                                 CellVal* __restrict__ out)
 {
 #pragma clang fp contract(off)
   __shared__ unsigned long long blk_below;
@@ -229,35 +149,28 @@ __global__ void __launch_bounds__(256) k_bg_deposit(BgArgs a, const unsigned* __
     blk_below = 0;
   __syncthreads();
   unsigned long long below = 0;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (long)gridDim.x * blockDim.x) {
-    const unsigned key = keys[i];
-    if (key >= a.sentinel || (i > 0 && keys[i - 1] == key))
+  const long nr = *nrun;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += (long)gridDim.x * blockDim.x) {
+    if (run_keys[i] >= a.sentinel)  // the dead galaxies' run (sorted last)
       continue;
+    const long off = run_off[i], len = run_len[i];
     float st = 0.f, ws = 0.f, bm = 0.f, br = 0.f, xr = 0.f, sf = 0.f, hh = 0.f, hs = 0.f;
-    // the run in chunks of U: each chunk's keys and indices, then its records, are loaded
-    // together (one round trip each); entries past the run load the chunk's first record again
-    // (a cache hit) and add 0.0, which leaves every sum unchanged (no sum is -0.0)
+    // the run in chunks of U: a chunk's indices, then its records, are loaded together (one
+    // round trip each); slots past the run reload the run's last record (a cache hit) and add
+    // 0.0, which leaves every sum unchanged (no sum is -0.0)
     constexpr int U = 8;
-    bool more = true;
-    for (long j = i; more; j += U) {
-      unsigned kk[U], vv[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const long jj = j + u < a.n ? j + u : a.n - 1;
-        kk[u] = keys[jj];
-        vv[u] = vals[jj];
-      }
-      bool in_run[U];
+    for (long j = 0; j < len; j += U) {
+      unsigned vv[U];
 #pragma unroll
       for (int u = 0; u < U; u++)
-        in_run[u] = j + u < a.n && kk[u] == key;
+        vv[u] = vals[off + (j + u < len ? j + u : len - 1)];
       GalRec rr[U];
 #pragma unroll
       for (int u = 0; u < U; u++)
-        rr[u] = rec[in_run[u] ? vv[u] : vv[0]];
+        rr[u] = rec[vv[u]];
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        const bool m = in_run[u];
+        const bool m = j + u < len;
         const GalRec& r = rr[u];
         st = (float)((double)st + (m ? r.st : 0.0));
         ws = (float)((double)ws + (m ? r.ws : 0.0));
@@ -269,15 +182,140 @@ __global__ void __launch_bounds__(256) k_bg_deposit(BgArgs a, const unsigned* __
         hh = (float)((double)hh + (m ? r.hh : 0.0));
         hs = (float)((double)hs + (m ? r.hs : 0.0));
       }
-      more = in_run[U - 1];
     }
-    deposit_cell(a, key, st, ws, bm, br, xr, sf, hh, hs);
+    CellVal v;
+    v.st = st < 0 ? 0.f : st;
+    v.ws = (float)((double)ws < 0 ? 0.0 : (double)ws);
+    v.bm = bm < 0 ? 0.f : bm;
+    v.br = br < 0 ? 0.f : br;
+    v.xr = xr;
+    double s = (double)sf;
+    if (s < 0)
+      s = 0.0;
+    if (!a.inst)
+      s /= a.sfr_timescale;
+    v.sf = (float)s;
+    v.hh = hh < 0 ? 0.f : hh;
+    v.hs = hs < 0 ? 0.f : hs;
+    out[i] = v;
   }
   if (below)
     atomicAdd(&blk_below, below);
   __syncthreads();
   if (threadIdx.x == 0 && blk_below)
     atomicAdd(a.below, blk_below);
+}
+
+// first occupied-cell index of each grid row (and of the end, row N^2): lower bound of the row's
+// first key in the sorted keys of the occupied cells
+__global__ void k_bg_row_start(const unsigned* __restrict__ run_keys, const unsigned* __restrict__ nrun, int N,
+                               unsigned* __restrict__ row_run)
+{
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > (long)N * N)
+    return;
+  const unsigned target = (unsigned)(r * N);
+  unsigned lo = 0, hi = *nrun;
+  while (lo < hi) {
+    const unsigned mid = lo + (hi - lo) / 2;
+    if (run_keys[mid] < target)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  row_run[r] = lo;
+}
+
+// One wavefront per grid row (rows of at most 64 x DENSE_CPL cells): the row's occupied cells
+// are placed by an LDS map from cell to lane, their values pulled from that lane, and every grid's
+// row is written once with coalesced stores: the values, zero elsewhere.  The padding of stars,
+// weighted_sfr and the current spin-temperature grids is zeroed as the reference's clear does;
+// the BH grids' and the new history slot's padding are left as they are.
+constexpr int DENSE_CPL = 8;
+__global__ void __launch_bounds__(256) k_bg_dense(BgArgs a, const unsigned* __restrict__ run_keys,
+                                                  const CellVal* __restrict__ vals,
+                                                  const unsigned* __restrict__ row_run)
+{
+  __shared__ short map[4][64 * DENSE_CPL];
+  const int N = a.N, H2 = 2 * (N / 2 + 1);
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const long row = (long)blockIdx.x * 4 + w;
+  if (row >= (long)N * N)
+    return;  // no block barrier below: each wavefront works on its own map
+  const int rs = (int)row_run[row], re = (int)row_run[row + 1];
+  const unsigned key0 = (unsigned)(row * N);
+  float v[8][DENSE_CPL];
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+#pragma unroll
+    for (int u = 0; u < DENSE_CPL; u++)
+      v[k][u] = 0.f;
+  for (int c0 = rs; c0 < re; c0 += 64) {
+#pragma unroll
+    for (int u = 0; u < DENSE_CPL; u++)
+      map[w][lane + 64 * u] = -1;
+    __builtin_amdgcn_wave_barrier();
+    const bool have = c0 + lane < re;
+    const int src = have ? c0 + lane : c0;
+    const unsigned key = run_keys[src];
+    const CellVal cv = vals[src];
+    if (have)
+      map[w][key - key0] = (short)lane;
+    __builtin_amdgcn_wave_barrier();
+    const float f[8] = {cv.st, cv.ws, cv.bm, cv.br, cv.xr, cv.sf, cv.hh, cv.hs};
+#pragma unroll
+    for (int u = 0; u < DENSE_CPL; u++) {
+      const int j = map[w][lane + 64 * u];
+      const int from = j < 0 ? lane : j;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const float x = __shfl(f[k], from, 64);
+        v[k][u] = j < 0 ? v[k][u] : x;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  const long base = row * H2;
+#pragma unroll
+  for (int u = 0; u < DENSE_CPL; u++) {
+    const int z = lane + 64 * u;
+    if (z >= N)
+      continue;
+    const long p = base + z;
+    a.stars[p] = v[0][u];
+    a.wsfr[p] = v[1][u];
+    if (a.bh) {
+      a.g_bhm[p] = v[2][u];
+      a.g_bhar[p] = v[3][u];
+    }
+    if (a.spin) {
+      a.g_x[p] = v[4][u];
+      a.h_x[p] = v[4][u];
+      a.g_sfr[p] = v[5][u];
+      a.h_sfr[p] = v[5][u];
+      if (a.hard) {
+        a.g_h[p] = v[6][u];
+        a.h_h[p] = v[6][u];
+      }
+      if (a.soft) {
+        a.g_s[p] = v[7][u];
+        a.h_s[p] = v[7][u];
+      }
+    }
+  }
+  if (lane < H2 - N) {
+    const long p = base + N + lane;
+    a.stars[p] = 0.f;
+    a.wsfr[p] = 0.f;
+    if (a.spin) {
+      a.g_x[p] = 0.f;
+      a.g_sfr[p] = 0.f;
+      if (a.hard)
+        a.g_h[p] = 0.f;
+      if (a.soft)
+        a.g_s[p] = 0.f;
+    }
+  }
 }
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -289,6 +327,10 @@ int run_construct_baryon_grids(Ctx* c, const mxr_baryon_params* p, const mxr_gal
 {
   const int N = p->ReionGridDim;
   const long n = gal->n;
+  if (N > 64 * DENSE_CPL) {
+    set_error("construct_baryon_grids: ReionGridDim %d exceeds %d", N, 64 * DENSE_CPL);
+    return MXR_ERR_UNSUPPORTED;
+  }
   const long npad = 2 * n_complex(N), nstore = p->Flag_IncludeSpinTemp ? p->NstoreSnapshots_Heating : 0;
   BgArgs a{};
   a.N = N;
@@ -329,23 +371,36 @@ int run_construct_baryon_grids(Ctx* c, const mxr_baryon_params* p, const mxr_gal
     a.nstore = 1;
   }
 
-  // device workspace: keys and values twice, the sort's temporary storage, the error flag and
-  // the below-limit counter, then (host galaxies) the staged galaxy arrays
-  size_t sort_bytes = 0;
+  // device workspace: keys and values twice (after the sort, the first pair holds the occupied
+  // cells' keys and run lengths), the run offsets, the library calls' temporary storage, the
+  // error flag, the below-limit counter and the run count, the galaxy records, the cells'
+  // values, the rows' first cells, then (host galaxies) the staged galaxy arrays
+  size_t tmp_bytes = 0;
   int end_bit = 1;
   while (end_bit < 32 && ((unsigned long long)a.sentinel >> end_bit) != 0)
     end_bit++;
-  if (n > 0)
-    MXR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const unsigned*)nullptr, (unsigned*)nullptr,
+  if (n > 0) {
+    size_t b = 0;
+    MXR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b, (const unsigned*)nullptr, (unsigned*)nullptr,
                                                (const unsigned*)nullptr, (unsigned*)nullptr, (int)n, 0, end_bit,
                                                c->stream));
+    tmp_bytes = std::max(tmp_bytes, b);
+    MXR_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, b, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                                  (unsigned*)nullptr, (unsigned*)nullptr, (int)n, c->stream));
+    tmp_bytes = std::max(tmp_bytes, b);
+    MXR_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, (const unsigned*)nullptr, (unsigned*)nullptr, (int)n,
+                                             c->stream));
+    tmp_bytes = std::max(tmp_bytes, b);
+  }
   const size_t kb = align256(sizeof(unsigned) * (size_t)n);
   const bool host_gal = gal->mem == MXR_MEM_HOST;
   const size_t stage = host_gal ? align256(sizeof(float) * 3 * n) + align256(sizeof(int) * n) +
                                     11 * align256(sizeof(double) * n)
                                 : 0;
   const size_t rb = align256(sizeof(GalRec) * (size_t)n);
-  const size_t need = 4 * kb + align256(sort_bytes) + 256 + rb + stage;
+  const size_t cvb = align256(sizeof(CellVal) * (size_t)n);
+  const size_t rrb = align256(sizeof(unsigned) * ((size_t)N * N + 1));
+  const size_t need = 5 * kb + align256(tmp_bytes) + 256 + rb + cvb + rrb + stage;
   if (c->bg_ws_cap < need) {
     (void)hipFree(c->bg_ws);
     c->bg_ws = nullptr;
@@ -358,12 +413,18 @@ int run_construct_baryon_grids(Ctx* c, const mxr_baryon_params* p, const mxr_gal
   unsigned* keys_out = reinterpret_cast<unsigned*>(w + kb);
   unsigned* vals_in = reinterpret_cast<unsigned*>(w + 2 * kb);
   unsigned* vals_out = reinterpret_cast<unsigned*>(w + 3 * kb);
-  void* sort_tmp = w + 4 * kb;
-  char* flags = w + 4 * kb + align256(sort_bytes);
+  unsigned* run_keys = keys_in;  // free once sorted
+  unsigned* run_len = vals_in;
+  unsigned* run_off = reinterpret_cast<unsigned*>(w + 4 * kb);
+  void* lib_tmp = w + 5 * kb;
+  char* flags = w + 5 * kb + align256(tmp_bytes);
   a.err = reinterpret_cast<int*>(flags);
   a.below = reinterpret_cast<unsigned long long*>(flags + 128);
+  unsigned* nrun = reinterpret_cast<unsigned*>(flags + 192);  // stays 0 without galaxies
   GalRec* recs = reinterpret_cast<GalRec*>(flags + 256);
-  char* st = flags + 256 + rb;
+  CellVal* cell_vals = reinterpret_cast<CellVal*>(flags + 256 + rb);
+  unsigned* row_run = reinterpret_cast<unsigned*>(flags + 256 + rb + cvb);
+  char* st = flags + 256 + rb + cvb + rrb;
 
   // galaxy arrays on the device
   auto dev_in = [&](const void* src, size_t bytes, const void** dst) -> int {
@@ -453,20 +514,54 @@ int run_construct_baryon_grids(Ctx* c, const mxr_baryon_params* p, const mxr_gal
     }
   }
   if (rc == MXR_OK && n > 0) {
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(sort_tmp, sort_bytes, keys_in, keys_out, vals_in, vals_out,
-                                                      (int)n, 0, end_bit, c->stream);
+    size_t b = tmp_bytes;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(lib_tmp, b, keys_in, keys_out, vals_in, vals_out, (int)n, 0,
+                                                      end_bit, c->stream);
     if (e != hipSuccess)
       rc = hip_fail(e, "hipcub::DeviceRadixSort::SortPairs", __FILE__, __LINE__);
+    // the occupied cells: their keys, run lengths and count, and the runs' offsets
+    b = tmp_bytes;
+    if (rc == MXR_OK &&
+        (e = hipcub::DeviceRunLengthEncode::Encode(lib_tmp, b, keys_out, run_keys, run_len, nrun, (int)n,
+                                                   c->stream)) != hipSuccess)
+      rc = hip_fail(e, "hipcub::DeviceRunLengthEncode::Encode", __FILE__, __LINE__);
+    b = tmp_bytes;
+    if (rc == MXR_OK &&
+        (e = hipcub::DeviceScan::ExclusiveSum(lib_tmp, b, run_len, run_off, (int)n, c->stream)) != hipSuccess)
+      rc = hip_fail(e, "hipcub::DeviceScan::ExclusiveSum", __FILE__, __LINE__);
   }
   if (rc == MXR_OK) {
-    hipLaunchKernelGGL(k_bg_prep, dim3(ew_blocks(npad)), dim3(256), 0, c->stream, a);
-    hipError_t e = hipGetLastError();
+    hipError_t e = hipSuccess;
+    // reionization.c:2093-2141, the history part: each history moves up by one slot, whole
+    // slots (padding included) from the top down, one device copy per slot (two streams each,
+    // which the memory system serves faster than all slots moved together); slot 0's real
+    // cells are then written by the row writer, so its padding keeps the old slot 0's (the
+    // reference's finalisation writes real cells only)
+    if (a.spin && a.nstore > 1) {
+      float* hs[4] = {a.h_sfr, a.h_x, a.hard ? a.h_h : nullptr, a.soft ? a.h_s : nullptr};
+      for (int k = 0; k < 4 && e == hipSuccess; k++)
+        for (int sl = a.nstore - 2; hs[k] && sl >= 0 && e == hipSuccess; sl--)
+          e = hipMemcpyAsync(hs[k] + (long)(sl + 1) * npad, hs[k] + (long)sl * npad, sizeof(float) * npad,
+                             hipMemcpyDeviceToDevice, c->stream);
+    }
     if (e == hipSuccess && n > 0) {
       hipLaunchKernelGGL(k_bg_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a, recs);
       e = hipGetLastError();
     }
     if (e == hipSuccess && n > 0) {
-      hipLaunchKernelGGL(k_bg_deposit, dim3(ew_blocks(n)), dim3(256), 0, c->stream, a, keys_out, vals_out, recs);
+      hipLaunchKernelGGL(k_bg_runs, dim3(ew_blocks(n)), dim3(256), 0, c->stream, a, run_keys, run_len, run_off, nrun,
+                         vals_out, recs, cell_vals);
+      e = hipGetLastError();
+    }
+    const long nrows = (long)N * N;
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_bg_row_start, dim3((unsigned)((nrows + 1 + 255) / 256)), dim3(256), 0, c->stream, run_keys,
+                         nrun, N, row_run);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_bg_dense, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0, c->stream, a, run_keys,
+                         cell_vals, row_run);
       e = hipGetLastError();
     }
     unsigned long long below = 0;

@@ -120,3 +120,26 @@ def test_512_totals_and_repeatability():
     assert float(out["sfr"].double().sum()) == pytest.approx(float(gals["Sfr"][live].sum()), rel=1e-5)
     # the second call shifted the first call's sfr into slot 1
     assert torch.equal(out["sfr_histories"][npad:2 * npad], out["sfr_histories"][:npad])
+
+
+def test_unaligned_device_histories_vs_oracle():
+    """Device grids at an odd float offset (the history shift then moves single floats instead of
+    16-byte vectors) give the oracle's grids bit for bit."""
+    import torch
+    N, box, nstore = 32, 40.0, 5
+    p = mxr.baryon_params(N, box, spin_temp=True, bh=True, agn_xray=mxr.MXR_AGN_XRAY_BOTH, nstore=nstore)
+    gals = synth.galaxies(30000, box, seed=21)
+    grids0 = synth.baryon_grids(N, nstore, seed=4)
+    ref = {k: v.copy() for k, v in grids0.items()}
+    below_ref = O.construct_baryon_grids(p, gals, ref)
+    g = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in gals.items()}
+    bufs, out = {}, {}
+    for k, v in grids0.items():
+        bufs[k] = torch.zeros(v.size + 1, device="cuda")
+        bufs[k][1:] = torch.from_numpy(v).cuda()
+        out[k] = bufs[k][1:]
+    below = mxr.Context(N).construct_baryon_grids(p, g, out)
+    torch.cuda.synchronize()
+    assert below == below_ref
+    for k in grids0:
+        assert np.array_equal(out[k].cpu().numpy(), ref[k]), k
