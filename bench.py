@@ -192,9 +192,10 @@ def grid_leg(args, torch):
     torch.cuda.synchronize()
     ms = 1e3 * (time.perf_counter() - t0) / args.grid_steps
     f = (n_hi // 2 + 1) / n_hi
-    # per hi-res cell: pad read 4 + write 4, r2c 4 + 8 f, y and x passes 16 f each, filter 16 f,
-    # inverse x and y 16 f each, c2r 8 f + 4; the subsample reads and writes 1/8 of that in N^3
-    model = n_hi ** 3 * (8 + 4 + 8 * f + 5 * 16 * f + 8 * f + 4) + N ** 3 * 8
+    # per hi-res cell: r2c from the real layout 4 + 8 f, forward y and x 16 f each, inverse x
+    # (window on its loads) and y 16 f each, c2r 8 f + 4; the subsample reads and writes N^3
+    # cells; the window table (1/8 of a grid, built once per box) is not counted
+    model = n_hi ** 3 * (4 + 8 * f + 4 * 16 * f + 8 * f + 4) + N ** 3 * 8
     out_j = {"metric": f"input cells/s (prepare_grid {n_hi}^3 -> {N}^3, smoothing)", "value": n_hi ** 3 / (ms / 1e3),
              "unit": "cells/s", "ms_per_call": round(ms, 3), "steps": args.grid_steps,
              "config": {"workload": f"smooth_grid + subsample_grid + overdensity, {n_hi}^3 -> {N}^3, box {box} cMpc/h",
