@@ -157,8 +157,8 @@ typedef struct mxr_tuning {
   int zdirect_rpb;  /* rows per wavefront of the direct z pass (1 .. 4096)                  */
   int live_tiles;   /* 1: x and y pass grids cover only the tiles that can hold kept sharp-k
                        columns; 0: every tile is launched and empty ones return at once    */
-  int col_tc_x;     /* columns per workgroup of the per-radius x pass (4, 8 or 16)          */
-  int col_tc_y;     /* columns per workgroup of the per-radius y pass (4, 8 or 16)          */
+  int col_tc_x;     /* columns per workgroup of the per-radius x pass (4, 8, 16 or 32)      */
+  int col_tc_y;     /* columns per workgroup of the per-radius y pass (4, 8, 16 or 32)      */
 } mxr_tuning;
 int mxr_get_tuning(const mxr_ctx* ctx, mxr_tuning* t);
 int mxr_set_tuning(mxr_ctx* ctx, const mxr_tuning* t);

@@ -501,7 +501,7 @@ struct Cfg {
   static constexpr int M = N / 2;
   static constexpr int H = N / 2 + 1;
   static constexpr int TC = 16;
-  static constexpr int COL_T = (N / 8 >= 16) ? 16 : (N / 8 < 1 ? 1 : N / 8);
+  static constexpr int COL_T = N == 512 ? 32 : (N / 8 >= 16) ? 16 : (N / 8 < 1 ? 1 : N / 8);
   static constexpr int COL_NT = TC * COL_T;
   static constexpr int COL_TPB = 1;  // column tiles per block (2 overlaps loads with transforms but needs ~280 VGPRs)
   static constexpr int ROW_T = pow2_floor((M / 8 >= 16) ? 16 : (M / 8 < 1 ? 1 : M / 8));
@@ -611,7 +611,7 @@ struct Passes {
     return MXR_OK;
   }
 
-  // the per-radius inverse passes take their tile width from `tc` (4, 8 or 16); the others use 16
+  // the per-radius inverse passes take their tile width from `tc` (4, 8, 16 or 32); the others use 16
   template <int S, int MODE>
   static int col(Ctx* c, CGridPtrs in, GridPtrs out, int ngrid, ColGeom geo, FiltArgs fa, double div, int tc = C::TC)
   {
@@ -620,6 +620,8 @@ struct Passes {
         return col_tc<S, MODE, 4>(c, in, out, ngrid, geo, fa, div);
       if (tc == 8)
         return col_tc<S, MODE, 8>(c, in, out, ngrid, geo, fa, div);
+      if (tc == 32)
+        return col_tc<S, MODE, 32>(c, in, out, ngrid, geo, fa, div);
     }
     return col_tc<S, MODE, C::TC>(c, in, out, ngrid, geo, fa, div);
   }

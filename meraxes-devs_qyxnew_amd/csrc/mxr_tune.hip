@@ -25,14 +25,14 @@
 namespace mxr {
 
 // ---- the tuning record --------------------------------------------------------------------
-static bool valid_tc(int tc) { return tc == 4 || tc == 8 || tc == 16; }
+static bool valid_tc(int tc) { return tc == 4 || tc == 8 || tc == 16 || tc == 32; }
 
 int tuning_check(const mxr_tuning* t)
 {
   if (t->zdirect_kmax < 0 || t->zdirect_kmax > 64 || t->zdirect_rpb < 1 || t->zdirect_rpb > 4096 ||
       (t->live_tiles != 0 && t->live_tiles != 1) || !valid_tc(t->col_tc_x) || !valid_tc(t->col_tc_y)) {
     set_error("tuning out of range: zdirect_kmax %d (0..64), zdirect_rpb %d (1..4096), live_tiles %d (0, 1), "
-              "col_tc_x %d, col_tc_y %d (4, 8, 16)",
+              "col_tc_x %d, col_tc_y %d (4, 8, 16, 32)",
               t->zdirect_kmax, t->zdirect_rpb, t->live_tiles, t->col_tc_x, t->col_tc_y);
     return MXR_ERR_INVALID_ARG;
   }
@@ -484,8 +484,8 @@ int autotune(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_tuning
     knobs.push_back({0, {0, 8, 16, 24, 32, 48, 64}});
     knobs.push_back({1, {1, 2, 4, 8, 16, 32}});
   }
-  knobs.push_back({4, {4, 8, 16}});
-  knobs.push_back({3, {4, 8, 16}});
+  knobs.push_back({4, {4, 8, 16, 32}});
+  knobs.push_back({3, {4, 8, 16, 32}});
   if (sharpk)
     knobs.push_back({2, {0, 1}});
   for (const Knob& k : knobs) {
