@@ -43,21 +43,40 @@ __global__ void k_ts_source(TsOpsArgs oa, const float* __restrict__ hs, const fl
                             const float* __restrict__ hs0, const float* __restrict__ hx0, long hstride,
                             float* __restrict__ ss, float* __restrict__ sx, long n)
 {
+  // ops in batches of OB: the batch's history loads are issued together (the op list is
+  // uniform, so the branches are scalar), then the ops are applied in order
+  constexpr int OB = 8;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     float v[2] = {oa.init ? hs0[i] : ss[i], oa.init ? hx0[i] : sx[i]};
-    for (int k = 0; k < oa.nops; k++) {
-      const TsOp o = oa.ops[k];
+    for (int k0 = 0; k0 < oa.nops; k0 += OB) {
+      float hv[OB][2];
 #pragma unroll
-      for (int g = 0; g < 2; g++) {
-        const float* __restrict__ h = g == 0 ? hs : hx;
-        if (o.kind == TS_OP_CLEAR)
-          v[g] = 0.0f;
-        else if (o.kind == TS_OP_SCALE)
-          v[g] = (float)((double)v[g] * o.wd);
-        else if (o.kind == TS_OP_LOAD)
-          v[g] = __fmul_rn(h[(long)o.hist * hstride + i], o.wf);
-        else
-          v[g] = __fadd_rn(v[g], __fmul_rn(h[(long)o.hist * hstride + i], o.wf));
+      for (int u = 0; u < OB; u++) {
+        hv[u][0] = hv[u][1] = 0.0f;
+        if (k0 + u < oa.nops) {
+          const TsOp o = oa.ops[k0 + u];
+          if (o.kind == TS_OP_LOAD || o.kind == TS_OP_ADD) {
+            hv[u][0] = hs[(long)o.hist * hstride + i];
+            hv[u][1] = hx[(long)o.hist * hstride + i];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < OB; u++) {
+        if (k0 + u >= oa.nops)
+          break;
+        const TsOp o = oa.ops[k0 + u];
+#pragma unroll
+        for (int g = 0; g < 2; g++) {
+          if (o.kind == TS_OP_CLEAR)
+            v[g] = 0.0f;
+          else if (o.kind == TS_OP_SCALE)
+            v[g] = (float)((double)v[g] * o.wd);
+          else if (o.kind == TS_OP_LOAD)
+            v[g] = __fmul_rn(hv[u][g], o.wf);
+          else
+            v[g] = __fadd_rn(v[g], __fmul_rn(hv[u][g], o.wf));
+        }
       }
     }
     ss[i] = v[0];
