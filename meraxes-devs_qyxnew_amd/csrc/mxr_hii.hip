@@ -400,25 +400,19 @@ __device__ __forceinline__ void hii_cell(const HiiEpi& ea, int ir, int ip, float
 //   sA = (A0+A2)+(A1+A3), dA = (A0+A2)-(A1+A3) (same for B).
 // Lane p = 0 takes (C_k, S_k) = w_k (1, 1) and forms y[0] = sA, y[N/2] = dA,
 // y[N/4] = (A0-A2) - (B1-B3), y[3N/4] = (A0-A2) + (B1-B3).  A block is one wavefront owning 64
-// consecutive p of `rpb` rows; the twiddles stay in registers across the rows and a row's
-// coefficients, uniform over the wave, are read through the scalar cache.
-typedef const float __attribute__((address_space(4)))* UniformF;
-
+// consecutive p of `rpb` rows; the twiddles stay in registers across the rows.  A row's
+// coefficients (k < kc of every grid; zero above) sit in LDS, where every lane reads the same
+// entry (a broadcast); the next row's are loaded, one or a few per lane, while the current row
+// is summed, and written to LDS after it.
 template <int KMAX>
-__device__ __forceinline__ void direct_row(const float2* grid, long row, int H, int kc, const float* C, const float* S,
-                                           bool lane0, float* o)
+__device__ __forceinline__ void direct_sum(const float2* cf, const float* C, const float* S, bool lane0, float* o)
 {
-  const UniformF src = (UniformF)(grid + row * H);
   float A[4] = {0.f, 0.f, 0.f, 0.f}, B[4] = {0.f, 0.f, 0.f, 0.f};
-  // KMAX < H: every load stays inside the row.  KMAX - 8 < kc <= KMAX; entries k >= kc hold
-  // stale values (the y pass skips cut columns) and are replaced by zero
 #pragma unroll
   for (int k = 0; k < KMAX; k++) {
-    const unsigned m = (k < KMAX - 8 || k < kc) ? ~0u : 0u;  // uniform: stays on the scalar unit
-    const float a = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(src[2 * k]) & m));
-    const float b = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(src[2 * k + 1]) & m));
-    A[k & 3] = fmaf(a, C[k], A[k & 3]);
-    B[k & 3] = fmaf(b, S[k], B[k & 3]);
+    const float2 ab = cf[k];
+    A[k & 3] = fmaf(ab.x, C[k], A[k & 3]);
+    B[k & 3] = fmaf(ab.y, S[k], B[k & 3]);
   }
   const float e02 = A[0] + A[2], e13 = A[1] + A[3], f02 = B[0] + B[2], f13 = B[1] + B[3];
   const float sA = e02 + e13, dA = e02 - e13, sB = f02 + f13, dB = f02 - f13;
@@ -436,6 +430,26 @@ __global__ void __launch_bounds__(64) k_row_direct(CGridPtrs in, long nrows, int
   constexpr int H = N / 2 + 1, Q = N / 4, SEGS = Q / 64;
   static_assert(Q % 64 == 0, "N/4 must be a multiple of the wavefront");
   static_assert(KMAX < H, "direct summation covers filtered radii only");
+  constexpr bool SPIN = epi_spin(EPI), BH = epi_bh(EPI);
+  constexpr int IXE = 3, IBHM = 3 + (SPIN ? 1 : 0), IBHAR = 4 + (SPIN ? 1 : 0);
+  // grids in summation order: weighted_sfr, [effective_bhar], [x_e], [effective_bhm], stars, deltax
+  constexpr int NG = 3 + (SPIN ? 1 : 0) + (BH ? 2 : 0);
+  constexpr int G_BHAR = 1, G_XE = 1 + (BH ? 1 : 0), G_BHM = G_XE + (SPIN ? 1 : 0), G_ST = NG - 2, G_DX = NG - 1;
+  constexpr int NL = (NG * KMAX + 63) / 64;  // coefficient loads per lane and row
+  __shared__ float2 cf[NG * KMAX];
+  const float2* gp[NG];
+  {
+    int s = 0;
+    gp[s++] = in.p[2];
+    if constexpr (BH)
+      gp[s++] = in.p[IBHAR];
+    if constexpr (SPIN)
+      gp[s++] = in.p[IXE];
+    if constexpr (BH)
+      gp[s++] = in.p[IBHM];
+    gp[s++] = in.p[1];
+    gp[s++] = in.p[0];
+  }
   const int seg = (int)(blockIdx.x % SEGS);
   const long row0 = (long)(blockIdx.x / SEGS) * rpb;
   const int p = seg * 64 + (int)threadIdx.x;
@@ -450,6 +464,37 @@ __global__ void __launch_bounds__(64) k_row_direct(CGridPtrs in, long nrows, int
   }
   const int n[4] = {lane0 ? 0 : p, lane0 ? N / 2 : N - p, lane0 ? N / 4 : N / 2 + p, lane0 ? 3 * N / 4 : N / 2 - p};
   const long row1 = row0 + rpb < nrows ? row0 + rpb : nrows;
+  // this lane's coefficient slots e = lane + 64 i: grid e / KMAX, mode e % KMAX; slots past the
+  // last grid and modes k >= kc hold zero (the y pass leaves stale values there), the loads of
+  // such slots read the row's first entry (in bounds) and are discarded
+  auto fetch = [&](long row, float2* pre) {
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+      const int e = (int)threadIdx.x + 64 * i;
+      const int g = e / KMAX, k = e % KMAX;
+      const bool ok = e < NG * KMAX && k < kc;
+      const float2* base = gp[0];
+#pragma unroll
+      for (int t = 1; t < NG; t++)
+        base = g == t ? gp[t] : base;
+      const float2 x = base[row * H + (ok ? k : 0)];
+      pre[i] = ok ? x : make_float2(0.f, 0.f);
+    }
+  };
+  auto commit = [&](const float2* pre) {
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+      const int e = (int)threadIdx.x + 64 * i;
+      if (e < NG * KMAX)
+        cf[e] = pre[i];
+    }
+    SyncWave::sync();
+  };
+  float2 pre[NL];
+  if (row0 < row1) {
+    fetch(row0, pre);
+    commit(pre);
+  }
   // flags one row ahead, so their latency hides behind the row's sums
   unsigned char fl[4] = {0, 0, 0, 0};
   if (!ea.first && row0 < row1)
@@ -457,16 +502,17 @@ __global__ void __launch_bounds__(64) k_row_direct(CGridPtrs in, long nrows, int
     for (int e = 0; e < 4; e++)
       fl[e] = ea.ion[(int)row0 * N + n[e]];
   for (long row = row0; row < row1; row++) {
+    const bool more = row + 1 < row1;
+    if (more)
+      fetch(row + 1, pre);
     unsigned char nfl[4] = {0, 0, 0, 0};
-    if (!ea.first && row + 1 < row1)
+    if (!ea.first && more)
 #pragma unroll
       for (int e = 0; e < 4; e++)
         nfl[e] = ea.ion[(int)(row + 1) * N + n[e]];
-    constexpr bool SPIN = epi_spin(EPI), BH = epi_bh(EPI);
-    constexpr int IXE = 3, IBHM = 3 + (SPIN ? 1 : 0), IBHAR = 4 + (SPIN ? 1 : 0);
     float sfr[4], xe[4] = {0.f, 0.f, 0.f, 0.f}, bf[4] = {0.f, 0.f, 0.f, 0.f}, st[4], dx[4];
     // weighted_sfr and effective_bhar: transformed, clamped, dropped (Gamma12 alone reads them)
-    direct_row<KMAX>(in.p[2], row, H, kc, C, S, lane0, sfr);
+    direct_sum<KMAX>(cf, C, S, lane0, sfr);
 #pragma unroll
     for (int e = 0; e < 4; e++) {
       float w = fmaxf(sfr[e], 0.0f);
@@ -475,7 +521,7 @@ __global__ void __launch_bounds__(64) k_row_direct(CGridPtrs in, long nrows, int
     }
     if constexpr (BH) {
       float bar[4];
-      direct_row<KMAX>(in.p[IBHAR], row, H, kc, C, S, lane0, bar);
+      direct_sum<KMAX>(cf + G_BHAR * KMAX, C, S, lane0, bar);
 #pragma unroll
       for (int e = 0; e < 4; e++) {
         float w = fmaxf(bar[e], 0.0f);
@@ -484,11 +530,16 @@ __global__ void __launch_bounds__(64) k_row_direct(CGridPtrs in, long nrows, int
       }
     }
     if constexpr (SPIN)
-      direct_row<KMAX>(in.p[IXE], row, H, kc, C, S, lane0, xe);
+      direct_sum<KMAX>(cf + G_XE * KMAX, C, S, lane0, xe);
     if constexpr (BH)
-      direct_row<KMAX>(in.p[IBHM], row, H, kc, C, S, lane0, bf);
-    direct_row<KMAX>(in.p[1], row, H, kc, C, S, lane0, st);
-    direct_row<KMAX>(in.p[0], row, H, kc, C, S, lane0, dx);
+      direct_sum<KMAX>(cf + G_BHM * KMAX, C, S, lane0, bf);
+    direct_sum<KMAX>(cf + G_ST * KMAX, C, S, lane0, st);
+    direct_sum<KMAX>(cf + G_DX * KMAX, C, S, lane0, dx);
+    // every lane has read this row's coefficients (a wavefront's LDS accesses run in order)
+    if (more) {
+      SyncWave::sync();
+      commit(pre);
+    }
 #pragma unroll
     for (int e = 0; e < 4; e++)
       hii_cell(ea, (int)row * N + n[e], (int)row * 2 * H + n[e], dx[e], st[e], xe[e], bf[e], fl[e]);
