@@ -424,8 +424,10 @@ __device__ __forceinline__ void direct_sum(const float2* cf, const float* C, con
 }
 
 template <int N, int KMAX, int EPI>
-__global__ void __launch_bounds__(64) k_row_direct(CGridPtrs in, long nrows, int rpb, const float2* __restrict__ twg,
-                                                   int kc, HiiEpi ea)
+// four waves per SIMD (128 VGPRs) for the find_HII epilogue with up to 16 modes; the x_e / BH
+// epilogues keep their registers (they would spill)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KMAX <= 16 && EPI == C2R_HII ? 4 : 1)))
+k_row_direct(CGridPtrs in, long nrows, int rpb, const float2* __restrict__ twg, int kc, HiiEpi ea)
 {
   constexpr int H = N / 2 + 1, Q = N / 4, SEGS = Q / 64;
   static_assert(Q % 64 == 0, "N/4 must be a multiple of the wavefront");
