@@ -496,7 +496,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this path runs the whole grid on a single MI355X and has "
-              "no multi-GPU mode (see DESIGN.md for how it would shard).", file=sys.stderr)
+              "no multi-GPU mode.", file=sys.stderr)
         sys.exit(2)
 
     import torch
