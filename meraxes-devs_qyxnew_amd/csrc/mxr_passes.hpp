@@ -620,8 +620,9 @@ struct Passes {
         return col_tc<S, MODE, 4>(c, in, out, ngrid, geo, fa, div);
       if (tc == 8)
         return col_tc<S, MODE, 8>(c, in, out, ngrid, geo, fa, div);
-      if (tc == 32)
-        return col_tc<S, MODE, 32>(c, in, out, ngrid, geo, fa, div);
+      if constexpr (32 * C::COL_T <= 1024)  // a workgroup holds at most 1024 threads
+        if (tc == 32)
+          return col_tc<S, MODE, 32>(c, in, out, ngrid, geo, fa, div);
     }
     return col_tc<S, MODE, C::TC>(c, in, out, ngrid, geo, fa, div);
   }
