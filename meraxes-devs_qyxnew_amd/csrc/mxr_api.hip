@@ -329,6 +329,12 @@ void mxr_destroy(mxr_ctx* h)
   (void)hipFree(c.partials);
   (void)hipFree(c.ps_part);
   (void)hipFree(c.bg_ws);
+  if (c.bg_ev_in)
+    (void)hipEventDestroy(c.bg_ev_in);
+  if (c.bg_ev_shift)
+    (void)hipEventDestroy(c.bg_ev_shift);
+  if (c.bg_stream)
+    (void)hipStreamDestroy(c.bg_stream);
   (void)hipFree(c.wtab);
   if (c.hires)
     mxr_destroy(c.hires);

@@ -8,7 +8,8 @@
 // sums its run of galaxies in order with the reference's arithmetic and finalises the cell's
 // values.  One wavefront per grid row then writes the row of every grid once, coalesced: the
 // occupied cells' values and zero elsewhere, which is what the reference's clear and
-// finalisation give a cell without galaxies.  The histories move up by device copies first.
+// finalisation give a cell without galaxies.  The histories move up by device copies on a second
+// stream meanwhile.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -513,6 +514,37 @@ int run_construct_baryon_grids(Ctx* c, const mxr_baryon_params* p, const mxr_gal
       rc = MXR_ERR_INVALID_ARG;
     }
   }
+  // reionization.c:2093-2141, the history part: each history moves up by one slot, whole
+  // slots (padding included) from the top down, one device copy per slot (two streams each,
+  // which the memory system serves faster than all slots moved together).  The copies run on a
+  // second stream, beside the keys, sort and cell sums, which do not touch the grids; the row
+  // writer waits for them.  They start once the positions are known to be valid, so a rejected
+  // call leaves the grids as they were.  Slot 0's real cells are then written by the row writer, so its
+  // padding keeps the old slot 0's (the reference's finalisation writes real cells only).
+  bool shifting = false;
+  if (rc == MXR_OK && a.spin && a.nstore > 1) {
+    hipError_t e = hipSuccess;
+    if (!c->bg_stream)
+      e = hipStreamCreateWithFlags(&c->bg_stream, hipStreamNonBlocking);
+    if (e == hipSuccess && !c->bg_ev_in)
+      e = hipEventCreateWithFlags(&c->bg_ev_in, hipEventDisableTiming);
+    if (e == hipSuccess && !c->bg_ev_shift)
+      e = hipEventCreateWithFlags(&c->bg_ev_shift, hipEventDisableTiming);
+    if (e == hipSuccess)
+      e = hipEventRecord(c->bg_ev_in, c->stream);  // staged grids in place
+    if (e == hipSuccess)
+      e = hipStreamWaitEvent(c->bg_stream, c->bg_ev_in, 0);
+    shifting = e == hipSuccess;
+    float* hs[4] = {a.h_sfr, a.h_x, a.hard ? a.h_h : nullptr, a.soft ? a.h_s : nullptr};
+    for (int k = 0; k < 4 && e == hipSuccess; k++)
+      for (int sl = a.nstore - 2; hs[k] && sl >= 0 && e == hipSuccess; sl--)
+        e = hipMemcpyAsync(hs[k] + (long)(sl + 1) * npad, hs[k] + (long)sl * npad, sizeof(float) * npad,
+                           hipMemcpyDeviceToDevice, c->bg_stream);
+    if (e == hipSuccess)
+      e = hipEventRecord(c->bg_ev_shift, c->bg_stream);
+    if (e != hipSuccess)
+      rc = hip_fail(e, "history shift", __FILE__, __LINE__);
+  }
   if (rc == MXR_OK && n > 0) {
     size_t b = tmp_bytes;
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(lib_tmp, b, keys_in, keys_out, vals_in, vals_out, (int)n, 0,
@@ -532,18 +564,6 @@ int run_construct_baryon_grids(Ctx* c, const mxr_baryon_params* p, const mxr_gal
   }
   if (rc == MXR_OK) {
     hipError_t e = hipSuccess;
-    // reionization.c:2093-2141, the history part: each history moves up by one slot, whole
-    // slots (padding included) from the top down, one device copy per slot (two streams each,
-    // which the memory system serves faster than all slots moved together); slot 0's real
-    // cells are then written by the row writer, so its padding keeps the old slot 0's (the
-    // reference's finalisation writes real cells only)
-    if (a.spin && a.nstore > 1) {
-      float* hs[4] = {a.h_sfr, a.h_x, a.hard ? a.h_h : nullptr, a.soft ? a.h_s : nullptr};
-      for (int k = 0; k < 4 && e == hipSuccess; k++)
-        for (int sl = a.nstore - 2; hs[k] && sl >= 0 && e == hipSuccess; sl--)
-          e = hipMemcpyAsync(hs[k] + (long)(sl + 1) * npad, hs[k] + (long)sl * npad, sizeof(float) * npad,
-                             hipMemcpyDeviceToDevice, c->stream);
-    }
     if (e == hipSuccess && n > 0) {
       hipLaunchKernelGGL(k_bg_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a, recs);
       e = hipGetLastError();
@@ -559,6 +579,8 @@ int run_construct_baryon_grids(Ctx* c, const mxr_baryon_params* p, const mxr_gal
                          nrun, N, row_run);
       e = hipGetLastError();
     }
+    if (e == hipSuccess && shifting)
+      e = hipStreamWaitEvent(c->stream, c->bg_ev_shift, 0);
     if (e == hipSuccess) {
       hipLaunchKernelGGL(k_bg_dense, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0, c->stream, a, run_keys,
                          cell_vals, row_run);
@@ -581,6 +603,8 @@ int run_construct_baryon_grids(Ctx* c, const mxr_baryon_params* p, const mxr_gal
         *grids->history_head = ring_slot;
     }
   }
+  if (shifting)  // the copies may still run when an earlier step failed
+    (void)hipStreamSynchronize(c->bg_stream);
   for (int k = 0; k < 12; k++)
     (void)hipFree(tmp[k]);
   return rc;

@@ -138,9 +138,12 @@ struct Ctx {
   float* wtab = nullptr;
   float wtab_dk = 0, wtab_R = 0;
   int wtab_ftype = -1;
-  // galaxies to grids: sort keys and values, the sort's storage, staged galaxy arrays
+  // galaxies to grids: sort keys and values, the sort's storage, staged galaxy arrays; a second
+  // stream (default priority) for the history shift's copies and the events ordering it
   void* bg_ws = nullptr;
   size_t bg_ws_cap = 0;            // bytes
+  hipStream_t bg_stream = nullptr;
+  hipEvent_t bg_ev_in = nullptr, bg_ev_shift = nullptr;
   double fin_total_n_cells = 0, fin_sfr_conv = 0;
   int fin_n_radii = 0;
   int fin_bh = 0;
