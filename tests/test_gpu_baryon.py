@@ -143,3 +143,25 @@ def test_unaligned_device_histories_vs_oracle():
     assert below == below_ref
     for k in grids0:
         assert np.array_equal(out[k].cpu().numpy(), ref[k]), k
+
+
+def test_bitwise_vs_oracle_512():
+    """Rows of 512 cells (every lane of the row writer holding 8 cells): all grids bit for bit
+    against the oracle, with spin temperature, BH feedback and both AGN bands, two history slots,
+    device memory."""
+    import torch
+    N, box, nstore = 512, 400.0, 2
+    p = mxr.baryon_params(N, box, spin_temp=True, bh=True, agn_xray=mxr.MXR_AGN_XRAY_BOTH, nstore=nstore,
+                          BlackHoleMassLimitReion=1e-5)
+    gals = synth.galaxies(400_000, box, seed=5120, n_halos=40_000)
+    grids0 = synth.baryon_grids(N, nstore, seed=7)
+    ref = {k: v.copy() for k, v in grids0.items()}
+    below_ref = O.construct_baryon_grids(p, gals, ref)
+    g = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in gals.items()}
+    out = {k: torch.from_numpy(v).cuda() for k, v in grids0.items()}
+    del grids0
+    below = mxr.Context(N).construct_baryon_grids(p, g, out)
+    torch.cuda.synchronize()
+    assert below == below_ref
+    for k in out:
+        assert np.array_equal(out[k].cpu().numpy(), ref[k]), k
