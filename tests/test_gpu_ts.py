@@ -8,6 +8,8 @@ oracle no larger than twice the FP32 restatement's own (near-empty cells of spar
 fields carry FP32 FFT rounding noise, ~1e-4 relative in T_S, in both); globals within 1e-5; the
 R_ct = 0 averages within 1e-9.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -201,3 +203,23 @@ def test_too_few_histories_is_an_error(ctxs):
     with pytest.raises(mxr.MxrError) as e:
         _step(ctx, p, d, 1, xe, Tk, np.zeros(len(ZZ)))
     assert e.value.code == mxr.MXR_ERR_INVALID_ARG
+
+
+@pytest.mark.skipif(os.environ.get("MXR_LONG_TESTS") != "1",
+                    reason="256^3 with the FP64 oracle takes about 2.5 minutes of host time; set MXR_LONG_TESTS=1")
+def test_parity_256_benchmark_size(ctxs):
+    """The benchmark's size: 256^3, 200 cMpc/h, 40 shells, snapshot 40.  At this size FP32
+    rounding alone moves T_S, T_K and x_e by ~1e-3 relative (the FP32 oracle against the FP64
+    one), so the bound is the FP64-anchored one: the GPU's largest error against the FP64 oracle
+    at most twice the FP32 oracle's own, in every field."""
+    print("ts N=256", flush=True)
+    gpu, ref, tab, ref64 = _case(ctxs(256), 256, 200.0, 40, 1e-2)
+    assert tab.homogeneous == 0 and tab.NO_LIGHT == 0
+    N = 256
+    fields = {"TS_box": (lambda r: r["TS_box"], 0.0), "Tk_box": (lambda r: r["Tk_box"], 0.0),
+              "x_e": (lambda r: _cells(N, r["x_e_box_prev"]), 1e-6)}
+    for k, (get, floor) in fields.items():
+        eg = rel_err(get(gpu), get(ref64), floor).max()
+        eo = rel_err(get(ref), get(ref64), floor).max()
+        print(f"N=256 {k}: vs FP64 max gpu {eg:.2e} fp32-oracle {eo:.2e}", flush=True)
+        assert eg <= 2 * eo + 2e-6, (k, eg, eo)
