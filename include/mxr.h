@@ -336,7 +336,9 @@ typedef struct mxr_grid_params {
  * (r2c, /n_hi^3, real-space top-hat of radius BoxSize/N/2, c2r) and every (n_hi/N)-th point
  * kept; then, for MXR_GRID_DENSITY, the overdensity max(rho mean_inv - 1, floor) with the
  * file's box size file_box.  slab: the padded real N^3 grid (mem_out); its padding entries
- * are zero.  For callers with their own reader (the VELOCIraptor HDF5 grids).
+ * are zero.  For callers with their own reader (the VELOCIraptor HDF5 grids).  Smoothing keeps
+ * a context of dimension n_hi and its window table ((n_hi/2+1)^3 floats, rebuilt when the box
+ * size changes) with ctx until mxr_destroy.
  * Replaces smooth_grid and subsample_grid (src/core/read_grids.c:128-287) and the density
  * step of read_grid__gbptrees (read_grids-gbptrees.c:201-218) and read_grid__velociraptor
  * (read_grids-velociraptor.c:213-230).                                                    */
