@@ -128,6 +128,7 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
     if (max_radii > 0 && n_radii >= max_radii)
       break;
     n_radii++;
+    oracle_progress("radius", n_radii, 0);
 
     /* :292-348 copy, then filter unless this is the last step */
     for (int gi = 0; gi < ngrid; gi++) {

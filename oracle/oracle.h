@@ -15,7 +15,23 @@
 #ifndef MXR_ORACLE_H
 #define MXR_ORACLE_H
 
+#include <stdio.h>
+#include <stdlib.h>
+
 #include "../include/mxr.h"
+
+/* One line on stderr per radius or shell when ORACLE_PROGRESS is set, so that a long oracle
+ * run at the benchmark sizes shows that it is moving. */
+static inline void oracle_progress(const char* what, int i, int n)
+{
+  static int on = -1;
+  if (on < 0)
+    on = getenv("ORACLE_PROGRESS") != NULL;
+  if (on) {
+    fprintf(stderr, "oracle %s %d/%d\n", what, i, n);
+    fflush(stderr);
+  }
+}
 
 #ifdef __cplusplus
 extern "C" {

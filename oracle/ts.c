@@ -291,6 +291,7 @@ static float cT_approx(float z) { return 0.58 - 0.006 * (z - 10.0); }
         for (long i = 0; i < npad; i++)                                                                                \
           src[k][i] = hist[k][i];                                                                                      \
       for (int r = 0; r < F; r++) {                                                                                    \
+        oracle_progress("shell", r, F);                                                                                \
         for (int q = 0; q < tb->n_ops[r]; q++) {                                                                       \
           const int kind = tb->op_kind[r][q], hi = tb->op_hist[r][q];                                                  \
           const double w = tb->op_weight[r][q];                                                                        \

@@ -1,7 +1,8 @@
 """BF16-storage variant of find_HII_bubbles (stars and weighted_sfr stored as bfloat16, FP32
 compute).  Correctness: bit-identical to the FP32 path fed the BF16-rounded grids (the load only
 widens).  Accuracy of the storage itself: flag mismatch and |dxH| against the FP64 oracle on the
-original FP32 grids, reported (the 0.05% flag bound is not expected to hold for BF16)."""
+original FP32 grids, asserted against the FP32 path's bounds in a strict xfail: the 8-bit
+mantissa misses them, and a change that meets them shows up as an unexpected pass."""
 import numpy as np
 import pytest
 
@@ -12,8 +13,7 @@ from conftest import realisation
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("N,box", [(64, 67.8), (128, 125.0)])
-def test_bf16_storage(N, box):
+def _bf16_run(N, box):
     import torch
 
     print(f"bf16 N={N}", flush=True)
@@ -40,7 +40,24 @@ def test_bf16_storage(N, box):
     gpu = {k: getattr(sa, k).cpu().numpy() for k in sa.arrays()}
     ref = {k: np.asarray(v) for k, v in s64.arrays().items()}
     flips = int(np.sum((gpu["xH"] < 1e-5) != (ref["xH"] < 1e-5)))
-    print(f"bf16 N={N}: flag mismatch {flips / N ** 3 * 100:.4f}% of cells, "
-          f"|dxH| {abs(ga.volume_weighted_global_xH - r64.volume_weighted_global_xH):.2e}", flush=True)
-    assert abs(ga.volume_weighted_global_xH - r64.volume_weighted_global_xH) < 0.05
+    dxh = abs(ga.volume_weighted_global_xH - r64.volume_weighted_global_xH)
+    print(f"bf16 N={N}: flag mismatch {flips / N ** 3 * 100:.4f}% of cells, |dxH| {dxh:.2e}", flush=True)
     ctx.close()
+    return flips, dxh
+
+
+@pytest.mark.parametrize("N,box", [(64, 67.8), (128, 125.0)])
+def test_bf16_storage_identity(N, box):
+    """BF16 storage equals the FP32 path fed the BF16-rounded grids, bit for bit."""
+    _bf16_run(N, box)
+
+
+@pytest.mark.xfail(strict=True, reason="BF16 storage of stars/sfr misses the FP32 path's bounds against the FP64 "
+                                       "oracle: |dxH| 6e-5 at 64^3, 0.083% flags and 6.7e-4 at 128^3")
+@pytest.mark.parametrize("N,box", [(64, 67.8), (128, 125.0)])
+def test_bf16_storage_meets_bounds(N, box):
+    """The bounds the FP32 path meets, against the FP64 oracle on the original grids: flag mismatch
+    at most 0.05% of the cells and |dxH_global| at most 1e-5."""
+    flips, dxh = _bf16_run(N, box)
+    assert flips <= 5e-4 * N ** 3
+    assert dxh <= 1e-5

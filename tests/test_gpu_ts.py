@@ -8,7 +8,6 @@ oracle no larger than twice the FP32 restatement's own (near-empty cells of spar
 fields carry FP32 FFT rounding noise, ~1e-4 relative in T_S, in both); globals within 1e-5; the
 R_ct = 0 averages within 1e-9.
 """
-import os
 
 import numpy as np
 import pytest
@@ -205,21 +204,66 @@ def test_too_few_histories_is_an_error(ctxs):
     assert e.value.code == mxr.MXR_ERR_INVALID_ARG
 
 
-@pytest.mark.skipif(os.environ.get("MXR_LONG_TESTS") != "1",
-                    reason="256^3 with the FP64 oracle takes about 2.5 minutes of host time; set MXR_LONG_TESTS=1")
-def test_parity_256_benchmark_size(ctxs):
-    """The benchmark's size: 256^3, 200 cMpc/h, 40 shells, snapshot 40.  At this size FP32
-    rounding alone moves T_S, T_K and x_e by ~1e-3 relative (the FP32 oracle against the FP64
-    one), so the bound is the FP64-anchored one: the GPU's largest error against the FP64 oracle
-    at most twice the FP32 oracle's own, in every field."""
-    print("ts N=256", flush=True)
-    gpu, ref, tab, ref64 = _case(ctxs(256), 256, 200.0, 40, 1e-2)
+@pytest.mark.timeout(900)
+def test_chain_256_benchmark_size(ctxs):
+    """The spin-temperature workload at its benchmark size, then the two steps that consume it:
+    ComputeTs at 256^3, 200 cMpc/h, 40 shells, snapshot 40 (z = 8); find_HII_bubbles with
+    Flag_IncludeSpinTemp (x_e_box as a fourth filtered grid, Tk_box in the temperatures) on that
+    step's x_e and T_K; and the 21-cm brightness box with the spin-temperature factor from its
+    T_S and the new xH.  Each step takes identical inputs on the GPU and in the oracle.
+
+    ComputeTs: at this size FP32 rounding of the smoothed shells alone moves T_S, T_K and x_e by
+    ~2e-3 relative at worst and ~1e-3 at the 99.9th percentile (the FP32 oracle against the FP64
+    one), so two FP32 implementations cannot agree to 1e-4 per cell there.  The per-cell bounds are
+    FP64-anchored: the GPU's largest and 99.9th-percentile errors against the FP64 oracle at most
+    twice the FP32 oracle's own; against the FP32 oracle the mean relative error stays within 1e-4
+    (x_e with an absolute floor of 1e-6); globals within 1e-5.  find_HII_bubbles: the parity bounds
+    of tests/parity.py.  Brightness: 1e-4 relative per cell at the 99.9th percentile and the
+    FP64 mean within 1e-6."""
+    from conftest import realisation
+    from parity import check, compare
+
+    N, box = 256, 200.0
+    print("ts chain N=256: ComputeTs", flush=True)
+    gpu, ref, tab, ref64 = _case(ctxs(N), N, box, 40, 1e-2)
     assert tab.homogeneous == 0 and tab.NO_LIGHT == 0
-    N = 256
     fields = {"TS_box": (lambda r: r["TS_box"], 0.0), "Tk_box": (lambda r: r["Tk_box"], 0.0),
               "x_e": (lambda r: _cells(N, r["x_e_box_prev"]), 1e-6)}
     for k, (get, floor) in fields.items():
-        eg = rel_err(get(gpu), get(ref64), floor).max()
-        eo = rel_err(get(ref), get(ref64), floor).max()
-        print(f"N=256 {k}: vs FP64 max gpu {eg:.2e} fp32-oracle {eo:.2e}", flush=True)
-        assert eg <= 2 * eo + 2e-6, (k, eg, eo)
+        e32 = rel_err(get(gpu), get(ref), floor)
+        eg = rel_err(get(gpu), get(ref64), floor)
+        eo = rel_err(get(ref), get(ref64), floor)
+        pg, po = float(np.percentile(eg, 99.9)), float(np.percentile(eo, 99.9))
+        print(f"N=256 {k}: vs FP32 mean {e32.mean():.2e} p99.9 {np.percentile(e32, 99.9):.2e} max {e32.max():.2e} "
+              f"| vs FP64 gpu p99.9 {pg:.2e} max {eg.max():.2e}, fp32-oracle p99.9 {po:.2e} max {eo.max():.2e}",
+              flush=True)
+        assert eg.max() <= 2 * eo.max() + 2e-6, (k, eg.max(), eo.max())
+        assert pg <= 2 * po + 2e-6, (k, pg, po)
+        assert e32.mean() <= TOL, (k, e32.mean())
+    for k in ("volume_ave_TS", "volume_ave_TK", "volume_ave_xe", "volume_ave_J_alpha", "volume_ave_Xheat"):
+        assert gpu["globals"][k] == pytest.approx(ref["globals"][k], rel=1e-5, abs=1e-30), k
+
+    print("ts chain N=256: find_HII_bubbles with x_e and T_K", flush=True)
+    inp = realisation(N, box, 1)
+    inp["x_e_box"] = gpu["x_e_box"]
+    inp["Tk_box"] = gpu["Tk_box"]
+    p = mxr.hii_params(N, box, filter_type=1, spin_temp=True)
+    sg, sr = mxr.HiiState.host(N), mxr.HiiState.host(N)
+    gg = ctxs(N).find_hii_bubbles(p, **inp, **sg.arrays())
+    rg, margin = ol.find_hii_bubbles(p, inp, sr, margin=True)
+    st = compare(N, {k: np.asarray(v) for k, v in sg.arrays().items()},
+                 {k: np.asarray(v) for k, v in sr.arrays().items()}, gg, rg, margin)
+    print(st, flush=True)
+    check(st)
+    assert 0.05 < rg.volume_weighted_global_xH < 0.95
+
+    print("ts chain N=256: brightness temperature", flush=True)
+    tp = mxr.tb_params(N, 8.0, True, box=box)
+    dT_ref, ave_ref = ol.brightness_temperature(tp, sg.xH, inp["deltax"], gpu["TS_box"])
+    dT = np.zeros(N ** 3, np.float32)
+    ave = ctxs(N).brightness_temperature(tp, sg.xH, inp["deltax"], dT, gpu["TS_box"])
+    e = np.abs(dT.astype(np.float64) - dT_ref) / np.maximum(np.abs(dT_ref), 1e-6)
+    print(f"N=256 delta_T: p99.9 {np.percentile(e, 99.9):.2e} max {e.max():.2e} mean {ave:.6e} vs {ave_ref:.6e}",
+          flush=True)
+    assert np.percentile(e, 99.9) <= 1e-4
+    assert abs(ave - ave_ref) <= 1e-6 * abs(ave_ref)
