@@ -54,26 +54,57 @@ def reset_state(st):
     st.temp_kinetic_all_gas.zero_()
 
 
-def cpu_baseline(N, box, inputs_host, n_radii_sample, n_radii_total):
-    """The oracle's FP32 restatement (own FFT, OpenMP) on a bounded sample: the forward
-    transforms plus the first `n_radii_sample` radii, extrapolated to a whole call."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_baseline(N, box, inputs_host, backend, n_radii_sample, n_radii_total):
+    """The oracle's FP32 restatement of find_HII_bubbles timed on this host.  backend "fftw":
+    one whole call (forward transforms, every radius, the globals) with its transforms on
+    FFTW3's float API as libmkl_rt exports it, threaded over the OpenMP threads; no
+    extrapolation.  backend "own" (or "fftw" without the library): the restatement's own FFT on
+    the forward transforms plus `n_radii_sample` radii, extrapolated to a whole call."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import mxr
     import oracle_lib as O
 
     p = mxr.hii_params(N, box, filter_type=1, zeta=1.0)
     st = mxr.HiiState.host(N)
+    cores = int(O.lib().oracle_num_threads())
+    where = f"{cpu_model()}, {cores} OpenMP threads"
+    if backend == "fftw" and O.fftw_available():
+        g, secs = O.find_hii_bubbles_fftw(p, inputs_host, st)
+        lib = O.lib().oracle_fftw_library().decode()
+        return {"value": N ** 3 * g.n_radii / secs, "unit": "cell-radius updates/s", "cores": cores, "kind": "port",
+                "seconds": round(secs, 2), "xH": g.volume_weighted_global_xH,
+                "sample": (f"one whole {N}^3 call ({g.n_radii} radii, forward transforms to globals) of the FP32 "
+                           f"restatement, transforms on the FFTW3 float API of {lib}: {secs:.1f} s on {where}")}
     n, secs, fwd = O.hii_sample(p, inputs_host, st, n_radii_sample)
     per_radius = (secs - fwd) / max(n, 1)
     t_call = fwd + n_radii_total * per_radius
+    why = "" if backend == "own" else " (no FFTW3-API library found)"
     return {
         "value": N ** 3 * n_radii_total / t_call,
         "unit": "cell-radius updates/s",
-        "cores": int(O.lib().oracle_num_threads()),
+        "cores": cores,
         "kind": "port",
-        "sample": (f"{N}^3 FP32 restatement: forward r2c x3 ({fwd:.1f} s) + {n} of {n_radii_total} radii "
-                   f"({per_radius:.2f} s each), extrapolated to one call ({t_call:.0f} s)"),
+        "sample": (f"{N}^3 FP32 restatement with its own FFT{why}: forward r2c x3 ({fwd:.1f} s) + {n} of "
+                   f"{n_radii_total} radii ({per_radius:.2f} s each), extrapolated to one call ({t_call:.0f} s) on "
+                   f"{where}"),
     }
+
+
+# What the SQ counters of the 512^3 workload say limits a kernel class
+# (profiles/find_hii_bubbles_512_sq_counters.md); the roofline prices bytes only.
+VALU_LIMITER = {
+    "inv_c2r_z_update": {"limited_by": "VALU issue", "source": "profiles/find_hii_bubbles_512_sq_counters.md"},
+}
 
 
 def profiled_traffic(kernel_class, N):
@@ -468,7 +499,10 @@ def main():
     ap.add_argument("--stars-to-matter", type=float, default=0.010438)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--bf16-steps", type=int, default=3, help="timed calls of the BF16-storage variant (0: skip)")
-    ap.add_argument("--cpu-radii", type=int, default=8, help="radii in the CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu", choices=("fftw", "own", "none"), default="fftw",
+                    help="CPU baseline: one whole call on the FFTW3 float API (fftw), a sample on the oracle's own "
+                         "FFT (own), or none")
+    ap.add_argument("--cpu-radii", type=int, default=8, help="radii in the own-FFT CPU sample")
     ap.add_argument("--ts-N", type=int, default=256, help="spin-temperature grid (0: skip that leg)")
     ap.add_argument("--ts-box", type=float, default=200.0)
     ap.add_argument("--ts-steps", type=int, default=5)
@@ -581,14 +615,15 @@ def main():
     ctx.profile(False)
     dom_name, dom = max(ks.items(), key=lambda kv: kv[1]["total_ms"])
     avg_s = dom["total_ms"] / dom["launches"] / 1e3
-    touched_per_launch = dom["alg_bytes"] / dom["launches"]
-    # algorithmic bytes per cell-radius of each radius pass in the out-of-core 3-pass model
-    # (B_cr = 3 grids x 5 x 8 f + 12 B, f = (N/2+1)/N): x and y passes read and write the half
-    # spectra, the z pass reads them and updates the state (xH read + write, z_in read)
-    f = (N // 2 + 1) / N
-    model_per_cell = {"inv_fft_x_filter": 3 * 16 * f, "inv_fft_y": 3 * 16 * f, "inv_c2r_z_update": 3 * 8 * f + 12}
-    bytes_per_launch = model_per_cell.get(dom_name, 0.0) * N ** 3 or touched_per_launch
+    # algorithmic bytes of the dominant kernel class: what its launches must read and write, the
+    # modes the sharp-k window zeroes excluded (counted by the library from the kept-mode
+    # geometry of each radius), averaged over its launches
+    bytes_per_launch = dom["alg_bytes"] / dom["launches"]
     achieved = bytes_per_launch / avg_s
+    call_ms = sum(v["total_ms"] for v in ks.values())
+    call_bytes = sum(v["alg_bytes"] for v in ks.values())
+    f = (N // 2 + 1) / N
+    dense_z = (3 * 8 * f + 12) * N ** 3
     traffic = profiled_traffic(dom_name, N)
     kernels = {k: dict(launches=v["launches"], total_ms=round(v["total_ms"], 3),
                        GBps=round(v["alg_bytes"] / max(v["total_ms"], 1e-9) / 1e6, 1)) for k, v in ks.items()}
@@ -612,12 +647,12 @@ def main():
 
     # ---- CPU baseline (oracle restatement on this host, bounded sample)
     cpu = None
-    if args.cpu_radii > 0:
+    if args.cpu != "none":
         host = {k: v.cpu().numpy() for k, v in inputs.items()}
         del inputs
         torch.cuda.empty_cache()
-        log("bench: CPU baseline sample")
-        cpu = cpu_baseline(N, box, host, args.cpu_radii, n_radii)
+        log("bench: CPU baseline")
+        cpu = cpu_baseline(N, box, host, args.cpu, args.cpu_radii, n_radii)
         log("bench: cpu " + json.dumps(cpu))
 
     ts = ts_leg(args, torch) if args.ts_N > 0 else None
@@ -645,11 +680,17 @@ def main():
                    "grid_dim": N, "radii": n_radii, "cells": N ** 3},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s * 1e3,
-                     "bytes_model": "B_cr share of the pass x N^3 (z pass: 3 x 8 f + 12 B per cell-radius)",
-                     "touched_bytes_per_launch": touched_per_launch,
-                     "touched_GBps": touched_per_launch / avg_s / 1e9,
-                     "call_frac": (N ** 3 * n_radii / (ms_step / 1e3)) * (3 * 5 * 8 * f + 12) / HBM_PEAK},
+                     "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s * 1e3, "launches": dom["launches"],
+                     "bytes_model": "kept modes only: per launch the spectrum values the sharp-k window keeps "
+                                    "(8 B each, read once per grid) plus 1 B of flags per cell and, on the last "
+                                    "radius, the state grids; counted by the library per radius",
+                     "limiter": VALU_LIMITER.get(dom_name),
+                     "call_alg_bytes": call_bytes, "call_kernel_ms": call_ms,
+                     "call_frac": call_bytes / (call_ms / 1e3) / HBM_PEAK,
+                     "dense_model_bytes_per_launch": dense_z,
+                     "dense_model_note": "the B_cr share of the z pass (3 x 8 f + 12 B per cell) counts every "
+                                         "mode; the sharp-k kernel skips the zeroed ones, so it is not what the "
+                                         "kernel moves"},
         "kernels": kernels,
         "plain": plain,
         "bf16_storage": bf16,

@@ -357,6 +357,120 @@ int oracle_filter_f64(double* box, int local_ix_start, int slab_nx, int grid_dim
 #undef FMAXR
 #undef FMINR
 
+/* ------------------------------------------- the same pipeline on FFTW3's float API
+ * The reference executes fftwf plans (src/core/reionization.c:907-997; find_HII_bubbles.c:
+ * 166-229, 351-370).  No FFTW library is installed here, but MKL's libmkl_rt exports FFTW3's
+ * float interface, so the CPU baseline runs the FP32 restatement above on that backend: the
+ * forward transform out of place from the padded real grid (advanced interface, inembed
+ * {N, N, 2(N/2+1)}), the inverse in place, as the reference plans them.  Symbols are resolved
+ * at run time; without the library the entry point returns MXR_ERR_UNSUPPORTED. */
+#include <dlfcn.h>
+#include <omp.h>
+
+#define FFTWF_ESTIMATE (1U << 6)
+
+static struct {
+  int tried, ok, N;
+  void* (*plan_many_dft_r2c)(int, const int*, int, float*, const int*, int, int, void*, const int*, int, int,
+                             unsigned);
+  void* (*plan_dft_c2r_3d)(int, int, int, void*, float*, unsigned);
+  void (*execute_dft_r2c)(void*, float*, void*);
+  void (*execute_dft_c2r)(void*, void*, float*);
+  void (*destroy_plan)(void*);
+  int (*init_threads)(void);
+  void (*plan_with_nthreads)(int);
+  void* r2c;
+  void* c2r;
+} fw;
+
+static const char* fftw_lib_path(void)
+{
+  const char* e = getenv("ORACLE_FFTW_LIB");
+  return e ? e : "/opt/conda/lib/libmkl_rt.so.1";
+}
+
+static int fftw_load(void)
+{
+  if (fw.tried)
+    return fw.ok;
+  fw.tried = 1;
+  /* MKL's GNU threading layer shares libgomp with the rest of the oracle */
+  setenv("MKL_THREADING_LAYER", "GNU", 0);
+  void* h = dlopen(fftw_lib_path(), RTLD_NOW | RTLD_LOCAL);
+  if (!h)
+    return 0;
+  *(void**)&fw.plan_many_dft_r2c = dlsym(h, "fftwf_plan_many_dft_r2c");
+  *(void**)&fw.plan_dft_c2r_3d = dlsym(h, "fftwf_plan_dft_c2r_3d");
+  *(void**)&fw.execute_dft_r2c = dlsym(h, "fftwf_execute_dft_r2c");
+  *(void**)&fw.execute_dft_c2r = dlsym(h, "fftwf_execute_dft_c2r");
+  *(void**)&fw.destroy_plan = dlsym(h, "fftwf_destroy_plan");
+  *(void**)&fw.init_threads = dlsym(h, "fftwf_init_threads");
+  *(void**)&fw.plan_with_nthreads = dlsym(h, "fftwf_plan_with_nthreads");
+  fw.ok = fw.plan_many_dft_r2c && fw.plan_dft_c2r_3d && fw.execute_dft_r2c && fw.execute_dft_c2r &&
+          fw.destroy_plan && fw.init_threads && fw.plan_with_nthreads;
+  if (fw.ok) {
+    fw.init_threads();
+    fw.plan_with_nthreads(omp_get_max_threads());
+  }
+  return fw.ok;
+}
+
+/* plans for dimension N, made on the first transform with these arrays' alignment (malloc'd
+ * grids all share it) and executed on new arrays afterwards */
+static int fftw_plans(int N, float* rin, void* cout, void* inplace)
+{
+  if (fw.N == N && fw.r2c && fw.c2r)
+    return 0;
+  if (fw.r2c)
+    fw.destroy_plan(fw.r2c);
+  if (fw.c2r)
+    fw.destroy_plan(fw.c2r);
+  fw.r2c = fw.c2r = NULL;
+  const int H = N / 2 + 1;
+  const int n[3] = {N, N, N}, inembed[3] = {N, N, 2 * H}, onembed[3] = {N, N, H};
+  if (rin && cout)
+    fw.r2c = fw.plan_many_dft_r2c(3, n, 1, rin, inembed, 1, 0, cout, onembed, 1, 0, FFTWF_ESTIMATE);
+  if (inplace)
+    fw.c2r = fw.plan_dft_c2r_3d(N, N, N, inplace, (float*)inplace, FFTWF_ESTIMATE);
+  fw.N = N;
+  return 0;
+}
+
+typedef cplx_f32 cplx_fftw;
+#define filter_fftw filter_f32
+
+static int r2c3d_fftw(int N, const float* in, cplx_fftw* out)
+{
+  if (fw.N != N || !fw.r2c) {
+    fw.N = 0;
+    fftw_plans(N, (float*)in, out, out);
+  }
+  fw.execute_dft_r2c(fw.r2c, (float*)in, out);
+  return 0;
+}
+
+static int c2r3d_fftw(int N, cplx_fftw* data)
+{
+  if (fw.N != N || !fw.c2r) {
+    fw.N = 0;
+    fftw_plans(N, NULL, NULL, data);
+  }
+  fw.execute_dft_c2r(fw.c2r, data, (float*)data);
+  return 0;
+}
+
+#define REAL float
+#define SUFFIX fftw
+#define FILTER_R_T float
+#define FMAXR fmaxf
+#define FMINR fminf
+#include "hii_tmpl.h"
+#undef REAL
+#undef SUFFIX
+#undef FILTER_R_T
+#undef FMAXR
+#undef FMINR
+
 static int check_args(const mxr_hii_params* p, const mxr_hii_grids* g)
 {
   if (!p || !g || !g->deltax || !g->stars || !g->weighted_sfr || !g->xH || !g->z_at_ionization ||
@@ -388,6 +502,21 @@ int oracle_find_hii_bubbles_f64(const mxr_hii_params* p, const mxr_hii_grids* g,
   if (rc)
     return rc;
   rc = hii_run_f64(p, g, out, margin, 0, NULL, NULL);
+  return rc < 0 ? rc : 0;
+}
+
+int oracle_fftw_available(void) { return fftw_load(); }
+const char* oracle_fftw_library(void) { return fftw_lib_path(); }
+
+int oracle_find_hii_bubbles_fftw(const mxr_hii_params* p, const mxr_hii_grids* g, mxr_hii_globals* out,
+                                 double* seconds)
+{
+  int rc = check_args(p, g);
+  if (rc)
+    return rc;
+  if (!fftw_load())
+    return MXR_ERR_UNSUPPORTED;
+  rc = hii_run_fftw(p, g, out, NULL, 0, seconds, NULL);
   return rc < 0 ? rc : 0;
 }
 

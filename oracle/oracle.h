@@ -73,6 +73,15 @@ int oracle_find_hii_bubbles_f64(const mxr_hii_params* p, const mxr_hii_grids* g,
 int oracle_hii_sample_f32(const mxr_hii_params* p, const mxr_hii_grids* g, int n_radii, double* seconds,
                           double* seconds_fwd);
 
+/* The f32 restatement with its transforms on FFTW3's float API (libmkl_rt's export, loaded at
+ * run time; ORACLE_FFTW_LIB names another library): the CPU baseline of the benchmark.
+ * *seconds gets the call's wall time (forward transforms to globals).  MXR_ERR_UNSUPPORTED when
+ * the library is missing. */
+int oracle_find_hii_bubbles_fftw(const mxr_hii_params* p, const mxr_hii_grids* g, mxr_hii_globals* out,
+                                 double* seconds);
+int oracle_fftw_available(void);
+const char* oracle_fftw_library(void);
+
 int oracle_brightness_temperature(const mxr_tb_params* p, const mxr_tb_grids* g, double* volume_ave_Tb);
 int oracle_velocity_gradient(float* box, int slab_nx, int grid_dim, double BoxSize);
 /* Input grids: smooth_grid + subsample_grid + the density step (one rank); out is padded N^3. */

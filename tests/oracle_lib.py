@@ -61,6 +61,10 @@ def lib():
     for name in ("oracle_find_hii_bubbles_f32", "oracle_find_hii_bubbles_f64"):
         getattr(L, name).argtypes = [ctypes.POINTER(mxr.HiiParams), ctypes.POINTER(mxr.HiiGrids),
                                      ctypes.POINTER(mxr.HiiGlobals), fp]
+    L.oracle_find_hii_bubbles_fftw.argtypes = [ctypes.POINTER(mxr.HiiParams), ctypes.POINTER(mxr.HiiGrids),
+                                               ctypes.POINTER(mxr.HiiGlobals), ctypes.POINTER(ctypes.c_double)]
+    L.oracle_fftw_available.restype = ctypes.c_int
+    L.oracle_fftw_library.restype = ctypes.c_char_p
     L.oracle_hii_sample_f32.argtypes = [ctypes.POINTER(mxr.HiiParams), ctypes.POINTER(mxr.HiiGrids), ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.oracle_brightness_temperature.argtypes = [ctypes.POINTER(mxr.TbParams), ctypes.POINTER(mxr.TbGrids),
@@ -152,6 +156,25 @@ def find_hii_bubbles(params, inputs: dict, state: "mxr.HiiState", precision="f32
     if rc:
         raise mxr.MxrError(rc, "oracle_find_hii_bubbles")
     return (out, m) if margin else out
+
+
+def fftw_available() -> bool:
+    return bool(lib().oracle_fftw_available())
+
+
+def find_hii_bubbles_fftw(params, inputs: dict, state: "mxr.HiiState"):
+    """The FP32 restatement with its transforms on FFTW3's float API (libmkl_rt); returns
+    (globals, seconds of the whole call)."""
+    arrays = dict(inputs)
+    arrays.update(state.arrays())
+    g = mxr.hii_grids(**arrays)
+    out = mxr.HiiGlobals()
+    secs = ctypes.c_double()
+    rc = lib().oracle_find_hii_bubbles_fftw(ctypes.byref(params), ctypes.byref(g), ctypes.byref(out),
+                                            ctypes.byref(secs))
+    if rc:
+        raise mxr.MxrError(rc, "oracle_find_hii_bubbles_fftw")
+    return out, secs.value
 
 
 def hii_sample(params, inputs: dict, state, n_radii: int):
