@@ -52,7 +52,7 @@ extern "C" {
 /* ---- parameters (names follow run_params_t / physics_params_t / run_units_t,
  *      src/meraxes.h:142-201, 262-398) ------------------------------------------------- */
 typedef struct mxr_hii_params {
-  int ReionGridDim;             /* N; N even, N = 2^a 3^b 5^c, 16 <= N <= 1024           */
+  int ReionGridDim;             /* N: one of the dimensions mxr_supported_dims() lists   */
   double BoxSize;               /* Mpc/h                                                  */
   double Hubble_h;
   double OmegaM;
@@ -140,6 +140,13 @@ typedef struct mxr_ctx mxr_ctx;
  * Replaces init_CUDA (src/meraxes_gpu.h:55, src/core/meraxes_gpu.cu:127-194) plus the
  * grid/plan set-up of malloc_reionization_grids (src/core/reionization.c:718-1000).      */
 int mxr_create(mxr_ctx** ctx, int device, int grid_dim);
+/* The grid dimensions this build compiles kernels for, in increasing order:
+ *   16, 32, 64, 96, 128, 160, 256, 512
+ * (the reference plans FFTW for any ReionGridDim, src/core/reionization.c:907-997; here each
+ * transform length is a compile-time radix plan).  Writes up to `capacity` of them into dims
+ * (may be NULL) and returns how many there are.  mxr_create rejects any other dimension with
+ * MXR_ERR_INVALID_ARG.  Needs no GPU.                                                      */
+int mxr_supported_dims(int* dims, int capacity);
 void mxr_destroy(mxr_ctx* ctx);
 const char* mxr_strerror(int code);
 const char* mxr_last_error(void);

@@ -238,6 +238,18 @@ const char* mxr_strerror(int code)
 
 const char* mxr_last_error(void) { return g_err; }
 
+int mxr_supported_dims(int* dims, int capacity)
+{
+  int n = 0;
+#define X(d)                                                                                                         \
+  if (dims && n < capacity)                                                                                          \
+    dims[n] = d;                                                                                                     \
+  n++;
+  MXR_DIMS(X)
+#undef X
+  return n;
+}
+
 int mxr_create(mxr_ctx** out, int device, int grid_dim)
 {
   if (!out) {

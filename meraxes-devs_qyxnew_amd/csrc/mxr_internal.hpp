@@ -160,6 +160,8 @@ float cT_approx(float z);
 int check_hii_params(const mxr_hii_params* p, int N);
 
 // grid-dimension dispatch (mxr_hii.hip)
+// grid dimensions the library instantiates (include/mxr.h lists them at mxr_supported_dims)
+#define MXR_DIMS(X) X(16) X(32) X(64) X(96) X(128) X(160) X(256) X(512)
 bool supported_dim(int N);
 int run_find_hii_bubbles(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_hii_globals* out);
 int run_filter(Ctx* c, float2* box, int local_ix_start, int slab_nx, int grid_dim, float dk, float R, int ftype);

@@ -582,8 +582,6 @@ static inline int kept_kz(const FiltArgs& fa)
   return kc;
 }
 
-// grid dimensions the library instantiates
-#define MXR_DIMS(X) X(16) X(32) X(64) X(96) X(128) X(160) X(256) X(512)
 
 // launchers of the shared passes for one grid dimension
 template <int N>

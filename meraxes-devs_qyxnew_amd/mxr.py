@@ -553,6 +553,8 @@ def lib():
     L.mxr_brightness_temperature.argtypes = [vp, ctypes.POINTER(TbParams), ctypes.POINTER(TbGrids),
                                              ctypes.POINTER(ctypes.c_double)]
     L.mxr_brightness_temperature.restype = ctypes.c_int
+    L.mxr_supported_dims.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    L.mxr_supported_dims.restype = ctypes.c_int
     L.mxr_ps_length.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.mxr_ps_length.restype = ctypes.c_int
     fpp = ctypes.POINTER(ctypes.c_float)
@@ -892,6 +894,14 @@ def _with_dummy_state(arrays: dict) -> dict:
     for k in ("xH", "z_at_ionization", "J_21_at_ionization", "r_bubble", "temp_kinetic_all_gas"):
         a.setdefault(k, ref)
     return a
+
+
+def supported_dims() -> list:
+    """mxr_supported_dims: the grid dimensions this build compiles."""
+    n = lib().mxr_supported_dims(None, 0)
+    out = (ctypes.c_int * n)()
+    lib().mxr_supported_dims(out, n)
+    return list(out)
 
 
 def ps_length(grid_dim: int, box_size: float, hubble_h: float) -> int:

@@ -77,3 +77,14 @@ def test_null_arguments_are_errors():
     assert L.mxr_ps_counts(None, 400.0, 0.678, 21, None) == mxr.MXR_ERR_INVALID_ARG
     assert L.mxr_ps_length(0, 400.0, 0.678) == mxr.MXR_ERR_INVALID_ARG
     assert L.mxr_ps_length(512, -1.0, 0.678) == mxr.MXR_ERR_INVALID_ARG
+
+
+def test_supported_dims_match_header():
+    """The dimension list include/mxr.h states at mxr_supported_dims is the build's list."""
+    import re
+
+    hdr = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "mxr.h")).read()
+    block = hdr[hdr.index("The grid dimensions this build compiles"):hdr.index("int mxr_supported_dims")]
+    listed = [int(x) for x in re.search(r"\*\s+([0-9, ]+)\n", block).group(1).replace(" ", "").split(",") if x]
+    assert mxr.supported_dims() == listed
+    assert listed == sorted(listed)
