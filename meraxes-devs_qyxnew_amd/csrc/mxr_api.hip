@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <new>
 #include <string>
@@ -701,6 +702,48 @@ void mxr_batch_destroy(mxr_batch* b)
   delete b;
 }
 
+// The bytes a captured realisation depends on, field by field (no struct padding, whose
+// contents a caller's stack leaves unspecified): every parameter, the tables directory's text
+// (it sets T_RECFAST when TK_recfast < 0), every grid pointer and the launch shapes.
+extern "C++" {
+template <class T>
+static void key_put(std::vector<unsigned char>& k, const T& v)
+{
+  const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+  k.insert(k.end(), p, p + sizeof(T));
+}
+
+static void graph_key_append(std::vector<unsigned char>& k, const mxr_hii_params& p, const mxr_hii_grids& g,
+                             const mxr_tuning& t)
+{
+  key_put(k, p.ReionGridDim);
+  for (double v : {p.BoxSize, p.Hubble_h, p.OmegaM, p.OmegaLambda, p.OmegaK, p.BaryonFrac, p.ReionDeltaRFactor})
+    key_put(k, v);
+  for (int v : {p.ReionFilterType, p.ReionRtoMFilterType, p.ReionUVBFlag, p.Flag_IncludeSpinTemp,
+                p.Flag_IncludeRecombinations, p.Flag_EvolvingReionRBubbleMax, p.Flag_BHFeedback})
+    key_put(k, v);
+  for (double v : {p.ReionSfrTimescale, p.ReionEfficiency, p.ReionNionPhotPerBary, p.ReionGammaHaloBias,
+                   p.ReionAlphaUV, p.ReionAlphaUVBH, p.ReionRBubbleMax, p.ReionRBubbleMaxRecomb, p.ReionRBubbleMin,
+                   p.UnitLength_in_cm, p.UnitMass_in_g, p.UnitVelocity_in_cm_per_s, p.redshift, p.prev_redshift,
+                   p.TK_recfast})
+    key_put(k, v);
+  const char* dir = p.TablesForXHeatingDir ? p.TablesForXHeatingDir : "";
+  const size_t len = std::strlen(dir);
+  key_put(k, len);
+  k.insert(k.end(), dir, dir + len);
+  key_put(k, g.mem);
+  key_put(k, g.storage);
+  for (const void* q : {(const void*)g.deltax, (const void*)g.stars, (const void*)g.weighted_sfr,
+                        (const void*)g.x_e_box, (const void*)g.Tk_box, (const void*)g.xH,
+                        (const void*)g.z_at_ionization, (const void*)g.J_21_at_ionization, (const void*)g.r_bubble,
+                        (const void*)g.temp_kinetic_all_gas, (const void*)g.effective_bhm,
+                        (const void*)g.effective_bhar})
+    key_put(k, q);
+  for (int v : {t.zdirect_kmax, t.zdirect_rpb, t.live_tiles, t.col_tc_x, t.col_tc_y})
+    key_put(k, v);
+}
+}  // extern "C++"
+
 int mxr_find_hii_bubbles_batched(mxr_batch* b, const mxr_hii_params* params, const mxr_hii_grids* grids,
                                  mxr_hii_globals* globals, int mode)
 {
@@ -771,14 +814,11 @@ int mxr_find_hii_bubbles_batched(mxr_batch* b, const mxr_hii_params* params, con
     MXR_HIP(hipEventRecord(b->t1, b->cap));
   } else {
     // the captured kernels depend on the parameters, the grids and each context's launch shapes
-    std::vector<unsigned char> key((sizeof(mxr_hii_params) + sizeof(mxr_hii_grids) + sizeof(mxr_tuning)) * n);
-    std::memcpy(key.data(), params, sizeof(mxr_hii_params) * n);
-    std::memcpy(key.data() + sizeof(mxr_hii_params) * n, grids, sizeof(mxr_hii_grids) * n);
+    std::vector<unsigned char> key;
     for (int i = 0; i < n; i++) {
       mxr_tuning t;
       tuning_get(b->ctx[i]->c, &t);
-      std::memcpy(key.data() + (sizeof(mxr_hii_params) + sizeof(mxr_hii_grids)) * n + sizeof(mxr_tuning) * i, &t,
-                  sizeof t);
+      graph_key_append(key, params[i], grids[i], t);
     }
     if (!b->exec || key != b->key) {
       batch_drop_graph(b);

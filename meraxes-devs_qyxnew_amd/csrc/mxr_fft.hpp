@@ -15,20 +15,36 @@
 
 namespace mxr {
 
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// The transform's rounding is fixed by the source: every building block below turns FP
+// contraction off and writes its fused multiply-adds explicitly, so the compiler cannot fuse a
+// product with a neighbouring sum differently in different kernels (with contraction left to
+// the compiler, the same butterfly compiled into two kernels with different epilogues took
+// different FMA choices and rounded differently).
+__device__ __forceinline__ float2 cadd(float2 a, float2 b)
+{
+#pragma clang fp contract(off)
+  return make_float2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ float2 csub(float2 a, float2 b)
+{
+#pragma clang fp contract(off)
+  return make_float2(a.x - b.x, a.y - b.y);
+}
+// a b = (a.x b.x - a.y b.y, a.x b.y + a.y b.x), each part one product and one FMA
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+#pragma clang fp contract(off)
+  return make_float2(__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.x, b.y, a.y * b.x));
 }
 // a * (S == -1 ? w : conj(w))
 template <int S>
 __device__ __forceinline__ float2 ctw(float2 a, float2 w)
 {
+#pragma clang fp contract(off)
   if constexpr (S < 0)
-    return make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+    return make_float2(__builtin_fmaf(a.x, w.x, -(a.y * w.y)), __builtin_fmaf(a.x, w.y, a.y * w.x));
   else
-    return make_float2(a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y);
+    return make_float2(__builtin_fmaf(a.x, w.x, a.y * w.y), __builtin_fmaf(a.y, w.x, -(a.x * w.y)));
 }
 // a * (S i)
 template <int S>
@@ -44,6 +60,7 @@ __device__ __forceinline__ float2 cmul_si(float2 a)
 template <int S>
 __device__ __forceinline__ void dft2(float2* v)
 {
+#pragma clang fp contract(off)
   float2 a = v[0], b = v[1];
   v[0] = cadd(a, b);
   v[1] = csub(a, b);
@@ -52,6 +69,7 @@ __device__ __forceinline__ void dft2(float2* v)
 template <int S>
 __device__ __forceinline__ void dft4(float2* v)
 {
+#pragma clang fp contract(off)
   float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
   float2 t2 = cadd(v[1], v[3]), t3 = cmul_si<S>(csub(v[1], v[3]));
   v[0] = cadd(t0, t2);
@@ -64,6 +82,7 @@ template <int S>
 __device__ __forceinline__ void dft8(float2* v)
 {
   // even/odd split: V[2p] = DFT4(v_r + v_{r+4}), V[2p+1] = DFT4((v_r - v_{r+4}) w8^r)
+#pragma clang fp contract(off)
   constexpr float h = 0.70710678118654752440f;
   float2 b[4], c[4];
 #pragma unroll
@@ -87,10 +106,11 @@ __device__ __forceinline__ void dft8(float2* v)
 template <int S>
 __device__ __forceinline__ void dft3(float2* v)
 {
+#pragma clang fp contract(off)
   constexpr float c3 = -0.5f, s3 = 0.86602540378443864676f;
   float2 t = cadd(v[1], v[2]);
   float2 d = csub(v[1], v[2]);
-  float2 m = make_float2(v[0].x + c3 * t.x, v[0].y + c3 * t.y);
+  float2 m = make_float2(__builtin_fmaf(c3, t.x, v[0].x), __builtin_fmaf(c3, t.y, v[0].y));
   float2 r = cmul_si<S>(make_float2(s3 * d.x, s3 * d.y));
   v[0] = cadd(v[0], t);
   v[1] = cadd(m, r);
@@ -100,14 +120,17 @@ __device__ __forceinline__ void dft3(float2* v)
 template <int S>
 __device__ __forceinline__ void dft5(float2* v)
 {
+#pragma clang fp contract(off)
   constexpr float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
   constexpr float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
   float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
   float2 d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
-  float2 a1 = make_float2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
-  float2 a2 = make_float2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
-  float2 b1 = cmul_si<S>(make_float2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y));
-  float2 b2 = cmul_si<S>(make_float2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y));
+  float2 a1 = make_float2(__builtin_fmaf(c2, t2.x, __builtin_fmaf(c1, t1.x, v[0].x)),
+                          __builtin_fmaf(c2, t2.y, __builtin_fmaf(c1, t1.y, v[0].y)));
+  float2 a2 = make_float2(__builtin_fmaf(c1, t2.x, __builtin_fmaf(c2, t1.x, v[0].x)),
+                          __builtin_fmaf(c1, t2.y, __builtin_fmaf(c2, t1.y, v[0].y)));
+  float2 b1 = cmul_si<S>(make_float2(__builtin_fmaf(s2, d2.x, s1 * d1.x), __builtin_fmaf(s2, d2.y, s1 * d1.y)));
+  float2 b2 = cmul_si<S>(make_float2(__builtin_fmaf(-s1, d2.x, s2 * d1.x), __builtin_fmaf(-s1, d2.y, s2 * d1.y)));
   v[0] = cadd(v[0], cadd(t1, t2));
   v[1] = cadd(a1, b1);
   v[4] = csub(a1, b1);

@@ -21,6 +21,7 @@
 //   per radius copy, filter, 3-pass c2r writing the real field, clamp, criterion kernel.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "mxr_passes.hpp"

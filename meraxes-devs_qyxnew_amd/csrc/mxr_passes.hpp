@@ -71,7 +71,8 @@ struct FiltArgs {
   const float* wtab;
 };
 
-// COL_SCALE divides by `div` in double (find_HII_bubbles.c:235-253), COL_FSCALE by (float)div
+// COL_SCALE divides by `div` in double (find_HII_bubbles.c:235-253; as an exact reciprocal
+// product when div is a power of two), COL_FSCALE by (float)div
 // in float (ComputeTs.c:485-487)
 enum { COL_PLAIN = 0, COL_SCALE = 1, COL_FILTER = 2, COL_YPRUNE = 3, COL_FSCALE = 4 };
 
@@ -272,6 +273,14 @@ __global__ void __launch_bounds__(NT) k_col(CGridPtrs in, GridPtrs out, ColGeom 
   }
 
   float2* __restrict__ dst = pick(out, g);
+  // COL_SCALE with a power-of-two divisor (N^3 for N = 2^a): the product with its reciprocal,
+  // which is exact, equals the double division bit for bit (no product of a float's value and
+  // 2^-k leaves the normal double range), at the cost of one multiply
+  double rdiv = 0.0;
+  if constexpr (MODE == COL_SCALE) {
+    if ((__double_as_longlong(div) & 0x000FFFFFFFFFFFFFLL) == 0)
+      rdiv = 1.0 / div;
+  }
 #pragma unroll
   for (int i = 0; i < TPB; i++) {
     // block-uniform: a tile whose columns are all zero for this radius is skipped; the barrier
@@ -289,8 +298,12 @@ __global__ void __launch_bounds__(NT) k_col(CGridPtrs in, GridPtrs out, ColGeom 
         for (int r = 0; r < SL::RL; r++) {
           const int e = j + r * SL::BL;
           float2 y = v[i][q * SL::RL + r];
-          if constexpr (MODE == COL_SCALE)
-            y = make_float2((float)((double)y.x / div), (float)((double)y.y / div));
+          if constexpr (MODE == COL_SCALE) {
+            if (rdiv != 0.0)
+              y = make_float2((float)((double)y.x * rdiv), (float)((double)y.y * rdiv));
+            else
+              y = make_float2((float)((double)y.x / div), (float)((double)y.y / div));
+          }
           if constexpr (MODE == COL_FSCALE)
             y = make_float2(__fdiv_rn(y.x, (float)div), __fdiv_rn(y.y, (float)div));
           dst[base[i] + (long)e * geo.es] = y;
@@ -298,6 +311,18 @@ __global__ void __launch_bounds__(NT) k_col(CGridPtrs in, GridPtrs out, ColGeom 
       }
     }
   }
+}
+
+// r2c post-twiddle of bin k from Z[k] and conj Z[M-k] (zm), t = exp(-2 pi i k / N):
+// X[k] = e + t o, e = (Z[k] + zm) / 2, o = (Z[k] - zm) / (2i); two FMAs per part, contraction off
+__device__ __forceinline__ float2 post_twiddle(float2 zk, float2 zm, float2 t)
+{
+#pragma clang fp contract(off)
+  const float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
+  const float2 d = make_float2(0.5f * (zk.x - zm.x), 0.5f * (zk.y - zm.y));
+  const float2 o = make_float2(d.y, -d.x);  // d / i
+  return make_float2(__builtin_fmaf(t.x, o.x, __builtin_fmaf(-t.y, o.y, e.x)),
+                     __builtin_fmaf(t.x, o.y, __builtin_fmaf(t.y, o.x, e.y)));
 }
 
 // ---- z-pass forward: padded real rows -> half-spectrum rows ------------------------------
@@ -380,10 +405,7 @@ __global__ void __launch_bounds__(NT) k_row_r2c(CRealPtrs in, GridPtrs out, long
     const float2 zk = lds[LI::at(c, k % M)];
     const float2 zm0 = lds[LI::at(c, (M - k) % M)];
     const float2 zm = make_float2(zm0.x, -zm0.y);
-    const float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
-    const float2 d = make_float2(0.5f * (zk.x - zm.x), 0.5f * (zk.y - zm.y));
-    const float2 o = make_float2(d.y, -d.x);  // d / i
-    dst[k] = cadd(e, cmul(tw[k], o));
+    dst[k] = post_twiddle(zk, zm, tw[k]);
   }
 }
 
@@ -415,6 +437,15 @@ __device__ __forceinline__ void block_sum(double* acc, double* out)
       out[q] = s;
     }
   }
+}
+
+// c2r pre-twiddle: z = s + i w d with w = conj(t) = exp(+2 pi i n / N), i.e.
+// z = (s.x - (t.x d.y - t.y d.x), s.y + (t.x d.x + t.y d.y)); two FMAs per part, contraction off
+__device__ __forceinline__ float2 pre_twiddle(float2 s, float2 d, float2 t)
+{
+#pragma clang fp contract(off)
+  return make_float2(__builtin_fmaf(t.y, d.x, __builtin_fmaf(-t.x, d.y, s.x)),
+                     __builtin_fmaf(t.y, d.y, __builtin_fmaf(t.x, d.x, s.y)));
 }
 
 // One grid's rows through the inverse z-pass: load the half-spectrum rows into LDS (only the
@@ -472,9 +503,7 @@ __device__ __forceinline__ void row_c2r_grid(const float2* __restrict__ base, bo
         const float2 xm = make_float2(xm0.x, -xm0.y);
         const float2 s = cadd(xn, xm);
         const float2 d = csub(xn, xm);
-        const float2 w = make_float2(tw[n].x, -tw[n].y);  // exp(+2 pi i n / N)
-        const float2 u = cmul(w, d);
-        z = make_float2(s.x - u.y, s.y + u.x);  // s + i u
+        z = pre_twiddle(s, d, tw[n]);
       }
       v[q * SL::R0 + r] = z;
     }
