@@ -289,6 +289,19 @@ typedef struct mxr_tb_grids {
 int mxr_brightness_temperature(mxr_ctx* ctx, const mxr_tb_params* params, const mxr_tb_grids* grids,
                                double* volume_ave_Tb);
 
+/* The same brightness box for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  No context: grids->mem must be MXR_MEM_DEVICE, volume_ave_Tb is a device pointer
+ * that receives the FP64 mean, and the work is enqueued on `stream` (a hipStream_t; NULL for the
+ * null stream) without synchronising.  Flag_IncludePecVelsFor21cm must be 0 when
+ * Flag_IncludeSpinTemp is set (the velocity terms need a context's FFT workspace;
+ * MXR_ERR_UNSUPPORTED); ReionGridDim even and at most 2048.  It calls no allocating routine: its
+ * scratch (8192 FP64 partial sums and the total, 64 KiB) is a fixed array in device memory, one
+ * per GPU, held until the process ends and shared by every call on that GPU, so calls that
+ * overlap on one GPU race.  Results equal mxr_brightness_temperature's bit for bit.          */
+int mxr_brightness_temperature_noalloc(const mxr_tb_params* params, const mxr_tb_grids* grids,
+                                       double* volume_ave_Tb, void* stream);
+
 /* ---- 21-cm power spectrum -------------------------------------------------------------------- */
 /* Number of k bins: ceilings delta_k 1.35^i below delta_k ReionGridDim, delta_k = (float)(2 pi
  * / (BoxSize / Hubble_h)).  Replaces Initialise_PowerSpectrum's PS_Length

@@ -1026,6 +1026,16 @@ int mxr_brightness_temperature(mxr_ctx* h, const mxr_tb_params* p, const mxr_tb_
   return run_brightness_temperature(&h->c, p, g, ave);
 }
 
+int mxr_brightness_temperature_noalloc(const mxr_tb_params* params, const mxr_tb_grids* grids,
+                                       double* volume_ave_Tb, void* stream)
+{
+  if (!params || !grids) {
+    set_error("null params or grids");
+    return MXR_ERR_INVALID_ARG;
+  }
+  return run_brightness_temperature_noalloc(params, grids, volume_ave_Tb, static_cast<hipStream_t>(stream));
+}
+
 int mxr_ps_length(int ReionGridDim, double BoxSize, double Hubble_h)
 {
   if (ReionGridDim <= 0 || !(BoxSize > 0) || !(Hubble_h > 0)) {

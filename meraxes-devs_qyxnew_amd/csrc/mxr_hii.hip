@@ -760,6 +760,94 @@ __global__ void __launch_bounds__(NT) k_tb(TbArgs a)
   block_sum<NT, 1>(acc, a.partials + blockIdx.x);
 }
 
+// ---- the brightness box without allocation ---------------------------------------------
+// Scratch of mxr_brightness_temperature_noalloc: the per-block FP64 partials of k_tb (at most
+// 8192 blocks, ew_blocks' cap) and the final sum.  Each GPU holds its own copy for the life of
+// the process; calls on one GPU share it, so calls that overlap on one GPU race.
+constexpr int TB_SCRATCH_BLOCKS = 8192;
+static __device__ __align__(256) double scratch[TB_SCRATCH_BLOCKS + 8];
+
+__global__ void k_mean_out(const double* __restrict__ sum, double total, double* __restrict__ out)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    out[0] = sum[0] / total;
+}
+
+// the device address of `scratch` on the current device, looked up once per device
+static int scratch_ptr(double** out)
+{
+  constexpr int MAXDEV = 64;
+  static double* addr[MAXDEV] = {};
+  static std::mutex mu;
+  int dev = 0;
+  MXR_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAXDEV) {
+    set_error("device %d beyond the %d this library tracks", dev, MAXDEV);
+    return MXR_ERR_INVALID_ARG;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    MXR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    addr[dev] = static_cast<double*>(p);
+  }
+  *out = addr[dev];
+  return MXR_OK;
+}
+
+int run_brightness_temperature_noalloc(const mxr_tb_params* p, const mxr_tb_grids* g, double* ave_dev,
+                                       hipStream_t stream)
+{
+  const int N = p->ReionGridDim;
+  if (N < 2 || N % 2 || N > 2048) {
+    set_error("ReionGridDim %d: must be even and at most 2048", N);
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (g->mem != MXR_MEM_DEVICE || !g->xH || !g->deltax || !g->delta_T || !ave_dev ||
+      (p->Flag_IncludeSpinTemp && !g->TS_box)) {
+    set_error("the no-allocation brightness box takes device pointers: xH, deltax, delta_T, the mean%s",
+              p->Flag_IncludeSpinTemp ? ", TS_box" : "");
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (p->Flag_IncludeSpinTemp && p->Flag_IncludePecVelsFor21cm) {
+    set_error("peculiar-velocity terms need the FFT workspace of a context (mxr_brightness_temperature)");
+    return MXR_ERR_UNSUPPORTED;
+  }
+  const long nreal = (long)N * N * N;
+  const int nblk = (int)ew_blocks(nreal);
+  if (nblk > TB_SCRATCH_BLOCKS) {
+    set_error("%d blocks of partial sums exceed the scratch of %d", nblk, TB_SCRATCH_BLOCKS);
+    return MXR_ERR_INVALID_ARG;
+  }
+  double* sc = nullptr;
+  RC(scratch_ptr(&sc));
+  const float redshift = (float)p->redshift;
+  const float Hubble_h = (float)p->Hubble_h;
+  const float OmegaM = (float)p->OmegaM;
+  const float OmegaB = (float)(OmegaM * p->BaryonFrac);
+  TbArgs a{};
+  a.N = N;
+  a.const_factor = (float)(27.0 * (OmegaB * Hubble_h * Hubble_h / 0.023) *
+                           std::sqrt((0.15 / OmegaM / Hubble_h / Hubble_h) * (1.0 + redshift) / 10.0));
+  a.T_rad = TCMB * (1. + redshift);
+  a.spin = p->Flag_IncludeSpinTemp ? 1 : 0;
+  a.tau = 0;
+  a.z = redshift;
+  a.xH = g->xH;
+  a.deltax = g->deltax;
+  a.TS = g->TS_box;
+  a.dT = g->delta_T;
+  a.partials = sc;
+  hipLaunchKernelGGL((k_tb<256>), dim3(nblk), dim3(256), 0, stream, a);
+  MXR_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, stream, sc, nblk, 1, sc + TB_SCRATCH_BLOCKS);
+  MXR_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_mean_out, dim3(1), dim3(64), 0, stream, sc + TB_SCRATCH_BLOCKS, std::pow((double)N, 3),
+                     ave_dev);
+  MXR_HIP(hipGetLastError());
+  return MXR_OK;
+}
+
 // ---- peculiar-velocity terms (BrightnessTemperature.c:158-510), each in the reference's types
 
 // velocity grid to comoving velocity, in place on the N^3 cells of the padded grid (:169-198)

@@ -163,6 +163,8 @@ int check_hii_params(const mxr_hii_params* p, int N);
 // grid dimensions the library instantiates (include/mxr.h lists them at mxr_supported_dims)
 #define MXR_DIMS(X) X(16) X(32) X(64) X(96) X(128) X(160) X(256) X(512)
 bool supported_dim(int N);
+int run_brightness_temperature_noalloc(const mxr_tb_params* p, const mxr_tb_grids* g, double* ave_dev,
+                                       hipStream_t stream);
 int run_find_hii_bubbles(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_hii_globals* out);
 int run_filter(Ctx* c, float2* box, int local_ix_start, int slab_nx, int grid_dim, float dk, float R, int ftype);
 // device table of the type 0 / 2 window at (dk, R) over (|n_x|, |n_y|, n_z) in 0..N/2, kept in

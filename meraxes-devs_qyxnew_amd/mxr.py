@@ -553,6 +553,8 @@ def lib():
     L.mxr_brightness_temperature.argtypes = [vp, ctypes.POINTER(TbParams), ctypes.POINTER(TbGrids),
                                              ctypes.POINTER(ctypes.c_double)]
     L.mxr_brightness_temperature.restype = ctypes.c_int
+    L.mxr_brightness_temperature_noalloc.argtypes = [ctypes.POINTER(TbParams), ctypes.POINTER(TbGrids), vp, vp]
+    L.mxr_brightness_temperature_noalloc.restype = ctypes.c_int
     L.mxr_supported_dims.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
     L.mxr_supported_dims.restype = ctypes.c_int
     L.mxr_ps_length.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
@@ -894,6 +896,18 @@ def _with_dummy_state(arrays: dict) -> dict:
     for k in ("xH", "z_at_ionization", "J_21_at_ionization", "r_bubble", "temp_kinetic_all_gas"):
         a.setdefault(k, ref)
     return a
+
+
+def brightness_temperature_noalloc(params: TbParams, xH, deltax, delta_T, ave, TS_box=None, stream=None):
+    """mxr_brightness_temperature_noalloc on device tensors: delta_T and the one-element FP64
+    tensor `ave` are written on `stream` (a hipStream_t value; None for the null stream),
+    without synchronising."""
+    g = TbGrids()
+    g.mem = _mem_of([xH, deltax, delta_T, ave, TS_box])  # host memory is refused by the library
+    g.xH, g.deltax, g.TS_box, g.delta_T = _ptr(xH), _ptr(deltax), _ptr(TS_box), _ptr(delta_T)
+    g.vel = None
+    check(lib().mxr_brightness_temperature_noalloc(ctypes.byref(params), ctypes.byref(g), _ptr(ave), stream),
+          "mxr_brightness_temperature_noalloc")
 
 
 def supported_dims() -> list:
