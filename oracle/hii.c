@@ -529,6 +529,10 @@ int oracle_hii_sample_f32(const mxr_hii_params* p, const mxr_hii_grids* g, int n
   return hii_run_f32(p, g, NULL, NULL, n_radii > 0 ? n_radii : 0, seconds, seconds_fwd);
 }
 
+/* the temperature helpers on their own, for the tests that pin them against closed forms */
+float oracle_fully_ionized_T(float z_re, float z, float delta) { return fully_ionized_T(z_re, z, delta); }
+float oracle_partially_ionized_T(float T_HI, float res_xH) { return partially_ionized_T(T_HI, res_xH); }
+
 /* ---------------------------------------------------------------- brightness box */
 #define N_RSD_STEPS 50            /* reionization.h:34 */
 #define MAX_DVDR (float)(0.2)     /* reionization.h:45 */

@@ -108,6 +108,11 @@ int oracle_ts_step_f32(const mxr_ts_params* p, const mxr_ts_tables* tb, const mx
 int oracle_ts_step_f64(const mxr_ts_params* p, const mxr_ts_tables* tb, const mxr_ts_grids* g,
                        mxr_ts_globals* out, double* red0, float* smoothed);
 
+/* Helpers on their own, for the tests that pin them. */
+float oracle_fully_ionized_T(float z_re, float z, float delta);
+float oracle_partially_ionized_T(float T_HI, float res_xH);
+int oracle_ts_probe(const mxr_ts_params* p, int what, const double* args, double* out);
+
 int oracle_num_threads(void);
 
 #ifdef __cplusplus

@@ -501,3 +501,34 @@ int oracle_ts_step_f64(const mxr_ts_params* p, const mxr_ts_tables* tb, const mx
 {
   return ts_step(p, tb, g, out, red0, smoothed, 1);
 }
+
+/* The helpers on their own, for the tests that pin them (tables read from the params' directory):
+ * what = 0 get_Ts(z, delta, TK, xe, Jalpha) -> out[0] T_S, out[1] x_alpha; 1 xcoll(z, TK,
+ * delta, xe); 2 alpha_A(T); 3 dT_comp(z, TK, xe); 4 hubble(z); 5 dtdz(z). */
+int oracle_ts_probe(const mxr_ts_params* p, int what, const double* a, double* out)
+{
+  ts_env* e = malloc(sizeof(ts_env));
+  if (!e)
+    return -3;
+  if (env_init(e, p)) {
+    free(e);
+    return MXR_ERR_TABLES;
+  }
+  int rc = 0;
+  switch (what) {
+    case 0: {
+      float xa = 0.f;
+      out[0] = get_Ts(e, (float)a[0], (float)a[1], (float)a[2], (float)a[3], (float)a[4], &xa);
+      out[1] = xa;
+      break;
+    }
+    case 1: out[0] = xcoll(e, a[0], a[1], a[2], a[3]); break;
+    case 2: out[0] = alpha_A(a[0]); break;
+    case 3: out[0] = dT_comp(e, a[0], a[1], a[2]); break;
+    case 4: out[0] = hubble(e, (float)a[0]); break;
+    case 5: out[0] = dtdz(e, (float)a[0]); break;
+    default: rc = MXR_ERR_INVALID_ARG;
+  }
+  free(e);
+  return rc;
+}

@@ -63,6 +63,13 @@ def lib():
                                      ctypes.POINTER(mxr.HiiGlobals), fp]
     L.oracle_find_hii_bubbles_fftw.argtypes = [ctypes.POINTER(mxr.HiiParams), ctypes.POINTER(mxr.HiiGrids),
                                                ctypes.POINTER(mxr.HiiGlobals), ctypes.POINTER(ctypes.c_double)]
+    L.oracle_fully_ionized_T.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float]
+    L.oracle_fully_ionized_T.restype = ctypes.c_float
+    L.oracle_partially_ionized_T.argtypes = [ctypes.c_float, ctypes.c_float]
+    L.oracle_partially_ionized_T.restype = ctypes.c_float
+    L.oracle_ts_probe.argtypes = [ctypes.POINTER(mxr.TsParams), ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double)]
+    L.oracle_ts_probe.restype = ctypes.c_int
     L.oracle_fftw_available.restype = ctypes.c_int
     L.oracle_fftw_library.restype = ctypes.c_char_p
     L.oracle_hii_sample_f32.argtypes = [ctypes.POINTER(mxr.HiiParams), ctypes.POINTER(mxr.HiiGrids), ctypes.c_int,
@@ -156,6 +163,16 @@ def find_hii_bubbles(params, inputs: dict, state: "mxr.HiiState", precision="f32
     if rc:
         raise mxr.MxrError(rc, "oracle_find_hii_bubbles")
     return (out, m) if margin else out
+
+
+def ts_probe(params, what: int, *args):
+    """oracle_ts_probe: 0 get_Ts -> (T_S, x_alpha); 1 xcoll; 2 alpha_A; 3 dT_comp; 4 hubble; 5 dtdz."""
+    a = (ctypes.c_double * 8)(*args)
+    out = (ctypes.c_double * 2)()
+    rc = lib().oracle_ts_probe(ctypes.byref(params), what, a, out)
+    if rc:
+        raise mxr.MxrError(rc, "oracle_ts_probe")
+    return (out[0], out[1]) if what == 0 else out[0]
 
 
 def fftw_available() -> bool:
