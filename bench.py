@@ -498,6 +498,7 @@ def main():
     # bisected once on this GPU path so that seed 2024 ends at a global xH of 0.500
     ap.add_argument("--stars-to-matter", type=float, default=0.010438)
     ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--graph-steps", type=int, default=10, help="timed replays of the captured call (0: skip)")
     ap.add_argument("--bf16-steps", type=int, default=3, help="timed calls of the BF16-storage variant (0: skip)")
     ap.add_argument("--cpu", choices=("fftw", "own", "none"), default="fftw",
                     help="CPU baseline: one whole call on the FFTW3 float API (fftw), a sample on the oracle's own "
@@ -577,6 +578,32 @@ def main():
     ms_step = 1e3 * wall / args.steps
     fused_xh = g.volume_weighted_global_xH
     log(f"bench: fused {ms_step:.2f} ms/step (device {sum(dev_ms) / len(dev_ms):.2f} ms), xH={fused_xh:.6f}")
+
+    # ---- the same call captured once as a hipGraph and replayed (mxr_find_hii_bubbles_batched with
+    # one realisation); its fields must equal the direct call's bit for bit
+    graph = None
+    if args.graph_steps > 0:
+        xh_direct = st.xH.clone()
+        b1 = mxr.Batch([N])
+        if args.autotune != "never":  # the launch shapes of the direct call (they set the rounding)
+            b1.load_wisdom([params], args.wisdom)
+        arrays = [dict(inputs, **st.arrays())]
+        reset_state(st)
+        b1.find_hii_bubbles([params], arrays, mxr.MXR_BATCH_GRAPH)  # capture, instantiate, launch
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.graph_steps):
+            gl = b1.find_hii_bubbles([params], arrays, mxr.MXR_BATCH_GRAPH)
+        torch.cuda.synchronize()
+        gms = 1e3 * (time.perf_counter() - t0) / args.graph_steps
+        same = bool(torch.equal(st.xH, xh_direct)) and gl[0].volume_weighted_global_xH == fused_xh
+        graph = {"value": N ** 3 * n_radii / (gms / 1e3), "ms_per_step": round(gms, 4),
+                 "device_ms": round(gl[0].device_ms, 4), "bit_identical_to_direct_call": same,
+                 "note": "the whole call (forward transforms, every radius, globals) captured once as a hipGraph "
+                         "and replayed while parameters and grid pointers are unchanged"}
+        log("bench: graph replay " + json.dumps(graph))
+        b1.close()
+        del xh_direct
 
     # ---- BF16-storage variant: stars and weighted_sfr held as bfloat16 in HBM, FP32 compute
     bf16 = None
@@ -693,6 +720,7 @@ def main():
                                          "kernel moves"},
         "kernels": kernels,
         "plain": plain,
+        "graph_replay": graph,
         "bf16_storage": bf16,
         "tuning": tuning,
         "cpu_baseline": cpu,

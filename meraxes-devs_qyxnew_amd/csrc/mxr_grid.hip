@@ -32,7 +32,8 @@ __global__ void k_grid_finish(Finish f)
 #pragma clang fp contract(off)
   const int N = f.N, H2 = 2 * (N / 2 + 1);
   const long npad = (long)N * N * H2;
-  const long hrow = f.hi_padded ? 2 * (f.n_hi / 2 + 1) : f.n_hi;
+  // the smoothed grid is the c2r output in a spectrum buffer: rows of 2 spec_pitch(n_hi) floats
+  const long hrow = f.hi_padded ? 2 * spec_pitch(f.n_hi) : f.n_hi;
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < npad; p += (long)gridDim.x * blockDim.x) {
     const int iz = (int)(p % H2);
     float v = 0.f;

@@ -1,8 +1,9 @@
 // mxr_hii.hip -- find_HII_bubbles on one gfx950 GPU.
 //
 // Data layout in HBM (all per grid g in {deltax, stars, weighted_sfr[, x_e]}):
-//   spec[g]  unfiltered half spectrum, reference layout I = k + H (j + N i), H = N/2 + 1,
-//            divided by N^3 (find_HII_bubbles.c:235-253); written once per call.
+//   spec[g]  unfiltered half spectrum I = k + HP (j + N i), k < H = N/2 + 1, HP = spec_pitch(N)
+//            (rows padded to 128 bytes), divided by N^3 (find_HII_bubbles.c:235-253); written
+//            once per call.
 //   work[g]  same shape: per radius the x-pass writes filtered+transformed columns here, the
 //            y-pass transforms them in place, the z-pass (c2r) consumes them.
 //   ion      one byte per cell: ionised during this call (replaces the xH = 1 reset and the
@@ -309,7 +310,8 @@ __device__ __forceinline__ void row_c2r_body(CGridPtrs in, int ngrid, long nrows
   if constexpr (EPI == C2R_STORE) {
     if (!valid)
       return;
-    float* __restrict__ dst = const_cast<float*>(reinterpret_cast<const float*>(pick(in, g0))) + row * 2 * H;
+    float* __restrict__ dst =
+        const_cast<float*>(reinterpret_cast<const float*>(pick(in, g0))) + row * 2 * spec_pitch(N);
 #pragma unroll
     for (int q = 0; q < SL::QL; q++) {
       const int j = t + T * q;
@@ -480,7 +482,7 @@ k_row_direct(CGridPtrs in, long nrows, int rpb, const float2* __restrict__ twg, 
 #pragma unroll
       for (int t = 1; t < NG; t++)
         base = g == t ? gp[t] : base;
-      const float2 x = base[row * H + (ok ? k : 0)];
+      const float2 x = base[row * spec_pitch(N) + (ok ? k : 0)];
       pre[i] = ok ? x : make_float2(0.f, 0.f);
     }
   };
@@ -569,14 +571,17 @@ __global__ void k_scale(float2* __restrict__ p, long n, double div)
 }
 
 // filter(): the window on a slab of a half spectrum
-__global__ void k_filter(float2* __restrict__ box, int local_ix_start, int slab_nx, FiltArgs fa)
+// pitch: complex values per row (N/2 + 1 for the caller's layout, spec_pitch(N) internally)
+__global__ void k_filter(float2* __restrict__ box, int local_ix_start, int slab_nx, FiltArgs fa, int pitch)
 {
   const int N = fa.N, H = N / 2 + 1;
-  const long n = (long)slab_nx * N * H;
+  const long n = (long)slab_nx * N * pitch;
   for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (long)gridDim.x * blockDim.x) {
-    const int nz = (int)(idx % H);
-    const int ny = (int)((idx / H) % N);
-    const int nx = (int)(idx / ((long)H * N)) + local_ix_start;
+    const int nz = (int)(idx % pitch);
+    if (nz >= H)
+      continue;
+    const int ny = (int)((idx / pitch) % N);
+    const int nx = (int)(idx / ((long)pitch * N)) + local_ix_start;
     if (fa.ftype == 1) {
       if (!sharpk_keep(nx, ny, nz, fa))
         box[idx] = make_float2(0.f, 0.f);
@@ -588,14 +593,14 @@ __global__ void k_filter(float2* __restrict__ box, int local_ix_start, int slab_
   }
 }
 
-// sanity_check_aliasing on one padded real grid: kind 0 deltax, 1 stars / sfr / BH grids, 2 x_e
+// sanity_check_aliasing on one real grid in a spectrum buffer (rows of 2 spec_pitch(N) floats):
+// kind 0 deltax, 1 stars / sfr / BH grids, 2 x_e
 __global__ void k_clamp(float* __restrict__ f, int N, int kind)
 {
-  const int H = N / 2 + 1;
   const long n = (long)N * N * N;
   for (long ir = (long)blockIdx.x * blockDim.x + threadIdx.x; ir < n; ir += (long)gridDim.x * blockDim.x) {
     const long row = ir / N;
-    const long ip = row * 2 * H + ir % N;
+    const long ip = row * 2 * spec_pitch(N) + ir % N;
     float x = f[ip];
     if (kind == 0) {
       x = fmaxf(x, -1.0f + REL_TOL);
@@ -613,6 +618,7 @@ __global__ void k_clamp(float* __restrict__ f, int N, int kind)
 // find_HII_bubbles_gpu_main_loop shape, with the CPU path's semantics (:451-544)
 struct PlainArgs {
   int N;
+  // filtered fields in the spectrum buffers: real rows of 2 spec_pitch(N) floats
   const float* dfl;
   const float* sfl;
   const float* xfl;
@@ -625,13 +631,14 @@ __global__ void k_criterion(PlainArgs pa, HiiEpi ea)
   const int N = pa.N, H = N / 2 + 1;
   const long n = (long)N * N * N;
   for (long ir = (long)blockIdx.x * blockDim.x + threadIdx.x; ir < n; ir += (long)gridDim.x * blockDim.x) {
-    const long ip = (ir / N) * 2 * H + ir % N;
-    const double density_over_mean = 1.0 + (double)pa.dfl[ip];
-    const float sf = pa.sfl[ip];
+    const long ip = (ir / N) * 2 * H + ir % N;                  // the caller's padded grids
+    const long iw = (ir / N) * 2 * spec_pitch(N) + ir % N;      // the filtered fields
+    const double density_over_mean = 1.0 + (double)pa.dfl[iw];
+    const float sf = pa.sfl[iw];
     const double f_coll_stars = (double)sf / density_over_mean;
-    const float bf = ea.bh ? pa.bfl[ip] : 0.0f;
+    const float bf = ea.bh ? pa.bfl[iw] : 0.0f;
     const double f_coll_effective_bhm = ea.bh ? (double)bf / density_over_mean : 0.0;
-    const double neutral_fraction = ea.spin ? 1.0 - (double)pa.xfl[ip] : 1.0;
+    const double neutral_fraction = ea.spin ? 1.0 - (double)pa.xfl[iw] : 1.0;
     const double lhs = (f_coll_effective_bhm + f_coll_stars) * ea.f_coll_prefactor;
     float xh = ea.xH[ir];
     if (lhs > neutral_fraction * (1. + 0.0)) {
@@ -871,15 +878,16 @@ __global__ void k_tb_velconv(float* __restrict__ vel, int N, int trees, float z)
 }
 
 // optionally x Hubble_h (double product, float result), then velocity_gradient
-// (reionization.c:3435-3450): (a + ib)(0 + i k_z), k_z = n_z dk in float
-__global__ void k_vgrad(float2* __restrict__ s, long n, int H, float dk, double h, int scale_h)
+// (reionization.c:3435-3450): (a + ib)(0 + i k_z), k_z = n_z dk in float; pitch = values per row
+// (N/2 + 1 in the caller's layout; the internal pitch's padding entries are scaled and unused)
+__global__ void k_vgrad(float2* __restrict__ s, long n, int pitch, float dk, double h, int scale_h)
 {
 #pragma clang fp contract(off)
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     float2 x = s[i];
     if (scale_h)
       x = make_float2((float)((double)x.x * h), (float)((double)x.y * h));
-    const float kz = (float)(int)(i % H) * dk;
+    const float kz = (float)(int)(i % pitch) * dk;
     s[i] = make_float2(x.x * 0.0f - x.y * kz, x.x * kz + x.y * 0.0f);
   }
 }
@@ -888,7 +896,7 @@ struct TbVelArgs {
   int N, pecvel;
   float H_z, z;
   double T_rad, max_v_deriv;
-  const float* grad;  // dv/dz, padded real
+  const float* grad;  // dv/dz, real rows of 2 spec_pitch(N) floats (a spectrum buffer)
   const float* TS;
   float* dT;
 };
@@ -897,12 +905,11 @@ struct TbVelArgs {
 __global__ void k_tb_pecvel(TbVelArgs a)
 {
 #pragma clang fp contract(off)
-  const int N = a.N, H = N / 2 + 1;
+  const int N = a.N;
   const long n = (long)N * N * N;
   const double Hz = (double)a.H_z, opz = 1. + (double)a.z;
   for (long ir = (long)blockIdx.x * blockDim.x + threadIdx.x; ir < n; ir += (long)gridDim.x * blockDim.x) {
-    const long ip = (ir / N) * 2 * H + ir % N;
-    const double g = (double)a.grad[ip];
+    const double g = (double)a.grad[(ir / N) * 2 * spec_pitch(N) + ir % N];
     if (a.pecvel == 1) {
       double dvdx = g;
       if (fabs(dvdx) > a.max_v_deriv)
@@ -1099,12 +1106,12 @@ struct Hii : Passes<N> {
 
 int ensure_buffers(Ctx* c, int ngrid)
 {
-  const long Nc = n_complex(c->N);
+  const long Ns = n_spec(c->N);
   for (int g = 0; g < ngrid; g++) {
     if (!c->spec[g])
-      MXR_HIP(hipMalloc(&c->spec[g], sizeof(float2) * Nc));
+      MXR_HIP(hipMalloc(&c->spec[g], sizeof(float2) * Ns));
     if (!c->work[g])
-      MXR_HIP(hipMalloc(&c->work[g], sizeof(float2) * Nc));
+      MXR_HIP(hipMalloc(&c->work[g], sizeof(float2) * Ns));
   }
   if (!c->ion)
     MXR_HIP(hipMalloc(&c->ion, (size_t)n_real(c->N)));
@@ -1263,7 +1270,8 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
          [&] { return K::template col<-1, COL_PLAIN>(c, cspec, spec, ngrid, K::xgeo(), none, 1.0); }));
     for (int i = 0; i < ngrid; i++)
       RC(T(K_SCALE, 2 * cplx, [&] {
-        hipLaunchKernelGGL(k_scale, dim3(ew_blocks(Nc)), dim3(256), 0, c->stream, c->spec[i], Nc, total_n_cells);
+        hipLaunchKernelGGL(k_scale, dim3(ew_blocks(n_spec(N))), dim3(256), 0, c->stream, c->spec[i], n_spec(N),
+                           total_n_cells);
         MXR_HIP(hipGetLastError());
         return MXR_OK;
       }));
@@ -1327,12 +1335,14 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
     } else {
       for (int i = 0; i < ngrid; i++) {
         RC(T(K_COPY, 2 * cplx, [&] {
-          MXR_HIP(hipMemcpyAsync(c->work[i], c->spec[i], sizeof(float2) * Nc, hipMemcpyDeviceToDevice, c->stream));
+          MXR_HIP(hipMemcpyAsync(c->work[i], c->spec[i], sizeof(float2) * n_spec(N), hipMemcpyDeviceToDevice,
+                                 c->stream));
           return MXR_OK;
         }));
         if (!last)
           RC(T(K_FILTER, 2 * cplx, [&] {
-            hipLaunchKernelGGL(k_filter, dim3(ew_blocks(Nc)), dim3(256), 0, c->stream, c->work[i], 0, N, fa);
+            hipLaunchKernelGGL(k_filter, dim3(ew_blocks(n_spec(N))), dim3(256), 0, c->stream, c->work[i], 0, N, fa,
+                               spec_pitch(N));
             MXR_HIP(hipGetLastError());
             return MXR_OK;
           }));
@@ -1472,7 +1482,7 @@ int run_filter(Ctx* c, float2* box, int ix0, int nx, int N, float dk, float R, i
   fa.ftype = ftype;
   fa.apply = 1;
   const long n = (long)nx * N * (N / 2 + 1);
-  hipLaunchKernelGGL(k_filter, dim3(ew_blocks(n)), dim3(256), 0, c->stream, box, ix0, nx, fa);
+  hipLaunchKernelGGL(k_filter, dim3(ew_blocks(n)), dim3(256), 0, c->stream, box, ix0, nx, fa, N / 2 + 1);
   MXR_HIP(hipGetLastError());
   MXR_HIP(hipStreamSynchronize(c->stream));
   return MXR_OK;
@@ -1512,13 +1522,12 @@ static double tb_hubble(const mxr_tb_params* p, float z)
          std::sqrt(p->OmegaM * std::pow((double)opz, 3) + p->OmegaR * std::pow((double)opz, 4) + p->OmegaLambda);
 }
 
-// dv/dz of the (converted) velocity grid into c->spec[0] as a padded real grid
+// dv/dz of the (converted) velocity grid into c->spec[0] as a real grid (rows of 2 spec_pitch(N) floats)
 // (BrightnessTemperature.c:200-215): r2c, /N^3 and x Hubble_h, x i k_z, c2r
 template <int N>
 static int tb_velocity_gradient(Ctx* c, const float* vel, const mxr_tb_params* p)
 {
   using K = Hii<N>;
-  const long Nc = n_complex(N);
   CRealPtrs rin{};
   rin.p[0] = vel;
   GridPtrs spec{};
@@ -1532,8 +1541,8 @@ static int tb_velocity_gradient(Ctx* c, const float* vel, const mxr_tb_params* p
   RC((K::template col<-1, COL_SCALE>(c, cspec, spec, 1, K::xgeo(), none, std::pow((double)N, 3))));
   const float box_size = (float)p->BoxSize;  // reionization.c:3438-3439
   const float dk = (float)(2.0 * M_PI / box_size);
-  hipLaunchKernelGGL(k_vgrad, dim3(ew_blocks(Nc)), dim3(256), 0, c->stream, c->spec[0], Nc, N / 2 + 1, dk,
-                     p->Hubble_h, 1);
+  hipLaunchKernelGGL(k_vgrad, dim3(ew_blocks(n_spec(N))), dim3(256), 0, c->stream, c->spec[0], n_spec(N),
+                     spec_pitch(N), dk, p->Hubble_h, 1);
   MXR_HIP(hipGetLastError());
   RC((K::template col<+1, COL_PLAIN>(c, cspec, spec, 1, K::xgeo(), none, 1.0)));
   RC((K::template col<+1, COL_PLAIN>(c, cspec, spec, 1, K::ygeo(), none, 1.0)));

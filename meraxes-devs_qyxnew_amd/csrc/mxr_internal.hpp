@@ -150,6 +150,13 @@ struct Ctx {
 };
 
 inline long n_complex(int N) { return (long)N * N * (N / 2 + 1); }
+// Row pitch, in complex values, of the internal half spectra (spec, work): the N/2 + 1 modes
+// of a row rounded up to a multiple of 16, i.e. 128 bytes, so that every row starts on a cache
+// line and a column tile's 16 adjacent modes are one aligned line (with 257 modes at N = 512 the
+// 2056-byte rows put most 128-byte segments across two lines).  The caller's grids keep the
+// reference's layouts; only these buffers are padded.  Padding entries are never read.
+constexpr int spec_pitch(int N) { return ((N / 2 + 1) + 15) / 16 * 16; }
+inline long n_spec(int N) { return (long)N * N * spec_pitch(N); }
 inline long n_real(int N) { return (long)N * N * N; }
 
 // host physics (mxr_api.hip)

@@ -51,9 +51,14 @@ __device__ __forceinline__ float deldel(float v, double ave, double rave, double
 
 __device__ __forceinline__ float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
-// column c of a column pass starts at (c / cpp) * ps + (c % cpp); its elements are es apart
+// The columns of a pass over a half spectrum of dimension L are (outer, k): outer = j (x pass) or
+// i (y pass), k < L/2 + 1 the mode along z.  Column (outer, k) starts at outer * ostride + k and
+// its L elements are es apart.  Columns are numbered with the L/2 leading modes of every outer
+// index first (outer L/2 + k), then the Nyquist modes k = L/2 of all outer indices: every tile
+// of TC consecutive columns but the last is full, and a tile of leading modes covers TC adjacent,
+// aligned values of one spectrum row (the row padding is never touched).
 struct ColGeom {
-  long cpp, ps, es, ncol;
+  long ostride, es;
 };
 
 // Window of one radius.  apply = 0 on the last (unfiltered) step.
@@ -174,35 +179,43 @@ __global__ void __launch_bounds__(NT) k_col(CGridPtrs in, GridPtrs out, ColGeom 
 #pragma unroll
   for (int i = 0; i < TPB; i++) {
     const long tile = (long)blockIdx.x * TPB + i;
-    long col = tile * TC + c;
-    bool valid = tile < ntile && col < geo.ncol;
+    constexpr int HM = L / 2;
+    const long m = tile * TC + c;
+    bool valid = tile < ntile && m < (long)L * (HM + 1);
+    int outer = 0, kk = 0;
+    if (valid) {
+      if (m < (long)L * HM) {
+        outer = (int)(m / HM);
+        kk = (int)(m % HM);
+      } else {
+        outer = (int)(m - (long)L * HM);
+        kk = HM;
+      }
+    }
     if constexpr (MODE == COL_FILTER || MODE == COL_YPRUNE) {
       if (fa.live > 0) {
         // live-tile grid: tiles enumerate (outer index, k_z tile) pairs whose k_z tile starts
         // below the kept count; for the x pass the outer index is j with |n_y| < live
-        const int H = fa.N / 2 + 1;
         const int nkt = (fa.live + TC - 1) / TC;
         const int jj = (int)(tile / nkt), kt = (int)(tile % nkt);
-        int outer = jj;
+        outer = jj;
         if constexpr (MODE == COL_FILTER)
           outer = jj < fa.live ? jj : fa.N - (2 * fa.live - 1 - jj);
-        const int k = kt * TC + c;
-        col = (long)outer * H + k;
-        valid = tile < ntile && k < H;
+        kk = kt * TC + c;
+        valid = tile < ntile && kk <= HM;
       }
     }
-    base[i] = valid ? (col / geo.cpp) * geo.ps + (col % geo.cpp) : 0;
+    base[i] = valid ? (long)outer * geo.ostride + kk : 0;
     colnz[i] = valid;
     fj[i] = 0;
     fk[i] = 0;
     if constexpr (MODE == COL_FILTER) {
-      const int H = fa.N / 2 + 1;
-      fj[i] = (int)(col / H);
-      fk[i] = (int)(col % H);
+      fj[i] = outer;
+      fk[i] = kk;
       if (valid && fa.apply && fa.prune && fa.ftype == 1)
         colnz[i] = sharpk_keep(0, fj[i], fk[i], fa);
     } else if constexpr (MODE == COL_YPRUNE) {
-      fk[i] = (int)(col % geo.cpp);
+      fk[i] = kk;
       if (valid && fa.apply && fa.prune)
         colnz[i] = sharpk_keep(0, 0, fk[i], fa);
     }
@@ -400,7 +413,7 @@ __global__ void __launch_bounds__(NT) k_row_r2c(CRealPtrs in, GridPtrs out, long
   __syncthreads();
   if (!valid)
     return;
-  float2* __restrict__ dst = pick(out, g) + row * H;
+  float2* __restrict__ dst = pick(out, g) + row * spec_pitch(N);
   for (int k = t; k <= M; k += T) {
     const float2 zk = lds[LI::at(c, k % M)];
     const float2 zm0 = lds[LI::at(c, (M - k) % M)];
@@ -462,7 +475,7 @@ __device__ __forceinline__ void row_c2r_grid(const float2* __restrict__ base, bo
   static_assert(64 % T == 0, "a row must not straddle two wavefronts");
   using SL = Slots<M, T>;
   using LI = LdsRowPad<P>;
-  const float2* __restrict__ src = base + (valid ? row : 0) * H;
+  const float2* __restrict__ src = base + (valid ? row : 0) * spec_pitch(N);
   // the loads of a batch first, then their LDS stores: the loads are independent, so a batch is
   // in flight together instead of one round trip per element (two batches bound the registers)
   constexpr int KI = (H + T - 1) / T;
@@ -617,8 +630,9 @@ template <int N>
 struct Passes {
   using C = Cfg<N>;
 
-  static ColGeom xgeo() { return ColGeom{(long)N * C::H, 0, (long)N * C::H, (long)N * C::H}; }
-  static ColGeom ygeo() { return ColGeom{(long)C::H, (long)N * C::H, (long)C::H, (long)N * C::H}; }
+  static constexpr int HP = spec_pitch(N);
+  static ColGeom xgeo() { return ColGeom{(long)HP, (long)N * HP}; }
+  static ColGeom ygeo() { return ColGeom{(long)N * HP, (long)HP}; }
 
   // one column pass with TC columns per workgroup (the per-column arithmetic does not depend on TC)
   template <int S, int MODE, int TC>
@@ -626,7 +640,7 @@ struct Passes {
   {
     constexpr int NT = TC * C::COL_T;
     constexpr int TPB = C::COL_TPB;
-    long ntile = (geo.ncol + TC - 1) / TC;
+    long ntile = ((long)N * C::H + TC - 1) / TC;
     if ((MODE == COL_FILTER || MODE == COL_YPRUNE) && fa.live > 0) {
       const long nkt = (fa.live + TC - 1) / TC;
       ntile = (MODE == COL_FILTER ? 2L * fa.live - 1 : (long)N) * nkt;

@@ -80,8 +80,10 @@ template <int N, bool GEOM>
 __global__ void __launch_bounds__(PS_NT) k_ps_bin(PsArgs a)
 {
 #pragma clang fp contract(off)
-  constexpr int NW = PS_NT / 64, H = N / 2 + 1, NQ = GEOM ? 2 : 1;
-  constexpr unsigned n = (unsigned)N * N * H, nchunk = (n + 63) / 64;
+  // the spectrum's rows are spec_pitch(N) values apart: the index space includes the padding,
+  // whose entries are skipped
+  constexpr int NW = PS_NT / 64, H = N / 2 + 1, HP = spec_pitch(N), NQ = GEOM ? 2 : 1;
+  constexpr unsigned n = (unsigned)N * N * HP, nchunk = (n + 63) / 64;
   __shared__ float2 epair[PS_MAXB + 1];  // (edge[g], edge[g + 1])
   __shared__ double acc[NW][PS_MAXB][NQ];
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64, nb = a.nb;
@@ -127,8 +129,8 @@ __global__ void __launch_bounds__(PS_NT) k_ps_bin(PsArgs a)
   };
   auto step = [&] {
     nz += 64;
-    while (nz >= H) {
-      nz -= H;
+    while (nz >= HP) {
+      nz -= HP;
       if (++ny == N) {
         ny = 0;
         nx++;
@@ -140,9 +142,9 @@ __global__ void __launch_bounds__(PS_NT) k_ps_bin(PsArgs a)
     const unsigned c = sg * PS_U, c1 = min(nchunk, c + PS_U);
     {
       const unsigned idx0 = c * 64 + lane;
-      nz = (int)(idx0 % H);
-      ny = (int)((idx0 / H) % N);
-      nx = (int)(idx0 / (H * (unsigned)N));
+      nz = (int)(idx0 % HP);
+      ny = (int)((idx0 / HP) % N);
+      nx = (int)(idx0 / (HP * (unsigned)N));
       row_k();
     }
     float2 xs[PS_U];
@@ -159,7 +161,7 @@ __global__ void __launch_bounds__(PS_NT) k_ps_bin(PsArgs a)
       const unsigned idx = (c + u) * 64 + lane;
       int b = -1;
       double val = 0.0;
-      if (idx < n) {
+      if (idx < n && nz < H) {
         // (:160-166) |k| = (float)sqrt((double)(k_x^2 + k_y^2 + k_z^2)) in the reference, which is
         // the correctly rounded float square root: rounding sqrt to double (53 bits) and then to
         // float (24 bits) equals rounding it to float once, as 53 >= 2 x 24 + 2
@@ -248,8 +250,8 @@ __global__ void __launch_bounds__(256) k_ps_reduce(const double* __restrict__ pa
 template <int N, bool GEOM>
 static int ps_bins(Ctx* c, const PsArgs& a0, double* host)
 {
-  constexpr int H = N / 2 + 1, NQ = GEOM ? 2 : 1;
-  const long nchunk = ((long)N * N * H + 63) / 64;
+  constexpr int NQ = GEOM ? 2 : 1;
+  const long nchunk = (n_spec(N) + 63) / 64;
   const int nblk = (int)std::min<long>(PS_MAXBLK, std::max<long>(1, (nchunk + 8 * PS_U - 1) / (8 * PS_U)));
   PsArgs a = a0;
   a.partials = c->ps_part;
