@@ -64,7 +64,8 @@ typedef struct mxr_hii_params {
   int ReionRtoMFilterType;      /* 0 top-hat, 1 Gaussian                                  */
   int ReionUVBFlag;             /* 0, 1 (J21 at ionisation) or 2 (recombinations only)   */
   int Flag_IncludeSpinTemp;     /* x_e_box becomes a 4th filtered grid, Tk_box is used    */
-  int Flag_IncludeRecombinations; /* must be 0 (MXR_ERR_UNSUPPORTED otherwise)            */
+  int Flag_IncludeRecombinations; /* N_rec becomes a filtered grid; Gamma12, r_bubble,
+                                   UVBFlag 2 and the post-loop recombination update apply */
   int Flag_EvolvingReionRBubbleMax;
   int Flag_BHFeedback;          /* effective_bhm / effective_bhar become filtered grids    */
   double ReionSfrTimescale;     /* in units of the Hubble time                            */
@@ -85,6 +86,9 @@ typedef struct mxr_hii_params {
                                    it from TablesForXHeatingDir/recfast_LCDM.dat          */
   const char* TablesForXHeatingDir; /* directory holding recfast_LCDM.dat (may be NULL
                                        when TK_recfast >= 0)                              */
+  int Flag_TemperatureDependentRec; /* recombinations: the post-loop rate at the cell's gas
+                                       temperature (1) or at 1e4 K (0)                     */
+  double Y_He;                  /* helium mass fraction (physics.Y_He; recombinations only) */
 } mxr_hii_params;
 
 /* In/out grids of one find_HII_bubbles call (reion_grids_t fields, src/meraxes.h:448-681). */
@@ -105,6 +109,13 @@ typedef struct mxr_hii_grids {
                                    layout), widened to FP32 on load; fused paths only        */
   const float* effective_bhm;   /* in  padded real (Flag_BHFeedback only, else NULL)      */
   const float* effective_bhar;  /* in  padded real (Flag_BHFeedback only, else NULL)      */
+  /* Flag_IncludeRecombinations only (else NULL): */
+  float* N_rec;                 /* in/out padded real: recombinations per baryon (starts 0) */
+  float* Gamma12;               /* in/out real: ionising background at the first crossing
+                                   (1e-12/s h^-2; persists across calls, starts 0)          */
+  float* residual_xH;           /* out real: 1e4 x residual neutral fraction               */
+  float* clumping_factor;       /* out real                                              */
+  float* t_resp;                /* out real: 1 / (Gamma + recombination rate), Myr        */
 } mxr_hii_grids;
 #define MXR_STORAGE_F32 0
 #define MXR_STORAGE_BF16 1
@@ -214,6 +225,17 @@ int mxr_velocity_gradient(mxr_ctx* ctx, void* box, int mem, int slab_nx, int gri
 /* Mass enclosed by a filter of radius R (internal units).  Replaces RtoM
  * (src/core/find_HII_bubbles.c:27-46, declared src/core/find_HII_bubbles.h:11).          */
 double mxr_RtoM(const mxr_hii_params* params, double R);
+
+/* The recombination tables find_HII_bubbles uses with Flag_IncludeRecombinations, for the
+ * params' cosmology (Hubble_h, OmegaM, BaryonFrac, Y_He): per (z_ct, t_ct, g) with z = 0.2 z_ct
+ * (z_ct < 200), log10 T = 2 + 0.03 t_ct (t_ct < 100), ln Gamma_12 = -10 + 0.1 g (g < 150), the
+ * integrals over the MHR00 density PDF with self-shielding of the recombination rate per baryon
+ * (ln, 1/s), the clumping factor, and 1e4 times the residual neutral fraction (ln), case B.
+ * Built on the GPU on first use and kept with the context.  Each output (may be NULL) receives
+ * 200 x 100 x 150 doubles, z_ct slowest; lnGamma the 150 nodes.  No counterpart entry point in
+ * the reference, which keeps them in recombinations.c (init_MHR, :208-374) or an HDF5 cache. */
+int mxr_recombination_tables(mxr_ctx* ctx, const mxr_hii_params* params, double* lnGamma, double* ln_rate,
+                             double* clumping, double* ln_residual);
 
 /* Number of filter radii find_HII_bubbles visits for these parameters, and (if radii is not
  * NULL and capacity allows) the radii themselves in Mpc/h, last (unfiltered) one included

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "mxr_internal.hpp"
+#include "mxr_recomb.hpp"
 #include "mxr_ts.hpp"
 
 namespace mxr {
@@ -162,9 +163,9 @@ int check_hii_params(const mxr_hii_params* p, int N)
     set_error("Unrecognised RtoM filter (%d)", p->ReionRtoMFilterType);
     return MXR_ERR_INVALID_ARG;
   }
-  if (p->Flag_IncludeRecombinations) {
-    set_error("Flag_IncludeRecombinations is not implemented by this library");
-    return MXR_ERR_UNSUPPORTED;
+  if (p->Flag_IncludeRecombinations && !(p->Y_He > 0 && p->Y_He < 1)) {
+    set_error("Flag_IncludeRecombinations needs 0 < Y_He < 1 (got %g)", p->Y_He);
+    return MXR_ERR_INVALID_ARG;
   }
   return MXR_OK;
 }
@@ -303,9 +304,9 @@ int mxr_create(mxr_ctx** out, int device, int grid_dim)
     return fail(hip_fail(e, "hipMalloc(twiddles)", __FILE__, __LINE__));
   if ((e = hipMemcpy(c.tw, tw.data(), sizeof(float2) * grid_dim, hipMemcpyHostToDevice)) != hipSuccess)
     return fail(hip_fail(e, "hipMemcpy(twiddles)", __FILE__, __LINE__));
-  if ((e = hipMalloc(&c.sums, sizeof(double) * 16)) != hipSuccess)
+  if ((e = hipMalloc(&c.sums, sizeof(double) * 32)) != hipSuccess)
     return fail(hip_fail(e, "hipMalloc(sums)", __FILE__, __LINE__));
-  if ((e = hipHostMalloc((void**)&c.sums_host, sizeof(double) * 16, hipHostMallocDefault)) != hipSuccess)
+  if ((e = hipHostMalloc((void**)&c.sums_host, sizeof(double) * 32, hipHostMallocDefault)) != hipSuccess)
     return fail(hip_fail(e, "hipHostMalloc(sums)", __FILE__, __LINE__));
   (void)rc;
   *out = h;
@@ -321,14 +322,17 @@ void mxr_destroy(mxr_ctx* h)
   if (c.stream)
     (void)hipStreamSynchronize(c.stream);
   (void)hipFree(c.tw);
-  for (int g = 0; g < 6; g++) {
+  for (int g = 0; g < 7; g++) {
     (void)hipFree(c.spec[g]);
     (void)hipFree(c.work[g]);
   }
   for (int g = 0; g < 7; g++)
     (void)hipFree(c.in_stage[g]);
-  for (int g = 0; g < 5; g++)
+  for (int g = 0; g < 10; g++)
     (void)hipFree(c.out_stage[g]);
+  (void)hipFree(c.rec_tab);
+  (void)hipFree(c.rec_coef);
+  (void)hipFree(c.rec_x);
   (void)hipFree(c.ion);
   for (int g = 0; g < 2; g++)
     (void)hipFree(c.ts_src[g]);
@@ -535,6 +539,20 @@ int mxr_find_hii_bubbles(mxr_ctx* h, const mxr_hii_params* params, const mxr_hii
   return run_find_hii_bubbles(&h->c, params, grids, globals);
 }
 
+int mxr_recombination_tables(mxr_ctx* h, const mxr_hii_params* p, double* lnGamma, double* ln_rate,
+                             double* clumping, double* ln_residual)
+{
+  if (!h || !p) {
+    set_error("null context or params");
+    return MXR_ERR_INVALID_ARG;
+  }
+  if (hipSetDevice(h->c.device) != hipSuccess) {
+    set_error("hipSetDevice(%d) failed", h->c.device);
+    return MXR_ERR_HIP;
+  }
+  return rec_tables_copy(&h->c, p, lnGamma, ln_rate, clumping, ln_residual);
+}
+
 int mxr_filter(mxr_ctx* h, void* box, int mem, int local_ix_start, int slab_nx, int grid_dim, double BoxSize,
                float R, int filter_type)
 {
@@ -720,12 +738,13 @@ static void graph_key_append(std::vector<unsigned char>& k, const mxr_hii_params
   for (double v : {p.BoxSize, p.Hubble_h, p.OmegaM, p.OmegaLambda, p.OmegaK, p.BaryonFrac, p.ReionDeltaRFactor})
     key_put(k, v);
   for (int v : {p.ReionFilterType, p.ReionRtoMFilterType, p.ReionUVBFlag, p.Flag_IncludeSpinTemp,
-                p.Flag_IncludeRecombinations, p.Flag_EvolvingReionRBubbleMax, p.Flag_BHFeedback})
+                p.Flag_IncludeRecombinations, p.Flag_EvolvingReionRBubbleMax, p.Flag_BHFeedback,
+                p.Flag_TemperatureDependentRec})
     key_put(k, v);
   for (double v : {p.ReionSfrTimescale, p.ReionEfficiency, p.ReionNionPhotPerBary, p.ReionGammaHaloBias,
                    p.ReionAlphaUV, p.ReionAlphaUVBH, p.ReionRBubbleMax, p.ReionRBubbleMaxRecomb, p.ReionRBubbleMin,
                    p.UnitLength_in_cm, p.UnitMass_in_g, p.UnitVelocity_in_cm_per_s, p.redshift, p.prev_redshift,
-                   p.TK_recfast})
+                   p.TK_recfast, p.Y_He})
     key_put(k, v);
   const char* dir = p.TablesForXHeatingDir ? p.TablesForXHeatingDir : "";
   const size_t len = std::strlen(dir);
@@ -737,7 +756,8 @@ static void graph_key_append(std::vector<unsigned char>& k, const mxr_hii_params
                         (const void*)g.x_e_box, (const void*)g.Tk_box, (const void*)g.xH,
                         (const void*)g.z_at_ionization, (const void*)g.J_21_at_ionization, (const void*)g.r_bubble,
                         (const void*)g.temp_kinetic_all_gas, (const void*)g.effective_bhm,
-                        (const void*)g.effective_bhar})
+                        (const void*)g.effective_bhar, (const void*)g.N_rec, (const void*)g.Gamma12,
+                        (const void*)g.residual_xH, (const void*)g.clumping_factor, (const void*)g.t_resp})
     key_put(k, q);
   for (int v : {t.zdirect_kmax, t.zdirect_rpb, t.live_tiles, t.col_tc_x, t.col_tc_y})
     key_put(k, v);
@@ -761,7 +781,9 @@ int mxr_find_hii_bubbles_batched(mxr_batch* b, const mxr_hii_params* params, con
     if (g.mem != MXR_MEM_DEVICE || !g.deltax || !g.stars || !g.weighted_sfr || !g.xH || !g.z_at_ionization ||
         !g.J_21_at_ionization || !g.r_bubble || !g.temp_kinetic_all_gas ||
         (params[i].Flag_IncludeSpinTemp && (!g.x_e_box || !g.Tk_box)) ||
-        (params[i].Flag_BHFeedback && (!g.effective_bhm || !g.effective_bhar))) {
+        (params[i].Flag_BHFeedback && (!g.effective_bhm || !g.effective_bhar)) ||
+        (params[i].Flag_IncludeRecombinations &&
+         (!g.N_rec || !g.Gamma12 || !g.residual_xH || !g.clumping_factor || !g.t_resp))) {
       set_error("realisation %d: batched grids must all be device memory", i);
       return MXR_ERR_INVALID_ARG;
     }
@@ -785,10 +807,16 @@ int mxr_find_hii_bubbles_batched(mxr_batch* b, const mxr_hii_params* params, con
   }
   for (int i = 0; i < n; i++) {
     Ctx& c = b->ctx[i]->c;
-    const int rc = ensure_buffers(&c, 3 + (params[i].Flag_IncludeSpinTemp ? 1 : 0) +
-                                            (params[i].Flag_BHFeedback ? 2 : 0));  // no allocation inside a capture
+    // no allocation inside a capture: buffers and the recombination tables are made first
+    int rc = ensure_buffers(&c, 3 + (params[i].Flag_IncludeSpinTemp ? 1 : 0) + (params[i].Flag_BHFeedback ? 2 : 0) +
+                                  (params[i].Flag_IncludeRecombinations ? 1 : 0));
     if (rc)
       return rc;
+    if (params[i].Flag_IncludeRecombinations) {
+      RecTab rt;
+      if ((rc = rec_tables(&c, &params[i], &rt)))
+        return rc;
+    }
     c.prof = false;
   }
   auto enqueue_all = [&]() -> int {

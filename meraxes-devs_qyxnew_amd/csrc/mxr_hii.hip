@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "mxr_passes.hpp"
+#include "mxr_recomb.hpp"
 
 namespace mxr {
 
@@ -58,7 +59,19 @@ struct HiiEpi {
   int sfr_bf16;         // sfr holds bfloat16 bit patterns
   const float* Tk_box;  // real (spin only)
   const float* bhar;    // unfiltered effective_bhar (padded; BH only)
-  double* partials;     // up to 8 per block
+  double* partials;     // up to 17 per block
+  // recombinations (find_HII_bubbles.c:226-230, 467-506, 613-636)
+  int rec, temp_dep;
+  float R;               // (float)R: r_bubble at the first crossing
+  float gpref, gpref_bh; // (float)Gamma_R_prefactor, (float)Gamma_R_prefactor_BH (with sigma_HI)
+  double redshift, hubble_h;
+  float fabs_dtdz, zstep;
+  float* Gamma12;        // real, in/out
+  float* N_rec;          // padded, in/out (updated after the radius loop)
+  float* residual_xH;    // real, out
+  float* clumping;       // real, out
+  float* t_resp;         // real, out
+  RecTab rt;
 };
 
 // x^y for x > 0 (or +0, +inf) in double as exp(y log x): the relative error is about
@@ -622,7 +635,10 @@ struct PlainArgs {
   const float* dfl;
   const float* sfl;
   const float* xfl;
-  const float* bfl;  // filtered effective_bhm (BH only)
+  const float* bfl;   // filtered effective_bhm (BH only)
+  const float* wfl;   // filtered weighted_sfr (recombinations only)
+  const float* bafl;  // filtered effective_bhar (recombinations with BH only)
+  const float* nfl;   // filtered N_rec (recombinations only)
 };
 
 __global__ void k_criterion(PlainArgs pa, HiiEpi ea)
@@ -639,9 +655,21 @@ __global__ void k_criterion(PlainArgs pa, HiiEpi ea)
     const float bf = ea.bh ? pa.bfl[iw] : 0.0f;
     const double f_coll_effective_bhm = ea.bh ? (double)bf / density_over_mean : 0.0;
     const double neutral_fraction = ea.spin ? 1.0 - (double)pa.xfl[iw] : 1.0;
+    const double rec = ea.rec ? (double)pa.nfl[iw] / density_over_mean : 0.0;
     const double lhs = (f_coll_effective_bhm + f_coll_stars) * ea.f_coll_prefactor;
     float xh = ea.xH[ir];
-    if (lhs > neutral_fraction * (1. + 0.0)) {
+    if (lhs > neutral_fraction * (1. + rec)) {
+      // first crossing in this call (largest R) with recombinations: the ionising background,
+      // the radius and, for UVBFlag 2, J_21 (:488-506)
+      if (ea.rec && xh > REL_TOL) {
+        float g12 = __fmul_rn(pa.wfl[iw], ea.gpref);
+        if (ea.bh)
+          g12 = __fadd_rn(g12, __fmul_rn(pa.bafl[iw], ea.gpref_bh));
+        ea.Gamma12[ir] = g12;
+        ea.r_bubble[ir] = ea.R;
+        if (ea.uvb == 2)
+          ea.J21[ir] = j21_at_crossing(ea, sf, bf);
+      }
       xh = 0.0f;
     } else if (ea.last && (xh > REL_TOL)) {
       const float T_HI = ea.spin ? ea.Tk_box[ir] : thermal_T(ea.TK, ea.cT_ad, ea.deltax[ip]);
@@ -670,13 +698,17 @@ __global__ void k_criterion(PlainArgs pa, HiiEpi ea)
 }
 
 // post-loop temperatures and FP64 partial sums (find_HII_bubbles.c:577-645)
-template <int NT>
+template <int NT, bool REC>
 __global__ void __launch_bounds__(NT) k_postloop(int N, HiiEpi ea)
 {
 #pragma clang fp contract(off)
   const int H = N / 2 + 1;
   const long n = (long)N * N * N;
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  constexpr int NV = REC ? 17 : 7;
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++)
+    acc[q] = 0.0;
   for (long ir = (long)blockIdx.x * NT + threadIdx.x; ir < n; ir += (long)gridDim.x * NT) {
     const long ip = (ir / N) * 2 * H + ir % N;
     const float xh = ea.xH[ir];
@@ -698,8 +730,38 @@ __global__ void __launch_bounds__(NT) k_postloop(int N, HiiEpi ea)
     acc[5] += (double)temp * dom;
     if (ea.bh)
       acc[6] += (double)ea.bhar[ip];
+    if constexpr (REC) {
+      // :612-636: the cell's recombinations over the snapshot step from the tables at its
+      // effective redshift, background Gamma_12 and temperature
+      const double z_eff = (1. + ea.redshift) * pow(dom, 1.0 / 3.0) - 1;
+      const double T = ea.temp_dep ? (double)temp : 1e4;
+      const double h = ea.hubble_h;
+      const float g12 = ea.Gamma12[ir];
+      double rate, rnh, cf;
+      splined_recombination(ea.rt, z_eff, (double)g12 * h * h, T, &rate, &rnh, &cf);
+      const float step = __fmul_rn(__fmul_rn((float)rate, ea.fabs_dtdz), ea.zstep);
+      const float nrec = (float)((double)ea.N_rec[ip] + (double)step * (1. - (double)(float)xh));
+      ea.N_rec[ip] = nrec;
+      const float res = (float)rnh, clump = (float)cf;
+      ea.residual_xH[ir] = res;
+      ea.clumping[ir] = clump;
+      float tr = (float)((double)g12 * h * h * 1e-12 + rate);
+      tr = tr > 0 ? (float)(1.0 / (double)tr / SEC_PER_MEGAYEAR) : 1e30f;
+      ea.t_resp[ir] = tr;
+      acc[7] += (double)nrec;
+      acc[8] += (double)nrec * dom;
+      acc[9] += (double)res;
+      acc[10] += (double)res * dom;
+      acc[11] += (double)clump;
+      acc[12] += (double)clump * dom;
+      acc[13] += (double)g12;
+      acc[14] += (double)g12 * dom;
+      const float rb = ea.r_bubble[ir];  // zero unless set at a first crossing with recombinations
+      acc[15] += (double)rb;
+      acc[16] += (double)rb * dom;
+    }
   }
-  block_sum<NT, 7>(acc, ea.partials + (long)blockIdx.x * 7);
+  block_sum<NT, NV>(acc, ea.partials + (long)blockIdx.x * NV);
 }
 
 // fixed-order final sum of per-block partials (deterministic)
@@ -1115,8 +1177,8 @@ int ensure_buffers(Ctx* c, int ngrid)
   }
   if (!c->ion)
     MXR_HIP(hipMalloc(&c->ion, (size_t)n_real(c->N)));
-  // per-block FP64 partials: up to 8 values for each of at most 8192 blocks (ew_blocks)
-  const size_t need = 8 * 8192 + 6 * (size_t)(((long)c->N * c->N + 15) / 16 + 16);
+  // per-block FP64 partials: up to 17 values for each of at most 8192 blocks (ew_blocks)
+  const size_t need = 17 * 8192 + 6 * (size_t)(((long)c->N * c->N + 15) / 16 + 16);
   if (c->partials_cap < need) {
     (void)hipFree(c->partials);
     c->partials = nullptr;
@@ -1146,6 +1208,19 @@ static int stage_out(Ctx* c, int slot, const float* host, size_t n, float** dev)
 
 
 
+// dtdz(float z) (XRayHeatingFunctions.c:426-440): dt/dz of a flat matter + Lambda universe, in s;
+// 1 + z is a float sum, as the reference's argument is a float
+static double hii_dtdz(const mxr_hii_params* p, float z)
+{
+  const double OMl = p->OmegaLambda, OMm = p->OmegaM;
+  const double x = std::sqrt(OMl / OMm) * std::pow(1 + z, -3.0 / 2.0);
+  const double dxdz = std::sqrt(OMl / OMm) * std::pow(1 + z, -5.0 / 2.0) * (-3.0 / 2.0);
+  const double const1 = 2 * std::sqrt(1 + OMm / OMl) / (3.0 * HUBBLE * p->Hubble_h);
+  const double numer = dxdz * (1 + x * std::pow(std::pow(x, 2) + 1, -0.5));
+  const double denom = x + std::sqrt(std::pow(x, 2) + 1);
+  return const1 * numer / denom;
+}
+
 template <int N>
 static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_hii_globals* out)
 {
@@ -1159,9 +1234,17 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
     return MXR_ERR_UNSUPPORTED;
   }
   const int bh = p->Flag_BHFeedback ? 1 : 0;
-  // grids: deltax, stars, weighted_sfr, [x_e], [effective_bhm, effective_bhar]
-  const int ngrid = 3 + spin + 2 * bh;
-  const int ixe = 3, ibhm = 3 + spin, ibhar = 4 + spin;
+  const int rec = p->Flag_IncludeRecombinations ? 1 : 0;
+  // grids: deltax, stars, weighted_sfr, [x_e], [effective_bhm, effective_bhar], [N_rec]
+  const int ngrid = 3 + spin + 2 * bh + rec;
+  const int ixe = 3, ibhm = 3 + spin, ibhar = 4 + spin, inrec = ngrid - 1;
+  if (rec && (!g->N_rec || !g->Gamma12 || !g->residual_xH || !g->clumping_factor || !g->t_resp)) {
+    set_error("Flag_IncludeRecombinations needs N_rec, Gamma12, residual_xH, clumping_factor and t_resp");
+    return MXR_ERR_INVALID_ARG;
+  }
+  RecTab rt{};
+  if (rec)
+    RC(rec_tables(c, p, &rt));
   RC(ensure_buffers(c, ngrid));
 
   // ---- inputs and state on the device
@@ -1173,15 +1256,21 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
     din[ibhar] = g->effective_bhar;
   }
   const float* dTk = spin ? g->Tk_box : nullptr;
-  float* dst[5] = {g->xH, g->z_at_ionization, g->J_21_at_ionization, g->r_bubble, g->temp_kinetic_all_gas};
+  // state: xH, z_at_ionization, J_21, r_bubble, temp, and with recombinations N_rec (padded, in/out),
+  // Gamma12 (in/out), residual_xH, clumping_factor, t_resp
+  const int nst = rec ? 10 : 5;
+  float* dst[10] = {g->xH, g->z_at_ionization, g->J_21_at_ionization, g->r_bubble, g->temp_kinetic_all_gas,
+                    g->N_rec, g->Gamma12, g->residual_xH, g->clumping_factor, g->t_resp};
   if (g->mem == MXR_MEM_HOST) {
-    for (int i = 0; i < ngrid; i++)
+    for (int i = 0; i < ngrid - rec; i++)
       RC(stage_in(c, i, din[i], (bf16 && (i == 1 || i == 2)) ? Nc : 2 * Nc, &din[i]));  // bf16: half the bytes
     if (spin)
       RC(stage_in(c, 6, dTk, nreal, &dTk));
-    for (int i = 0; i < 5; i++)
-      RC(stage_out(c, i, dst[i], nreal, &dst[i]));
+    for (int i = 0; i < nst; i++)
+      RC(stage_out(c, i, dst[i], i == 5 ? 2 * Nc : nreal, &dst[i]));
   }
+  if (rec)
+    din[inrec] = dst[5];
 
   // ---- scalars (find_HII_bubbles.c:90-148)
   const double box_size = p->BoxSize;
@@ -1222,6 +1311,21 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
   ea.sfr_bf16 = bf16 ? 1 : 0;
   ea.Tk_box = dTk;
   ea.partials = c->partials;
+  ea.rec = rec;
+  if (rec) {
+    ea.temp_dep = p->Flag_TemperatureDependentRec ? 1 : 0;
+    ea.redshift = redshift;
+    ea.hubble_h = p->Hubble_h;
+    // find_HII_bubbles.c:128-130: the snapshot step and |dt/dz| / h in float
+    ea.zstep = (float)(p->prev_redshift - redshift);
+    ea.fabs_dtdz = (float)std::fabs(hii_dtdz(p, (float)redshift) / p->Hubble_h);
+    ea.Gamma12 = dst[6];
+    ea.N_rec = dst[5];
+    ea.residual_xH = dst[7];
+    ea.clumping = dst[8];
+    ea.t_resp = dst[9];
+    ea.rt = rt;
+  }
 
   FiltArgs fa{};
   fa.N = N;
@@ -1275,6 +1379,15 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
         MXR_HIP(hipGetLastError());
         return MXR_OK;
       }));
+  } else {
+    RC(T(K_FWD_X, G * 2 * cplx,
+         [&] { return K::template col<-1, COL_SCALE>(c, cspec, spec, ngrid, K::xgeo(), none, total_n_cells); }));
+  }
+  // the criterion runs as its own kernel on the stored fields (the plain path, and every path with
+  // recombinations, whose first-crossing update reads weighted_sfr and N_rec too): xH and r_bubble
+  // start at 1 and 0 as the reference resets them (find_HII_bubbles.c:150-158)
+  const bool split = plain || rec;
+  if (split)
     for (int f = 0; f < 2; f++)
       RC(T(K_SCALE, 4.0 * nreal, [&] {
         hipLaunchKernelGGL(k_fill, dim3(ew_blocks(nreal)), dim3(256), 0, c->stream, dst[f == 0 ? 0 : 3],
@@ -1282,10 +1395,6 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
         MXR_HIP(hipGetLastError());
         return MXR_OK;
       }));
-  } else {
-    RC(T(K_FWD_X, G * 2 * cplx,
-         [&] { return K::template col<-1, COL_SCALE>(c, cspec, spec, ngrid, K::xgeo(), none, total_n_cells); }));
-  }
 
   // ---- radius loop (:279-547)
   for (int ri = 0; ri < nR; ri++) {
@@ -1296,9 +1405,15 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
     Gamma_R_prefactor *= (UM / UT) / PROTONMASS * p->ReionNionPhotPerBary;
     ea.J21c = (float)(Gamma_R_prefactor * PLANCK * 1e21 / (4.0 * M_PI) * p->ReionGammaHaloBias / sfr_timescale *
                       p->ReionAlphaUV);
-    const double Gamma_R_prefactor_BH = Gamma_R_prefactor;  // :429-433
+    double Gamma_R_prefactor_BH = Gamma_R_prefactor;  // :429-433
     ea.J21c_bh = (float)(Gamma_R_prefactor_BH * PLANCK * 1e21 / (4.0 * M_PI) * p->ReionGammaHaloBias /
                          sfr_timescale * p->ReionAlphaUVBH);
+    // :438-441: the ionising-background prefactors, used at first crossings with recombinations
+    Gamma_R_prefactor *= SIGMA_HI * p->ReionAlphaUV / (p->ReionAlphaUV + 2.75) * 1e12;
+    Gamma_R_prefactor_BH *= SIGMA_HI * p->ReionAlphaUVBH / (p->ReionAlphaUVBH + 2.75) * 1e12;
+    ea.gpref = (float)Gamma_R_prefactor;
+    ea.gpref_bh = (float)Gamma_R_prefactor_BH;
+    ea.R = (float)R;
     ea.f_coll_prefactor = p->ReionEfficiency * (4.0 / 3.0) * M_PI / pixel_volume / M_mean * (R * R * R);
     ea.first = (ri == 0);
     ea.last = last;
@@ -1313,7 +1428,7 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
       // written and the deltax / temp (or Tk_box) needed for T_HI read
       const double state_b = last ? (double)nreal * (1 + 4 + 4 + 4 + 4 + 4) : (double)nreal * 1.0;
       const int kc = kept_kz(fa);
-      const bool direct = (N % 256 == 0) && kc < H && kc <= c->zdirect_kmax && kc <= 64;
+      const bool direct = !rec && (N % 256 == 0) && kc < H && kc <= c->zdirect_kmax && kc <= 64;
       // live-tile grids for the x and y passes when the window cuts k_z (the kept columns then
       // have |n_y| < kc and k_z < kc)
       fa.live = (kc < H && c->live_tiles) ? kc : 0;
@@ -1321,17 +1436,23 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
            [&] { return K::template col<+1, COL_FILTER>(c, cspec, work, ngrid, K::xgeo(), fa, 1.0, c->col_tc_x); }));
       RC(T(K_INV_Y, G * 8.0 * ((double)N * km.c2 + (double)N * N * km.k1),
            [&] { return K::template col<+1, COL_YPRUNE>(c, cwork, work, ngrid, K::ygeo(), fa, 1.0, c->col_tc_y); }));
-      RC(T(K_INV_Z, G * 8.0 * (double)N * N * km.k1 + state_b, [&] {
-        if (direct)
-          return bh ? (spin ? K::template c2r_direct<C2R_HII_SPIN_BH>(c, cwork, kc, c->zdirect_rpb, ea)
-                            : K::template c2r_direct<C2R_HII_BH>(c, cwork, kc, c->zdirect_rpb, ea))
-                    : (spin ? K::template c2r_direct<C2R_HII_SPIN>(c, cwork, kc, c->zdirect_rpb, ea)
-                            : K::template c2r_direct<C2R_HII>(c, cwork, kc, c->zdirect_rpb, ea));
-        return bh ? (spin ? K::template c2r<C2R_HII_SPIN_BH>(c, cwork, ngrid, fa, ea)
-                          : K::template c2r<C2R_HII_BH>(c, cwork, ngrid, fa, ea))
-                  : (spin ? K::template c2r<C2R_HII_SPIN>(c, cwork, ngrid, fa, ea)
-                          : K::template c2r<C2R_HII>(c, cwork, ngrid, fa, ea));
-      }));
+      if (rec) {
+        // every filtered field stored as a real grid (kept k_z only), then the criterion kernel
+        RC(T(K_INV_Z, G * 8.0 * (double)N * N * km.k1 + G * 4.0 * nreal,
+             [&] { return K::template c2r<C2R_STORE>(c, cwork, ngrid, fa, ea); }));
+      } else {
+        RC(T(K_INV_Z, G * 8.0 * (double)N * N * km.k1 + state_b, [&] {
+          if (direct)
+            return bh ? (spin ? K::template c2r_direct<C2R_HII_SPIN_BH>(c, cwork, kc, c->zdirect_rpb, ea)
+                              : K::template c2r_direct<C2R_HII_BH>(c, cwork, kc, c->zdirect_rpb, ea))
+                      : (spin ? K::template c2r_direct<C2R_HII_SPIN>(c, cwork, kc, c->zdirect_rpb, ea)
+                              : K::template c2r_direct<C2R_HII>(c, cwork, kc, c->zdirect_rpb, ea));
+          return bh ? (spin ? K::template c2r<C2R_HII_SPIN_BH>(c, cwork, ngrid, fa, ea)
+                            : K::template c2r<C2R_HII_BH>(c, cwork, ngrid, fa, ea))
+                    : (spin ? K::template c2r<C2R_HII_SPIN>(c, cwork, ngrid, fa, ea)
+                            : K::template c2r<C2R_HII>(c, cwork, ngrid, fa, ea));
+        }));
+      }
     } else {
       for (int i = 0; i < ngrid; i++) {
         RC(T(K_COPY, 2 * cplx, [&] {
@@ -1355,16 +1476,22 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
         RC(T(K_INV_X, 2 * cplx, [&] { return K::template col<+1, COL_PLAIN>(c, ci, wi, 1, K::xgeo(), none, 1.0); }));
         RC(T(K_INV_Y, 2 * cplx, [&] { return K::template col<+1, COL_PLAIN>(c, ci, wi, 1, K::ygeo(), none, 1.0); }));
         RC(T(K_INV_Z, 2 * cplx, [&] { return K::template c2r<C2R_STORE>(c, ci, 1, none, ea); }));
+      }
+    }
+    if (split) {
+      // the aliasing clamp of each stored field (:372-422), then the criterion and state update
+      for (int i = 0; i < ngrid; i++)
         RC(T(K_CLAMP, 8.0 * nreal, [&] {
           hipLaunchKernelGGL(k_clamp, dim3(ew_blocks(nreal)), dim3(256), 0, c->stream, (float*)c->work[i], N,
                              i == 0 ? 0 : ((spin && i == ixe) ? 2 : 1));
           MXR_HIP(hipGetLastError());
           return MXR_OK;
         }));
-      }
       PlainArgs pa{N, (const float*)c->work[0], (const float*)c->work[1],
-                   spin ? (const float*)c->work[ixe] : nullptr, bh ? (const float*)c->work[ibhm] : nullptr};
-      RC(T(K_CRIT, (double)nreal * (4 + 4 + (spin ? 4 : 0) + 4 + 4 + 4), [&] {
+                   spin ? (const float*)c->work[ixe] : nullptr, bh ? (const float*)c->work[ibhm] : nullptr,
+                   rec ? (const float*)c->work[2] : nullptr, (rec && bh) ? (const float*)c->work[ibhar] : nullptr,
+                   rec ? (const float*)c->work[inrec] : nullptr};
+      RC(T(K_CRIT, (double)nreal * (4 + 4 + (spin ? 4 : 0) + 4 + 4 + 4 + (rec ? 8 : 0)), [&] {
         hipLaunchKernelGGL(k_criterion, dim3(ew_blocks(nreal)), dim3(256), 0, c->stream, pa, ea);
         MXR_HIP(hipGetLastError());
         return MXR_OK;
@@ -1375,25 +1502,34 @@ static int hii_run(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_
   // ---- reductions (:549-664)
   // post-loop temperatures and FP64 sums: xH, deltax, z_in, temp, weighted_sfr read, temp written
   const int nblk = (int)ew_blocks(nreal);
-  RC(T(K_POST, (double)nreal * (4 + 4 + 4 + 4 + 4 + 4 + (spin ? 4 : 0)), [&] {
-    hipLaunchKernelGGL((k_postloop<256>), dim3(nblk), dim3(256), 0, c->stream, N, ea);
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, c->stream, c->partials, nblk, 7, c->sums);
+  // with recombinations also Gamma12, N_rec and r_bubble read, N_rec, residual_xH,
+  // clumping_factor and t_resp written
+  const int nsum = rec ? 17 : 7;
+  RC(T(K_POST, (double)nreal * (4 + 4 + 4 + 4 + 4 + 4 + (spin ? 4 : 0) + (bh ? 4 : 0) + (rec ? 7 * 4 : 0)), [&] {
+    if (rec)
+      hipLaunchKernelGGL((k_postloop<256, true>), dim3(nblk), dim3(256), 0, c->stream, N, ea);
+    else
+      hipLaunchKernelGGL((k_postloop<256, false>), dim3(nblk), dim3(256), 0, c->stream, N, ea);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, c->stream, c->partials, nblk, nsum, c->sums);
     MXR_HIP(hipGetLastError());
     return MXR_OK;
   }));
   if (!c->enqueue_only)
     MXR_HIP(hipEventRecord(c->ev1, c->stream));
-  MXR_HIP(hipMemcpyAsync(c->sums_host, c->sums, sizeof(double) * 7, hipMemcpyDeviceToHost, c->stream));
+  MXR_HIP(hipMemcpyAsync(c->sums_host, c->sums, sizeof(double) * nsum, hipMemcpyDeviceToHost, c->stream));
   c->fin_total_n_cells = total_n_cells;
   c->fin_sfr_conv = UM / UT * SEC_PER_YEAR / SOLAR_MASS;
   c->fin_n_radii = nR;
   c->fin_bh = bh;
+  c->fin_rec = rec;
   if (c->enqueue_only)  // batched: the caller synchronises and calls hii_globals
     return MXR_OK;
   if (g->mem == MXR_MEM_HOST) {
-    float* hdst[5] = {g->xH, g->z_at_ionization, g->J_21_at_ionization, g->r_bubble, g->temp_kinetic_all_gas};
-    for (int i = 0; i < 5; i++)
-      MXR_HIP(hipMemcpyAsync(hdst[i], dst[i], sizeof(float) * nreal, hipMemcpyDeviceToHost, c->stream));
+    float* hdst[10] = {g->xH,     g->z_at_ionization, g->J_21_at_ionization, g->r_bubble,        g->temp_kinetic_all_gas,
+                       g->N_rec, g->Gamma12,         g->residual_xH,        g->clumping_factor, g->t_resp};
+    for (int i = 0; i < nst; i++)
+      MXR_HIP(hipMemcpyAsync(hdst[i], dst[i], sizeof(float) * (i == 5 ? 2 * Nc : nreal), hipMemcpyDeviceToHost,
+                             c->stream));
   }
   MXR_HIP(hipStreamSynchronize(c->stream));
   float ms = 0.f;
@@ -1415,6 +1551,18 @@ void hii_globals(const Ctx* c, mxr_hii_globals* out)
   out->volume_weighted_global_temp_kinetic_all_gas = s[4] / c->fin_total_n_cells;
   out->mass_weighted_global_temp_kinetic_all_gas = s[5] / s[2];
   out->volume_weighted_global_effective_bhar = c->fin_bh ? s[6] / c->fin_total_n_cells * c->fin_sfr_conv : 0.0;
+  if (c->fin_rec) {  // :683-700
+    out->volume_weighted_global_N_rec = s[7] / c->fin_total_n_cells;
+    out->mass_weighted_global_N_rec = s[8] / s[2];
+    out->volume_weighted_global_residual_xH = s[9] / c->fin_total_n_cells;
+    out->mass_weighted_global_residual_xH = s[10] / s[2];
+    out->volume_weighted_global_clumping_factor = s[11] / c->fin_total_n_cells;
+    out->mass_weighted_global_clumping_factor = s[12] / s[2];
+    out->volume_weighted_global_Gamma12 = s[13] / c->fin_total_n_cells;
+    out->mass_weighted_global_Gamma12 = s[14] / s[2];
+    out->volume_weighted_global_r_bubble = s[15] / c->fin_total_n_cells;
+    out->mass_weighted_global_r_bubble = s[16] / s[2];
+  }
   out->n_radii = c->fin_n_radii;
 }
 

@@ -20,6 +20,8 @@ constexpr double PLANCK = 6.6262e-27;
 constexpr double PROTONMASS = 1.6726e-24;
 constexpr double HUBBLE = 3.2407789e-18;
 constexpr double SEC_PER_YEAR = 3.155e7;
+constexpr double SEC_PER_MEGAYEAR = 3.155e13;
+constexpr double SIGMA_HI = 6.3e-18;  // HI ionisation cross section at 13.6 eV, cm^2
 constexpr double TCMB = 2.728;
 constexpr double MPC = 3.086e24;          // cm
 constexpr double T_RE = 1e4;
@@ -98,11 +100,18 @@ struct Ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float2* tw = nullptr;            // tw[m] = exp(-2 pi i m / N), m < N (FP64-evaluated, FP32-stored)
-  float2* spec[6] = {};            // unfiltered half spectra (deltax, stars, weighted_sfr,
-                                   // [x_e], [effective_bhm, effective_bhar])
-  float2* work[6] = {};            // filtered spectra / FFT work (same size)
+  float2* spec[7] = {};            // unfiltered half spectra (deltax, stars, weighted_sfr,
+                                   // [x_e], [effective_bhm, effective_bhar], [N_rec])
+  float2* work[7] = {};            // filtered spectra / FFT work (same size)
   float* in_stage[7] = {};         // device copies of host inputs: 6 padded grids + Tk_box
-  float* out_stage[5] = {};        // device copies of host state grids
+  float* out_stage[10] = {};       // device copies of host state grids: xH, z_in, J21, r_bubble,
+                                   // temp, and with recombinations N_rec (padded), Gamma12,
+                                   // residual_xH, clumping_factor, t_resp
+  // recombination tables (mxr_recomb.hip), built for the cosmology in rec_key
+  double* rec_tab = nullptr;
+  double* rec_coef = nullptr;
+  double* rec_x = nullptr;
+  double rec_key[4] = {};
   uint8_t* ion = nullptr;          // per-cell "ionised during this call" flags
   double* partials = nullptr;      // per-block FP64 partial sums
   size_t partials_cap = 0;
@@ -147,6 +156,7 @@ struct Ctx {
   double fin_total_n_cells = 0, fin_sfr_conv = 0;
   int fin_n_radii = 0;
   int fin_bh = 0;
+  int fin_rec = 0;
 };
 
 inline long n_complex(int N) { return (long)N * N * (N / 2 + 1); }

@@ -11,8 +11,8 @@
 
 namespace mxr {
 
-// up to 6 grids: deltax, stars, weighted_sfr, [x_e], [effective_bhm, effective_bhar]
-constexpr int MAX_GRIDS = 6;
+// up to 7 grids: deltax, stars, weighted_sfr, [x_e], [effective_bhm, effective_bhar], [N_rec]
+constexpr int MAX_GRIDS = 7;
 struct GridPtrs {
   float2* p[MAX_GRIDS];
 };
@@ -85,7 +85,13 @@ enum { COL_PLAIN = 0, COL_SCALE = 1, COL_FILTER = 2, COL_YPRUNE = 3, COL_FSCALE 
 template <class P>
 __device__ __forceinline__ auto pick(const P& ps, int g)
 {
-  return g == 0 ? ps.p[0] : g == 1 ? ps.p[1] : g == 2 ? ps.p[2] : g == 3 ? ps.p[3] : g == 4 ? ps.p[4] : ps.p[5];
+  return g == 0   ? ps.p[0]
+         : g == 1 ? ps.p[1]
+         : g == 2 ? ps.p[2]
+         : g == 3 ? ps.p[3]
+         : g == 4 ? ps.p[4]
+         : g == 5 ? ps.p[5]
+                  : ps.p[6];
 }
 // ---- the reference's window arithmetic (float, unfused) --------------------------------
 // correctly rounded float square root of x = 0 or a normal x > 0 (sqrtf's result): the hardware

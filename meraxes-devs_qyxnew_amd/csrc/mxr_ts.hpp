@@ -22,7 +22,6 @@ constexpr double NUIONIZATION = 13.60 * NU_over_EV;
 constexpr double HeI_NUIONIZATION = 24.59 * NU_over_EV;
 constexpr double HeII_NUIONIZATION = NUIONIZATION * 4;
 constexpr double NU_LA = 10.2 * NU_over_EV;
-constexpr double SIGMA_HI = 6.3e-18;
 constexpr double TINY = 1e-30;
 constexpr double FRACT_FLOAT_ERR = 1e-7;
 constexpr float MAX_TK = 5e4f;

@@ -19,7 +19,7 @@
 #include <thread>
 #include <vector>
 
-#include "mxr_gk61.hpp"
+#include "mxr_qag.hpp"
 #include "mxr_ts.hpp"
 
 namespace mxr {
@@ -440,217 +440,14 @@ static double HeI_ion_crosssec(double nu)
          std::pow(1.0 + std::sqrt(y / 1.469), -3.188);
 }
 
-// ---- adaptive Gauss-Kronrod (61) quadrature with GSL QAG's bisection and error rules --------
-struct QagWs {
-  std::vector<double> alist, blist, rlist, elist;
-  std::vector<size_t> order, level;
-  size_t size = 0, nrmax = 0, i = 0, maximum_level = 0, limit = 0;
-  explicit QagWs(size_t n) : alist(n), blist(n), rlist(n), elist(n), order(n), level(n), limit(n) {}
-};
-
-static double rescale_error(double err, double result_abs, double result_asc)
-{
-  err = std::fabs(err);
-  if (result_asc != 0 && err != 0) {
-    const double scale = std::pow((200 * err / result_asc), 1.5);
-    err = scale < 1 ? result_asc * scale : result_asc;
-  }
-  if (result_abs > 2 * DBL_MINV / (50 * DBL_EPS)) {
-    const double min_err = 50 * DBL_EPS * result_abs;
-    if (min_err > err)
-      err = min_err;
-  }
-  return err;
-}
-
+// ---- adaptive Gauss-Kronrod (61) quadrature: GSL's QAG (mxr_qag.hpp) with the reference's
+// workspace of 1000 intervals, one workspace per host thread
 template <class F>
-static void qk61(const F& f, double a, double b, double* result, double* abserr, double* resabs, double* resasc)
+static int qag_gsl(const F& f, double a, double b, double epsabs, double epsrel, int limit, double* result,
+                   double* abserr)
 {
-  const int n = 31;
-  double fv1[31], fv2[31];
-  const double center = 0.5 * (a + b);
-  const double half_length = 0.5 * (b - a);
-  const double abs_half_length = std::fabs(half_length);
-  const double f_center = f(center);
-  double result_gauss = 0;
-  double result_kronrod = f_center * GK61_WGK[n - 1];
-  double result_abs = std::fabs(result_kronrod);
-  for (int j = 0; j < (n - 1) / 2; j++) {
-    const int jtw = j * 2 + 1;
-    const double abscissa = half_length * GK61_XGK[jtw];
-    const double fval1 = f(center - abscissa), fval2 = f(center + abscissa);
-    const double fsum = fval1 + fval2;
-    fv1[jtw] = fval1;
-    fv2[jtw] = fval2;
-    result_gauss += GK61_WG[j] * fsum;
-    result_kronrod += GK61_WGK[jtw] * fsum;
-    result_abs += GK61_WGK[jtw] * (std::fabs(fval1) + std::fabs(fval2));
-  }
-  for (int j = 0; j < n / 2; j++) {
-    const int jtwm1 = j * 2;
-    const double abscissa = half_length * GK61_XGK[jtwm1];
-    const double fval1 = f(center - abscissa), fval2 = f(center + abscissa);
-    fv1[jtwm1] = fval1;
-    fv2[jtwm1] = fval2;
-    result_kronrod += GK61_WGK[jtwm1] * (fval1 + fval2);
-    result_abs += GK61_WGK[jtwm1] * (std::fabs(fval1) + std::fabs(fval2));
-  }
-  const double mean = result_kronrod * 0.5;
-  double result_asc = GK61_WGK[n - 1] * std::fabs(f_center - mean);
-  for (int j = 0; j < n - 1; j++)
-    result_asc += GK61_WGK[j] * (std::fabs(fv1[j] - mean) + std::fabs(fv2[j] - mean));
-  const double err = (result_kronrod - result_gauss) * half_length;
-  result_kronrod *= half_length;
-  result_abs *= abs_half_length;
-  result_asc *= abs_half_length;
-  *result = result_kronrod;
-  *resabs = result_abs;
-  *resasc = result_asc;
-  *abserr = rescale_error(err, result_abs, result_asc);
-}
-
-// keeps order[] sorted by decreasing error over the part of the list still to be bisected
-static void qpsrt(QagWs& w)
-{
-  const size_t last = w.size - 1, limit = w.limit;
-  size_t i_nrmax = w.nrmax, i_maxerr = w.order[i_nrmax];
-  if (last < 2) {
-    w.order[0] = 0;
-    w.order[1] = 1;
-    w.i = i_maxerr;
-    return;
-  }
-  const double errmax = w.elist[i_maxerr];
-  while (i_nrmax > 0 && errmax > w.elist[w.order[i_nrmax - 1]]) {
-    w.order[i_nrmax] = w.order[i_nrmax - 1];
-    i_nrmax--;
-  }
-  long top = (last < (limit / 2 + 2)) ? (long)last : (long)(limit - last + 1);
-  long i = (long)i_nrmax + 1;
-  while (i < top && errmax < w.elist[w.order[i]]) {
-    w.order[i - 1] = w.order[i];
-    i++;
-  }
-  w.order[i - 1] = i_maxerr;
-  const double errmin = w.elist[last];
-  long k = top - 1;
-  while (k > i - 2 && errmin >= w.elist[w.order[k]]) {
-    w.order[k + 1] = w.order[k];
-    k--;
-  }
-  w.order[k + 1] = last;
-  i_maxerr = w.order[i_nrmax];
-  w.i = i_maxerr;
-  w.nrmax = i_nrmax;
-}
-
-static void qag_update(QagWs& w, double a1, double b1, double area1, double error1, double a2, double b2,
-                       double area2, double error2)
-{
-  const size_t i_max = w.i, i_new = w.size;
-  const size_t new_level = w.level[i_max] + 1;
-  if (error2 > error1) {
-    w.alist[i_max] = a2;
-    w.rlist[i_max] = area2;
-    w.elist[i_max] = error2;
-    w.level[i_max] = new_level;
-    w.alist[i_new] = a1;
-    w.blist[i_new] = b1;
-    w.rlist[i_new] = area1;
-    w.elist[i_new] = error1;
-    w.level[i_new] = new_level;
-  } else {
-    w.blist[i_max] = b1;
-    w.rlist[i_max] = area1;
-    w.elist[i_max] = error1;
-    w.level[i_max] = new_level;
-    w.alist[i_new] = a2;
-    w.blist[i_new] = b2;
-    w.rlist[i_new] = area2;
-    w.elist[i_new] = error2;
-    w.level[i_new] = new_level;
-  }
-  w.size++;
-  if (new_level > w.maximum_level)
-    w.maximum_level = new_level;
-  qpsrt(w);
-}
-
-// returns 0 on success (tolerance reached), nonzero where GSL returns an error status
-template <class F>
-static int qag(const F& f, double a, double b, double epsabs, double epsrel, size_t limit, double* result,
-               double* abserr)
-{
-  QagWs w(limit);
-  w.size = 0;
-  w.nrmax = 0;
-  w.i = 0;
-  w.alist[0] = a;
-  w.blist[0] = b;
-  w.rlist[0] = 0.0;
-  w.elist[0] = 0.0;
-  w.order[0] = 0;
-  w.level[0] = 0;
-  w.maximum_level = 0;
-  *result = 0;
-  *abserr = 0;
-  double result0, abserr0, resabs0, resasc0;
-  qk61(f, a, b, &result0, &abserr0, &resabs0, &resasc0);
-  w.size = 1;
-  w.rlist[0] = result0;
-  w.elist[0] = abserr0;
-  double tolerance = std::max(epsabs, epsrel * std::fabs(result0));
-  const volatile double round_off = 50 * DBL_EPS * resabs0;
-  if (abserr0 <= round_off && abserr0 > tolerance) {
-    *result = result0;
-    *abserr = abserr0;
-    return 1;
-  } else if ((abserr0 <= tolerance && abserr0 != resasc0) || abserr0 == 0.0) {
-    *result = result0;
-    *abserr = abserr0;
-    return 0;
-  } else if (limit == 1) {
-    *result = result0;
-    *abserr = abserr0;
-    return 2;
-  }
-  double area = result0, errsum = abserr0;
-  size_t iteration = 1;
-  int roundoff_type1 = 0, roundoff_type2 = 0, error_type = 0;
-  do {
-    const size_t ii = w.i;
-    const double a_i = w.alist[ii], b_i = w.blist[ii], r_i = w.rlist[ii], e_i = w.elist[ii];
-    const double a1 = a_i, b1 = 0.5 * (a_i + b_i), a2 = b1, b2 = b_i;
-    double area1, error1, resabs1, resasc1, area2, error2, resabs2, resasc2;
-    qk61(f, a1, b1, &area1, &error1, &resabs1, &resasc1);
-    qk61(f, a2, b2, &area2, &error2, &resabs2, &resasc2);
-    const double area12 = area1 + area2, error12 = error1 + error2;
-    errsum += (error12 - e_i);
-    area += area12 - r_i;
-    if (resasc1 != error1 && resasc2 != error2) {
-      const double delta = r_i - area12;
-      if (std::fabs(delta) <= 1.0e-5 * std::fabs(area12) && error12 >= 0.99 * e_i)
-        roundoff_type1++;
-      if (iteration >= 10 && error12 > e_i)
-        roundoff_type2++;
-    }
-    tolerance = std::max(epsabs, epsrel * std::fabs(area));
-    if (errsum > tolerance) {
-      if (roundoff_type1 >= 6 || roundoff_type2 >= 20)
-        error_type = 2;
-      const double tmp = (1 + 100 * DBL_EPS) * (std::fabs(a2) + 1000 * DBL_MINV);
-      if (std::fabs(a1) <= tmp && std::fabs(b2) <= tmp)
-        error_type = 3;
-    }
-    qag_update(w, a1, b1, area1, error1, a2, b2, area2, error2);
-    iteration++;
-  } while (iteration < limit && !error_type && errsum > tolerance);
-  double s = 0;
-  for (size_t k = 0; k < w.size; k++)
-    s += w.rlist[k];
-  *result = s;
-  *abserr = errsum;
-  return errsum <= tolerance ? 0 : 3;
+  static thread_local QagWork<1000> w;
+  return qag<1000>(w, f, a, b, epsabs, epsrel, limit, result, abserr);
 }
 
 // ---- Brent's root bracketing (gsl_root_fsolver_brent + gsl_root_test_interval) --------------
@@ -956,7 +753,7 @@ static double tauX(const TauCtx& t, double nu, double x_e, double zp, double zpp
     return drpropdz * n * ff * sig;
   };
   double result, err;
-  if (qag(integrand, zpp, zp, 0, 0.005, 1000, &result, &err) != 0)
+  if (qag_gsl(integrand, zpp, zp, 0, 0.005, 1000, &result, &err) != 0)
     return 1e10;
   return result;
 }
@@ -1025,11 +822,11 @@ static double integrate_over_nu(const TsData* d, const TsCosmo& cz, double zp, d
   double result = 0, err = 0;
   int st;
   if (flag == 0)
-    st = qag(heat, lo, hi, 0, 0.01, 1000, &result, &err);
+    st = qag_gsl(heat, lo, hi, 0, 0.01, 1000, &result, &err);
   else if (flag == 1)
-    st = qag(ion, lo, hi, 0, 0.01, 1000, &result, &err);
+    st = qag_gsl(ion, lo, hi, 0, 0.01, 1000, &result, &err);
   else
-    st = qag(lya, lo, hi, 0, 0.01, 1000, &result, &err);
+    st = qag_gsl(lya, lo, hi, 0, 0.01, 1000, &result, &err);
   if (st != 0)
     return 0.0;
   if (flag == 2)

@@ -433,10 +433,12 @@ int autotune(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_tuning
       *dst[i] = (const float*)q;
     }
   }
-  // scratch state
-  float* st[5] = {};
-  for (int i = 0; i < 5 && rc == MXR_OK; i++)
-    rc = dev_alloc(sizeof(float) * nreal, (void**)&st[i]);
+  // scratch state: xH, z_at_ionization, J_21, r_bubble, temp, and with recombinations N_rec
+  // (padded), Gamma12, residual_xH, clumping_factor, t_resp
+  const int nst = p->Flag_IncludeRecombinations ? 10 : 5;
+  float* st[10] = {};
+  for (int i = 0; i < nst && rc == MXR_OK; i++)
+    rc = dev_alloc(sizeof(float) * (i == 5 ? npad : nreal), (void**)&st[i]);
   if (rc) {
     release();
     return rc;
@@ -447,15 +449,22 @@ int autotune(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_tuning
   s.J_21_at_ionization = st[2];
   s.r_bubble = st[3];
   s.temp_kinetic_all_gas = st[4];
+  if (nst == 10) {
+    s.N_rec = st[5];
+    s.Gamma12 = st[6];
+    s.residual_xH = st[7];
+    s.clumping_factor = st[8];
+    s.t_resp = st[9];
+  }
 
   const bool saved_prof = c->prof;
   c->prof = false;
   mxr_tuning start;
   tuning_get(*c, &start);
   auto run_once = [&](double* ms) -> int {
-    const float init[5] = {1.0f, -1.0f, 0.0f, 0.0f, 0.0f};
-    for (int i = 0; i < 5; i++)
-      hipLaunchKernelGGL(k_fill_f, dim3(1024), dim3(256), 0, c->stream, st[i], init[i], nreal);
+    const float init[10] = {1.0f, -1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < nst; i++)
+      hipLaunchKernelGGL(k_fill_f, dim3(1024), dim3(256), 0, c->stream, st[i], init[i], i == 5 ? npad : nreal);
     MXR_HIP(hipGetLastError());
     mxr_hii_globals out;
     RC(run_find_hii_bubbles(c, p, &s, &out));

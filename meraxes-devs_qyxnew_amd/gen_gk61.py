@@ -134,12 +134,12 @@ def emit(xgk, wg, wgk) -> str:
         "",
         "namespace mxr {",
         "",
-        "static const double GK61_XGK[31] = {",
+        "static constexpr double GK61_XGK[31] = {",
     ]
     lines += [f"  {fmt(v)}," for v in xgk]
-    lines += ["};", "", "static const double GK61_WGK[31] = {"]
+    lines += ["};", "", "static constexpr double GK61_WGK[31] = {"]
     lines += [f"  {fmt(v)}," for v in wgk]
-    lines += ["};", "", "static const double GK61_WG[15] = {"]
+    lines += ["};", "", "static constexpr double GK61_WG[15] = {"]
     lines += [f"  {fmt(v)}," for v in wg]
     lines += ["};", "", "}  // namespace mxr", ""]
     return "\n".join(lines)

@@ -78,6 +78,8 @@ class HiiParams(ctypes.Structure):
         ("prev_redshift", ctypes.c_double),
         ("TK_recfast", ctypes.c_double),
         ("TablesForXHeatingDir", ctypes.c_char_p),
+        ("Flag_TemperatureDependentRec", ctypes.c_int),
+        ("Y_He", ctypes.c_double),
     ]
 
 
@@ -97,6 +99,11 @@ class HiiGrids(ctypes.Structure):
         ("storage", ctypes.c_int),
         ("effective_bhm", ctypes.c_void_p),
         ("effective_bhar", ctypes.c_void_p),
+        ("N_rec", ctypes.c_void_p),
+        ("Gamma12", ctypes.c_void_p),
+        ("residual_xH", ctypes.c_void_p),
+        ("clumping_factor", ctypes.c_void_p),
+        ("t_resp", ctypes.c_void_p),
     ]
 
 
@@ -396,6 +403,8 @@ def hii_params(N: int, box: float, filter_type: int = 1, zeta: float = 1.0, reds
     p.prev_redshift = redshift
     p.TK_recfast = -1.0
     p.TablesForXHeatingDir = (tables_dir or TABLES_DIR).encode()
+    p.Flag_TemperatureDependentRec = 0
+    p.Y_He = 0.24  # input/params/defaults.par
     for k, v in over.items():
         setattr(p, k, v)
     return p
@@ -541,6 +550,9 @@ def lib():
     L.mxr_find_hii_bubbles.argtypes = [vp, ctypes.POINTER(HiiParams), ctypes.POINTER(HiiGrids),
                                        ctypes.POINTER(HiiGlobals)]
     L.mxr_find_hii_bubbles.restype = ctypes.c_int
+    _dp = ctypes.POINTER(ctypes.c_double)
+    L.mxr_recombination_tables.argtypes = [vp, ctypes.POINTER(HiiParams), _dp, _dp, _dp, _dp]
+    L.mxr_recombination_tables.restype = ctypes.c_int
     L.mxr_filter.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                              ctypes.c_float, ctypes.c_int]
     L.mxr_filter.restype = ctypes.c_int
@@ -636,6 +648,7 @@ def _mem_of(arrays):
 
 HII_IN = ("deltax", "stars", "weighted_sfr", "x_e_box", "Tk_box", "effective_bhm", "effective_bhar")
 HII_OUT = ("xH", "z_at_ionization", "J_21_at_ionization", "r_bubble", "temp_kinetic_all_gas")
+HII_REC = ("N_rec", "Gamma12", "residual_xH", "clumping_factor", "t_resp")  # Flag_IncludeRecombinations
 
 
 def _is_bf16(a):
@@ -648,7 +661,7 @@ def hii_grids(**arrays) -> HiiGrids:
     storage = arrays.pop("storage", None)
     g = HiiGrids()
     g.mem = _mem_of(list(arrays.values()))
-    for name in HII_IN + HII_OUT:
+    for name in HII_IN + HII_OUT + HII_REC:
         setattr(g, name, _ptr(arrays.get(name)))
     if storage is None:
         storage = MXR_STORAGE_BF16 if _is_bf16(arrays.get("stars")) else MXR_STORAGE_F32
@@ -746,6 +759,16 @@ class Context:
         check(lib().mxr_find_hii_bubbles(self._h, ctypes.byref(params), ctypes.byref(g), ctypes.byref(out)),
               "mxr_find_hii_bubbles")
         return out
+
+    def recombination_tables(self, params: HiiParams):
+        """mxr_recombination_tables: (lnGamma[150], ln_rate, clumping, ln_residual), the tables as
+        float64 arrays of shape (200, 100, 150) (z_ct, t_ct, ln Gamma_12 node)."""
+        x = np.zeros(150, np.float64)
+        tabs = [np.zeros((200, 100, 150), np.float64) for _ in range(3)]
+        dp = ctypes.POINTER(ctypes.c_double)
+        check(lib().mxr_recombination_tables(self._h, ctypes.byref(params), x.ctypes.data_as(dp),
+                                             *[t.ctypes.data_as(dp) for t in tabs]), "mxr_recombination_tables")
+        return (x, *tabs)
 
     def filter(self, box, local_ix_start: int, slab_nx: int, grid_dim: int, box_size: float, R: float,
                filter_type: int):
@@ -1000,13 +1023,28 @@ class HiiState:
     J_21_at_ionization: object
     r_bubble: object
     temp_kinetic_all_gas: object
+    # Flag_IncludeRecombinations only: N_rec (padded, persists, starts 0), Gamma12 (persists,
+    # starts 0), residual_xH, clumping_factor, t_resp
+    N_rec: object = None
+    Gamma12: object = None
+    residual_xH: object = None
+    clumping_factor: object = None
+    t_resp: object = None
 
     @staticmethod
-    def host(N: int) -> "HiiState":
+    def host(N: int, recombinations: bool = False) -> "HiiState":
         n = N ** 3
-        return HiiState(np.ones(n, np.float32), np.full(n, -1.0, np.float32), np.zeros(n, np.float32),
-                        np.zeros(n, np.float32), np.zeros(n, np.float32))
+        st = HiiState(np.ones(n, np.float32), np.full(n, -1.0, np.float32), np.zeros(n, np.float32),
+                      np.zeros(n, np.float32), np.zeros(n, np.float32))
+        if recombinations:
+            st.N_rec = np.zeros(N * N * 2 * (N // 2 + 1), np.float32)
+            st.Gamma12, st.residual_xH, st.clumping_factor, st.t_resp = (np.zeros(n, np.float32) for _ in range(4))
+        return st
 
     def arrays(self):
-        return dict(xH=self.xH, z_at_ionization=self.z_at_ionization, J_21_at_ionization=self.J_21_at_ionization,
-                    r_bubble=self.r_bubble, temp_kinetic_all_gas=self.temp_kinetic_all_gas)
+        d = dict(xH=self.xH, z_at_ionization=self.z_at_ionization, J_21_at_ionization=self.J_21_at_ionization,
+                 r_bubble=self.r_bubble, temp_kinetic_all_gas=self.temp_kinetic_all_gas)
+        if self.N_rec is not None:
+            d.update(N_rec=self.N_rec, Gamma12=self.Gamma12, residual_xH=self.residual_xH,
+                     clumping_factor=self.clumping_factor, t_resp=self.t_resp)
+        return d

@@ -109,6 +109,8 @@ static void mxr_fill_hii_params(mxr_hii_params* p, int snapshot)
   p->prev_redshift = run_globals.ZZ[snapshot > 0 ? snapshot - 1 : 0];
   p->TK_recfast = -1.0; /* the library evaluates T_RECFAST from the tables directory */
   p->TablesForXHeatingDir = rp->TablesForXHeatingDir;
+  p->Flag_TemperatureDependentRec = rp->Flag_TemperatureDependentRec;
+  p->Y_He = ph->Y_He;
 }
 
 /* _find_HII_bubbles (src/core/find_HII_bubbles.c:86-710) on the GPU, timed as the reference's
@@ -118,6 +120,7 @@ void find_HII_bubbles_mxr(int snapshot, timer_info* timer_total)
   reion_grids_t* g = &run_globals.reion_grids;
   const int spin = run_globals.params.Flag_IncludeSpinTemp;
   const int bh = run_globals.params.physics.Flag_BHFeedback;
+  const int rec = run_globals.params.Flag_IncludeRecombinations;
   timer_info timer;
   timer_start(&timer);
   mxr_hii_params p;
@@ -137,6 +140,11 @@ void find_HII_bubbles_mxr(int snapshot, timer_info* timer_total)
   io.storage = MXR_STORAGE_F32;
   io.effective_bhm = bh ? g->effective_bhm : NULL;
   io.effective_bhar = bh ? g->effective_bhar : NULL;
+  io.N_rec = rec ? g->N_rec : NULL;
+  io.Gamma12 = rec ? g->Gamma12 : NULL;
+  io.residual_xH = rec ? g->residual_xH : NULL;
+  io.clumping_factor = rec ? g->clumping_factor : NULL;
+  io.t_resp = rec ? g->t_resp : NULL;
   mxr_hii_globals out;
   mxr_check(mxr_find_hii_bubbles(mxr_context(p.ReionGridDim), &p, &io, &out), "mxr_find_hii_bubbles");
   g->volume_weighted_global_xH = out.volume_weighted_global_xH;
@@ -149,6 +157,12 @@ void find_HII_bubbles_mxr(int snapshot, timer_info* timer_total)
   g->mass_weighted_global_temp_kinetic_all_gas = out.mass_weighted_global_temp_kinetic_all_gas;
   g->volume_weighted_global_Gamma12 = out.volume_weighted_global_Gamma12;
   g->mass_weighted_global_Gamma12 = out.mass_weighted_global_Gamma12;
+  g->volume_weighted_global_N_rec = out.volume_weighted_global_N_rec;
+  g->volume_weighted_global_residual_xH = out.volume_weighted_global_residual_xH;
+  g->volume_weighted_global_clumping_factor = out.volume_weighted_global_clumping_factor;
+  g->mass_weighted_global_N_rec = out.mass_weighted_global_N_rec;
+  g->mass_weighted_global_residual_xH = out.mass_weighted_global_residual_xH;
+  g->mass_weighted_global_clumping_factor = out.mass_weighted_global_clumping_factor;
   timer_stop(&timer);
   timer_stop(timer_total);
   timer_gpu += timer_delta(timer);

@@ -28,6 +28,8 @@
 #define PROTONMASS 1.6726e-24
 #define HUBBLE 3.2407789e-18
 #define SEC_PER_YEAR 3.155e7
+#define SEC_PER_MEGAYEAR 3.155e13
+#define SIGMA_HI (double)(6.3e-18)
 #define TCMB 2.728
 #define MPC 3.086e24
 #define T_RE (double)(1e4)

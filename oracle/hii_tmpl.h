@@ -4,7 +4,8 @@
  * TEST INFRASTRUCTURE ONLY.
  *
  * Follows src/core/find_HII_bubbles.c:86-710 line by line for the configurations the
- * library supports (no recombinations, no mini-haloes).
+ * library supports (no mini-haloes).  With Flag_IncludeRecombinations the post-loop lookup uses
+ * the tables set by oracle_rec_set_tables.
  */
 #define CAT_(a, b) a##b
 #define CAT(a, b) CAT_(a, b)
@@ -18,8 +19,9 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
   const long Nc = (long)N * N * H;
   const long n_real = (long)N * N * N;
 
-  if (p->Flag_IncludeRecombinations)
-    return MXR_ERR_UNSUPPORTED;
+  const int recomb = p->Flag_IncludeRecombinations;
+  if (recomb && (!g->N_rec || !g->Gamma12 || !g->residual_xH || !g->clumping_factor || !g->t_resp))
+    return MXR_ERR_INVALID_ARG;
   if (p->ReionFilterType < 0 || p->ReionFilterType > 2)
     return MXR_ERR_FILTER;
 
@@ -48,16 +50,24 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
 
   const int spin = p->Flag_IncludeSpinTemp;
   const int bh = p->Flag_BHFeedback;
-  /* grids: deltax, stars, weighted_sfr, [x_e], [effective_bhm, effective_bhar] */
-  const int ngrid = 3 + (spin ? 1 : 0) + (bh ? 2 : 0);
-  const int IXE = 3, IBHM = 3 + (spin ? 1 : 0), IBHAR = 4 + (spin ? 1 : 0);
-  const float* src[6] = {g->deltax, g->stars, g->weighted_sfr, NULL, NULL, NULL};
+  /* grids: deltax, stars, weighted_sfr, [x_e], [effective_bhm, effective_bhar], [N_rec] */
+  const int ngrid = 3 + (spin ? 1 : 0) + (bh ? 2 : 0) + (recomb ? 1 : 0);
+  const int IXE = 3, IBHM = 3 + (spin ? 1 : 0), IBHAR = 4 + (spin ? 1 : 0), INREC = ngrid - 1;
+  const float* src[7] = {g->deltax, g->stars, g->weighted_sfr, NULL, NULL, NULL, NULL};
   if (spin)
     src[IXE] = g->x_e_box;
   if (bh) {
     src[IBHM] = g->effective_bhm;
     src[IBHAR] = g->effective_bhar;
   }
+  if (recomb)
+    src[INREC] = g->N_rec;
+  float* Gamma12 = g->Gamma12;
+  float* N_rec = g->N_rec;
+  /* :128-130 */
+  const float zstep = (float)(p->prev_redshift - redshift);
+  const float fabs_dtdz = (float)fabs(oracle_dtdz(p->OmegaM, p->OmegaLambda, p->Hubble_h, (float)redshift) /
+                                      p->Hubble_h);
   float* xH = g->xH;
   float* r_bubble = g->r_bubble;
   float* z_in = g->z_at_ionization;
@@ -80,8 +90,8 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
       margin[ii] = -HUGE_VAL;
 
   /* :163-229 forward transforms (out of place, padded real input) */
-  REAL* unfiltered[6] = {0};
-  REAL* filtered[6] = {0};
+  REAL* unfiltered[7] = {0};
+  REAL* filtered[7] = {0};
   REAL* rin = (REAL*)malloc(sizeof(REAL) * 2 * Nc);
   for (int gi = 0; gi < ngrid; gi++) {
     unfiltered[gi] = (REAL*)malloc(sizeof(REAL) * 2 * Nc);
@@ -163,6 +173,11 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
               if (filtered[gi][ip] < (REAL)ABS_TOL)
                 filtered[gi][ip] = 0;
             }
+          if (recomb) {
+            filtered[INREC][ip] = FMAXR(filtered[INREC][ip], (REAL)0.0);
+            if (filtered[INREC][ip] < (REAL)ABS_TOL)
+              filtered[INREC][ip] = 0;
+          }
         }
 
     /* :427-442 per-radius prefactors */
@@ -171,11 +186,13 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
     double Gamma_R_prefactor =
       (1.0 + redshift) * (1.0 + redshift) * R * UnitLength_in_cm / pixel_volume * pow(UnitLength_in_cm, -3.);
     Gamma_R_prefactor *= (UnitMass_in_g / UnitTime_in_s) / PROTONMASS * ReionNionPhotPerBary;
-    const double Gamma_R_prefactor_BH = Gamma_R_prefactor;
+    double Gamma_R_prefactor_BH = Gamma_R_prefactor;
     const float J_21_aux_constant = (float)(Gamma_R_prefactor * PLANCK * 1e21 / (4.0 * M_PI) * ReionGammaHaloBias /
                                             sfr_timescale * p->ReionAlphaUV);
     const float J_21_aux_constant_BH = (float)(Gamma_R_prefactor_BH * PLANCK * 1e21 / (4.0 * M_PI) *
                                                ReionGammaHaloBias / sfr_timescale * p->ReionAlphaUVBH);
+    Gamma_R_prefactor *= SIGMA_HI * p->ReionAlphaUV / (p->ReionAlphaUV + 2.75) * 1e12;
+    Gamma_R_prefactor_BH *= SIGMA_HI * p->ReionAlphaUVBH / (p->ReionAlphaUVBH + 2.75) * 1e12;
     const double f_coll_prefactor = ReionEfficiency * (4.0 / 3.0) * M_PI / pixel_volume / M_mean * R_cubed;
 
     /* :451-544 per-cell criterion and state */
@@ -188,7 +205,7 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
           const double density_over_mean = 1.0 + (double)filtered[0][ip];
           const double f_coll_stars = (double)filtered[1][ip] / density_over_mean;
           const double f_coll_effective_bhm = bh ? (double)filtered[IBHM][ip] / density_over_mean : 0.0;
-          const double rec = 0.0;
+          const double rec = recomb ? (double)filtered[INREC][ip] / density_over_mean : 0.0;
           double neutral_fraction;
           if (spin)
             neutral_fraction = 1.0 - (double)filtered[IXE][ip];
@@ -200,6 +217,19 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
               margin[ir] = m;
           }
           if ((f_coll_effective_bhm + f_coll_stars) * f_coll_prefactor > neutral_fraction * (1. + rec)) {
+            /* :488-506 first crossing (largest R) with recombinations */
+            if ((xH[ir] > REL_TOL) && recomb) {
+              Gamma12[ir] = (float)((REAL)filtered[2][ip] * (REAL)(float)Gamma_R_prefactor);
+              if (bh)
+                Gamma12[ir] = (float)((REAL)Gamma12[ir] + (REAL)filtered[IBHAR][ip] * (REAL)(float)Gamma_R_prefactor_BH);
+              r_bubble[ir] = (float)R;
+              if (flag_ReionUVBFlag == 2) {
+                J_21_at_ionization[ir] = (float)((REAL)filtered[1][ip] * (REAL)J_21_aux_constant);
+                if (bh)
+                  J_21_at_ionization[ir] = (float)((REAL)J_21_at_ionization[ir] +
+                                                   (REAL)filtered[IBHM][ip] * (REAL)J_21_aux_constant_BH);
+              }
+            }
             xH[ir] = 0;
           } else if (flag_last_filter_step && (xH[ir] > REL_TOL)) {
             if (spin)
@@ -240,6 +270,12 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
   double volume_weighted_global_effective_bhar = 0.0;
   double mass_weight = 0.0, mass_weighted_global_xH = 0.0, mass_weighted_global_r_bubble = 0.0;
   double mass_weighted_global_temp_kinetic_all_gas = 0.0;
+  double volume_weighted_global_N_rec = 0.0, volume_weighted_global_residual_xH = 0.0;
+  double volume_weighted_global_clumping_factor = 0.0, volume_weighted_global_Gamma12 = 0.0;
+  double mass_weighted_global_N_rec = 0.0, mass_weighted_global_residual_xH = 0.0;
+  double mass_weighted_global_clumping_factor = 0.0, mass_weighted_global_Gamma12 = 0.0;
+  const double Hubble_h = p->Hubble_h;
+  int rec_fail = 0;
   for (int ix = 0; ix < N; ix++)
     for (int iy = 0; iy < N; iy++)
       for (int iz = 0; iz < N; iz++) {
@@ -267,7 +303,36 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
         }
         volume_weighted_global_temp_kinetic_all_gas += (double)temp_kinetic_all_gas[ir];
         mass_weighted_global_temp_kinetic_all_gas += (double)temp_kinetic_all_gas[ir] * density_over_mean;
+        if (recomb) {
+          /* :612-636 */
+          double recombination_rate, rnh, cf;
+          const double z_eff = (1. + redshift) * pow(density_over_mean, 1.0 / 3.0) - 1;
+          const double temp = p->Flag_TemperatureDependentRec ? (double)temp_kinetic_all_gas[ir] : 1e4;
+          if (oracle_splined_recombination(z_eff, (double)Gamma12[ir] * Hubble_h * Hubble_h, temp,
+                                           &recombination_rate, &rnh, &cf) != 0) {
+            rec_fail = 1;
+            continue;
+          }
+          N_rec[ip] += (float)recombination_rate * fabs_dtdz * zstep * (1. - (float)cell_xH);
+          g->residual_xH[ir] = (float)rnh;
+          g->clumping_factor[ir] = (float)cf;
+          volume_weighted_global_N_rec += (double)N_rec[ip];
+          g->t_resp[ir] = Gamma12[ir] * Hubble_h * Hubble_h * 1e-12 + recombination_rate;
+          if (g->t_resp[ir] > 0)
+            g->t_resp[ir] = 1.0 / g->t_resp[ir] / SEC_PER_MEGAYEAR;
+          else
+            g->t_resp[ir] = 1e30;
+          volume_weighted_global_residual_xH += (double)g->residual_xH[ir];
+          volume_weighted_global_clumping_factor += (double)g->clumping_factor[ir];
+          volume_weighted_global_Gamma12 += (double)Gamma12[ir];
+          mass_weighted_global_N_rec += (double)N_rec[ip] * density_over_mean;
+          mass_weighted_global_residual_xH += (double)g->residual_xH[ir] * density_over_mean;
+          mass_weighted_global_clumping_factor += (double)g->clumping_factor[ir] * density_over_mean;
+          mass_weighted_global_Gamma12 += (double)Gamma12[ir] * density_over_mean;
+        }
       }
+  if (rec_fail)
+    return MXR_ERR_TABLES;
 
   clock_gettime(CLOCK_MONOTONIC, &t1);
   if (seconds)
@@ -285,6 +350,14 @@ static int CAT(hii_run_, SUFFIX)(const mxr_hii_params* p, const mxr_hii_grids* g
     out->mass_weighted_global_xH = mass_weighted_global_xH / mass_weight;
     out->mass_weighted_global_r_bubble = mass_weighted_global_r_bubble / mass_weight;
     out->mass_weighted_global_temp_kinetic_all_gas = mass_weighted_global_temp_kinetic_all_gas / mass_weight;
+    out->volume_weighted_global_N_rec = volume_weighted_global_N_rec / total_n_cells;
+    out->volume_weighted_global_residual_xH = volume_weighted_global_residual_xH / total_n_cells;
+    out->volume_weighted_global_clumping_factor = volume_weighted_global_clumping_factor / total_n_cells;
+    out->volume_weighted_global_Gamma12 = volume_weighted_global_Gamma12 / total_n_cells;
+    out->mass_weighted_global_N_rec = mass_weighted_global_N_rec / mass_weight;
+    out->mass_weighted_global_residual_xH = mass_weighted_global_residual_xH / mass_weight;
+    out->mass_weighted_global_clumping_factor = mass_weighted_global_clumping_factor / mass_weight;
+    out->mass_weighted_global_Gamma12 = mass_weighted_global_Gamma12 / mass_weight;
     out->n_radii = n_radii;
     out->device_ms = 0.0;
   }

@@ -114,6 +114,15 @@ float oracle_partially_ionized_T(float T_HI, float res_xH);
 int oracle_ts_probe(const mxr_ts_params* p, int what, const double* args, double* out);
 
 int oracle_num_threads(void);
+/* Recombinations (recomb.c): the tables find_HII_bubbles interpolates (the library's, as
+ * mxr_recombination_tables reports them: 150 ln Gamma_12 nodes, then 200 x 100 x 150 doubles
+ * each), the per-cell lookup, dt/dz, and probes of the table integrands' pieces. */
+int oracle_rec_set_tables(const double* lnGamma, const double* ln_rate, const double* clumping,
+                          const double* ln_residual);
+int oracle_splined_recombination(double z_eff, double gamma12_bg, double temp, double* recombination_rate,
+                                 double* residual_xH, double* clumping_factor);
+double oracle_dtdz(double OmegaM, double OmegaLambda, double Hubble_h, float z);
+int oracle_rec_probe(const mxr_hii_params* p, int what, const double* args, double* out);
 
 #ifdef __cplusplus
 }

@@ -92,6 +92,15 @@ def lib():
     L.oracle_ps_length.restype = ctypes.c_int
     L.oracle_power_spectrum.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, fp, fp, fp, fp, ctypes.c_int]
     L.oracle_power_spectrum.restype = ctypes.c_int
+    _dp = ctypes.POINTER(ctypes.c_double)
+    L.oracle_rec_set_tables.argtypes = [fp, fp, fp, fp]
+    L.oracle_rec_set_tables.restype = ctypes.c_int
+    L.oracle_splined_recombination.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+    L.oracle_splined_recombination.restype = ctypes.c_int
+    L.oracle_dtdz.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_float]
+    L.oracle_dtdz.restype = ctypes.c_double
+    L.oracle_rec_probe.argtypes = [ctypes.POINTER(mxr.HiiParams), ctypes.c_int, _dp, _dp]
+    L.oracle_rec_probe.restype = ctypes.c_int
     for name in ("oracle_ts_step_f32", "oracle_ts_step_f64"):
         getattr(L, name).argtypes = [ctypes.POINTER(mxr.TsParams), ctypes.POINTER(mxr.TsTables),
                                      ctypes.POINTER(mxr.TsGrids), ctypes.POINTER(mxr.TsGlobals), fp, fp]
@@ -163,6 +172,41 @@ def find_hii_bubbles(params, inputs: dict, state: "mxr.HiiState", precision="f32
     if rc:
         raise mxr.MxrError(rc, "oracle_find_hii_bubbles")
     return (out, m) if margin else out
+
+
+def rec_set_tables(lnGamma, ln_rate, clumping, ln_residual):
+    """The recombination tables the oracle's find_HII_bubbles interpolates (float64, the
+    layout mxr_recombination_tables reports)."""
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (lnGamma, ln_rate, clumping, ln_residual)]
+    assert arrs[0].size == 150 and all(a.size == 200 * 100 * 150 for a in arrs[1:])
+    rc = lib().oracle_rec_set_tables(*[_p(a) for a in arrs])
+    if rc:
+        raise mxr.MxrError(rc, "oracle_rec_set_tables")
+
+
+def splined_recombination(z_eff: float, gamma12_bg: float, temp: float):
+    """recombinations.c:35-82 -> (recombination rate, residual_xH, clumping factor)."""
+    r, x, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    rc = lib().oracle_splined_recombination(z_eff, gamma12_bg, temp, ctypes.byref(r), ctypes.byref(x),
+                                            ctypes.byref(c))
+    if rc:
+        raise mxr.MxrError(rc, "oracle_splined_recombination")
+    return r.value, x.value, c.value
+
+
+def rec_probe(params, what: int, *args):
+    """oracle_rec_probe: 0 neutral_fraction(n, T4, G12); 1 Gamma_SS(Gbg, D, T4, z); 2 (C, beta)(z);
+    3 PDF without A (D, z); 4 the four integrands (lnD, z, G12, T4, A); 5 No; 6 dtdz(z)."""
+    a = (ctypes.c_double * 8)(*args)
+    out = (ctypes.c_double * 4)()
+    rc = lib().oracle_rec_probe(ctypes.byref(params), what, a, out)
+    if rc:
+        raise mxr.MxrError(rc, "oracle_rec_probe")
+    if what == 2:
+        return out[0], out[1]
+    if what == 4:
+        return tuple(out[:4])
+    return out[0]
 
 
 def ts_probe(params, what: int, *args):

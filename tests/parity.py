@@ -24,7 +24,10 @@ MARGIN_TOL = 1e-3
 FLAG_FRAC = 5e-4
 
 
-def compare(N, gpu: dict, ref: dict, gglob, rglob, margin=None, flag_frac=FLAG_FRAC, min_cells=2):
+def compare(N, gpu: dict, ref: dict, gglob, rglob, margin=None, flag_frac=FLAG_FRAC, min_cells=2, exclude=None):
+    """`exclude` (optional mask): cells that disagreed in an earlier call of a sequence, whose
+    carried state (z_at_ionization, J_21, N_rec) keeps that difference; they are left out of the
+    per-cell comparisons of agreeing cells."""
     n = N ** 3
     ion_g = gpu["xH"] == 0
     ion_r = ref["xH"] == 0
@@ -36,6 +39,9 @@ def compare(N, gpu: dict, ref: dict, gglob, rglob, margin=None, flag_frac=FLAG_F
     if margin is not None and nm:
         stats["max_margin_mismatch"] = float(np.abs(margin[mism]).max())
     ok = ~mism
+    if exclude is not None:
+        ok &= ~exclude
+        stats["history_excluded"] = int(exclude.sum())
     dx = np.abs(gpu["xH"][ok] - ref["xH"][ok]) if ok.any() else np.zeros(1)
     stats["max_dxH_cell"] = float(dx.max())
     stats["p999_dxH_cell"] = float(np.quantile(dx, 0.999))

@@ -52,6 +52,7 @@ typedef struct run_params_t {
   int Flag_IncludeSpinTemp;
   int Flag_IncludeRecombinations;
   int Flag_EvolvingReionRBubbleMax;
+  int Flag_TemperatureDependentRec;
   int Flag_IncludePecVelsFor21cm;
   int TsNumFilterSteps;
   double ReionSfrTimescale;
@@ -82,6 +83,11 @@ typedef struct reion_grids_t {
   float* temp_kinetic_all_gas;
   float* delta_T;
   float* vel;
+  float* N_rec;
+  float* Gamma12;
+  float* residual_xH;
+  float* clumping_factor;
+  float* t_resp;
   double volume_weighted_global_xH;
   double volume_weighted_global_Gamma12;
   double volume_weighted_global_r_bubble;
@@ -92,6 +98,12 @@ typedef struct reion_grids_t {
   double mass_weighted_global_Gamma12;
   double mass_weighted_global_r_bubble;
   double mass_weighted_global_temp_kinetic_all_gas;
+  double volume_weighted_global_N_rec;
+  double volume_weighted_global_residual_xH;
+  double volume_weighted_global_clumping_factor;
+  double mass_weighted_global_N_rec;
+  double mass_weighted_global_residual_xH;
+  double mass_weighted_global_clumping_factor;
   double volume_ave_J_alpha;
   double volume_ave_xalpha;
   double volume_ave_Xheat;

@@ -258,7 +258,8 @@ def test_error_codes(ctxs):
     inp = realisation(N, 40.0, 1)
     st = _state(N)
     for over, code in ((dict(ReionFilterType=3), mxr.MXR_ERR_FILTER),
-                       (dict(Flag_IncludeRecombinations=1), mxr.MXR_ERR_UNSUPPORTED),
+                       (dict(Flag_IncludeRecombinations=1), mxr.MXR_ERR_INVALID_ARG),  # no N_rec etc.
+                       (dict(Flag_IncludeRecombinations=1, Y_He=0.0), mxr.MXR_ERR_INVALID_ARG),
                        (dict(ReionGridDim=64), mxr.MXR_ERR_INVALID_ARG)):
         p = mxr.hii_params(N, 40.0, **over)
         with pytest.raises(mxr.MxrError) as e:

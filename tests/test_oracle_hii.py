@@ -143,12 +143,12 @@ def test_fp32_restatement_against_fp64_oracle():
     assert abs(ga.volume_weighted_global_xH - gb.volume_weighted_global_xH) < 1e-5 + mism.sum() / N ** 3
 
 
-def test_unsupported_flags_are_errors():
+def test_recombinations_need_their_grids():
     N = 16
     p = mxr.hii_params(N, 20.0, Flag_IncludeRecombinations=1)
     with pytest.raises(mxr.MxrError) as e:
         O.find_hii_bubbles(p, uniform_inputs(N, 1.0), mxr.HiiState.host(N))
-    assert e.value.code == mxr.MXR_ERR_UNSUPPORTED
+    assert e.value.code == mxr.MXR_ERR_INVALID_ARG
 
 
 # ---- BH feedback (Flag_BHFeedback): effective_bhm and effective_bhar as filtered grids ----
