@@ -100,11 +100,25 @@ def cpu_baseline(N, box, inputs_host, backend, n_radii_sample, n_radii_total):
     }
 
 
-# What the SQ counters of the 512^3 workload say limits a kernel class
-# (profiles/find_hii_bubbles_512_sq_counters.md); the roofline prices bytes only.
-VALU_LIMITER = {
-    "inv_c2r_z_update": {"limited_by": "VALU issue", "source": "profiles/find_hii_bubbles_512_sq_counters.md"},
-}
+# Vector-issue peak of one MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane-instructions/s (a
+# wave64 VALU instruction occupies a SIMD16 for 4 cycles).
+VALU_PEAK = 256 * 4 * 16 * 2.4e9
+
+
+def profiled_valu(kernel_class, N):
+    """VALU lane-instructions per launch of a kernel class (SQ_INSTS_VALU x 64, weighted over the
+    forms the class launches per call) from the committed SQ-counter summary of the 512^3
+    workload, or None."""
+    path = os.path.join(REPO, "profiles", f"find_hii_bubbles_{N}_sq_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("valu-lane-instr-per-launch:"):
+                _, cls, val = line.split()
+                if cls == kernel_class:
+                    return float(val)
+    except OSError:
+        pass
+    return None
 
 
 def profiled_traffic(kernel_class, N):
@@ -652,6 +666,15 @@ def main():
     f = (N // 2 + 1) / N
     dense_z = (3 * 8 * f + 12) * N ** 3
     traffic = profiled_traffic(dom_name, N)
+    # the z pass runs out of vector issue before HBM bandwidth (SQ counters): its VALU figure is the
+    # profiled lane-instructions per launch over this run's hipEvent launch time
+    lanes = profiled_valu(dom_name, N)
+    valu = None
+    if lanes:
+        valu = {"achieved": lanes / avg_s / 1e12, "peak": VALU_PEAK / 1e12, "unit": "T lane-instr/s",
+                "frac": lanes / avg_s / VALU_PEAK, "lane_instr_per_launch": lanes,
+                "lane_instr_per_cell": lanes / N ** 3,
+                "source": f"profiles/find_hii_bubbles_{N}_sq_counters.md (SQ_INSTS_VALU x 64)"}
     kernels = {k: dict(launches=v["launches"], total_ms=round(v["total_ms"], 3),
                        GBps=round(v["alg_bytes"] / max(v["total_ms"], 1e-9) / 1e6, 1)) for k, v in ks.items()}
     log("bench: kernels " + json.dumps(kernels))
@@ -705,13 +728,13 @@ def main():
         "config": {"workload": f"find_HII_bubbles N={N} box={box} cMpc/h, {n_radii} radii R_max=50 -> cell, "
                                f"ReionFilterType=1, 3 grids, z=8, ReionEfficiency=1",
                    "grid_dim": N, "radii": n_radii, "cells": N ** 3},
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+        "roofline": {"bound": "valu" if valu else "hbm", "kernel": dom_name, "achieved": achieved / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "valu": valu,
                      "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s * 1e3, "launches": dom["launches"],
                      "bytes_model": "kept modes only: per launch the spectrum values the sharp-k window keeps "
                                     "(8 B each, read once per grid) plus 1 B of flags per cell and, on the last "
                                     "radius, the state grids; counted by the library per radius",
-                     "limiter": VALU_LIMITER.get(dom_name),
                      "call_alg_bytes": call_bytes, "call_kernel_ms": call_ms,
                      "call_frac": call_bytes / (call_ms / 1e3) / HBM_PEAK,
                      "dense_model_bytes_per_launch": dense_z,
