@@ -513,6 +513,9 @@ typedef struct mxr_ts_params {
   double NuXraySoftCut;           /* eV                                                    */
   double NuXrayMax;               /* eV                                                    */
   double SpecIndexXrayGal;
+  int Flag_IncludeAGNXray;        /* MXR_AGN_XRAY_*: AGN X-ray bands heating and ionising  */
+  double SpecIndexXrayAGNSoft;    /* soft band NuXrayThreshold..NuXraySoftCut               */
+  double SpecIndexXrayAGNHard;    /* hard band NuXraySoftCut..NuXrayMax                     */
   double UnitLength_in_cm;
   double UnitMass_in_g;
   double UnitVelocity_in_cm_per_s;
@@ -529,6 +532,9 @@ typedef struct mxr_ts_grids {
   const float* sfr_histories;     /* in  n_histories padded grids, [s] = s snapshots back;
                                      [0] is the current sfr grid (reionization.c:2106-2141) */
   const float* xray_luminosity_histories; /* in, same layout (1e40 erg/s units)             */
+  const float* bh_xray_histories_hard; /* in, same layout (1e10 L_sun per cell): read when
+                                     Flag_IncludeAGNXray is BOTH or HARD, else may be NULL  */
+  const float* bh_xray_histories_soft; /* in, same layout: read when BOTH or SOFT            */
   int n_histories;
   float* x_e_box_prev;            /* in/out padded real                                     */
   float* x_e_box;                 /* out padded real (x_e_box_prev after the step)          */
@@ -557,6 +563,16 @@ typedef struct mxr_ts_tables {
   double freq_int_heat[MXR_X_INT_NXHII][MXR_TS_MAX_STEPS];
   double freq_int_ion[MXR_X_INT_NXHII][MXR_TS_MAX_STEPS];
   double freq_int_lya[MXR_X_INT_NXHII][MXR_TS_MAX_STEPS];
+  /* AGN bands (ComputeTs.c:835-899, 1045-1107): the same integrals over each band's limits
+   * with its spectral index (zero for a band the flag leaves out), and the band prefactors
+   * (computed whatever the flag, as the reference does) */
+  double const_zp_prefactor_AGN_soft, const_zp_prefactor_AGN_hard;
+  double freq_int_heat_AGN_soft[MXR_X_INT_NXHII][MXR_TS_MAX_STEPS];
+  double freq_int_ion_AGN_soft[MXR_X_INT_NXHII][MXR_TS_MAX_STEPS];
+  double freq_int_lya_AGN_soft[MXR_X_INT_NXHII][MXR_TS_MAX_STEPS];
+  double freq_int_heat_AGN_hard[MXR_X_INT_NXHII][MXR_TS_MAX_STEPS];
+  double freq_int_ion_AGN_hard[MXR_X_INT_NXHII][MXR_TS_MAX_STEPS];
+  double freq_int_lya_AGN_hard[MXR_X_INT_NXHII][MXR_TS_MAX_STEPS];
   float x_int_XHII[MXR_X_INT_NXHII];
   int n_ops[MXR_TS_MAX_STEPS];        /* source operations before each shell's FFT         */
   int op_kind[MXR_TS_MAX_STEPS][MXR_TS_MAX_OPS];
@@ -576,16 +592,24 @@ typedef struct mxr_ts_globals {
   double XrayEmissivity_HMXB;     /* volume-averaged galaxy X-ray emissivity density of the
                                      unfiltered shell (stored_XrayEmissivity_HMXB[snapshot]);
                                      not written in the homogeneous branch               */
+  double XrayEmissivity_hard;     /* the same for the AGN bands (stored_XrayEmissivity_hard /
+                                     _soft[snapshot], erg/s/cm^3): 0 for a band the flag   */
+  double XrayEmissivity_soft;     /* leaves out; not written in the homogeneous branch     */
+  double volume_ave_Xheat_AGN_soft; /* AGN heating terms of dT_K/dz (reion_grids_t); 0 for */
+  double volume_ave_Xheat_AGN_hard; /* a band left out; not written when homogeneous      */
   double device_ms;
 } mxr_ts_globals;
 
 /* One spin-temperature step: the homogeneous RECFAST branch above ReionMaxHeatingRedshift,
- * otherwise TsNumFilterSteps shells of history-weighted SFR and X-ray luminosity grids, each
- * r2c -> window -> c2r on the GPU and kept in HBM, the host's per-shell frequency tables, and
- * the per-cell forward-Euler update of x_e and T_K with the spin temperature.  `tables` may be
- * NULL; otherwise it receives the tables used.  Replaces _ComputeTs
- * (src/core/ComputeTs.c:85-1554; entry ComputeTs, src/core/ComputeTs.h:13) for
- * Flag_IncludeAGNXray = 0, no mini-haloes, no Lyman-Werner feedback.                       */
+ * otherwise TsNumFilterSteps shells of history-weighted SFR, X-ray luminosity and (with
+ * Flag_IncludeAGNXray) AGN hard / soft X-ray emissivity grids, each r2c -> window -> c2r on
+ * the GPU and kept in HBM, the host's per-shell frequency tables, and the per-cell
+ * forward-Euler update of x_e and T_K with the spin temperature.  The AGN grids follow the
+ * history plan's loads and accumulations only: clearing and rescaling the SFR grids leaves
+ * them as they are (load_reion_bh_grids, reionization.c:2799-2870).  `tables` may be NULL;
+ * otherwise it receives the tables used.  Replaces _ComputeTs (src/core/ComputeTs.c:85-1554;
+ * entry ComputeTs, src/core/ComputeTs.h:13) without mini-haloes or Lyman-Werner feedback.
+ * A non-finite AGN prefactor is MXR_ERR_INVALID_ARG whatever the flag (the reference aborts). */
 int mxr_compute_ts(mxr_ctx* ctx, const mxr_ts_params* params, const mxr_ts_grids* grids,
                    mxr_ts_globals* globals, mxr_ts_tables* tables);
 

@@ -334,7 +334,7 @@ void mxr_destroy(mxr_ctx* h)
   (void)hipFree(c.rec_coef);
   (void)hipFree(c.rec_x);
   (void)hipFree(c.ion);
-  for (int g = 0; g < 2; g++)
+  for (int g = 0; g < 4; g++)
     (void)hipFree(c.ts_src[g]);
   (void)hipFree(c.ts_store);
   (void)hipFree(c.ts_hist);

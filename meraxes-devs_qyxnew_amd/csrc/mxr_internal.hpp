@@ -118,7 +118,7 @@ struct Ctx {
   double* sums = nullptr;          // final FP64 sums (device)
   double* sums_host = nullptr;     // pinned copy
   // spin temperature
-  float* ts_src[2] = {};           // current sfr / xray_luminosity source grids (padded real)
+  float* ts_src[4] = {};           // current sfr, xray_luminosity and AGN band source grids (padded real)
   float* ts_store = nullptr;       // smoothed shells: [grid][shell][cell] FP32, real layout
   size_t ts_store_cap = 0;         // floats
   float* ts_hist = nullptr;        // staged host histories (sfr then xray), padded grids

@@ -32,33 +32,46 @@
 namespace mxr {
 
 // ---- source grids ---------------------------------------------------------------------------
+// G source grids: 0 sfr, 1 xray_luminosity, then the AGN bands the call includes.  The AGN
+// grids take the plan's loads and accumulations only; clearing and rescaling touch the first
+// two (load_reion_bh_grids, reionization.c:2799-2870, runs beside load_reion_sfr_grids, and
+// ComputeTs.c:366-384, 418-422 clear and rescale sfr and xray_luminosity alone).
+constexpr int TS_MAXG = 4;
 struct TsOpsArgs {
   int nops;
   int init;  // start from history[0] (the current snapshot's grid) instead of the source buffer
   TsOp ops[TS_MAX_OPS];
 };
+struct TsSrcArgs {
+  const float* h[TS_MAXG];   // histories of each grid (history slot s at h[g] + s * hstride)
+  const float* h0[TS_MAXG];  // the current snapshot's grid (history 0, wherever a ring keeps it)
+  float* s[TS_MAXG];         // the source grids the shells transform
+};
 
-// hs0, hx0: the current snapshot's grids (history 0, wherever a ring keeps it)
-__global__ void k_ts_source(TsOpsArgs oa, const float* __restrict__ hs, const float* __restrict__ hx,
-                            const float* __restrict__ hs0, const float* __restrict__ hx0, long hstride,
-                            float* __restrict__ ss, float* __restrict__ sx, long n)
+template <int G>
+__global__ void k_ts_source(TsOpsArgs oa, TsSrcArgs sa, long hstride, long n)
 {
   // ops in batches of OB: the batch's history loads are issued together (the op list is
   // uniform, so the branches are scalar), then the ops are applied in order
   constexpr int OB = 8;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    float v[2] = {oa.init ? hs0[i] : ss[i], oa.init ? hx0[i] : sx[i]};
+    float v[G];
+#pragma unroll
+    for (int g = 0; g < G; g++)
+      v[g] = oa.init ? sa.h0[g][i] : sa.s[g][i];
     for (int k0 = 0; k0 < oa.nops; k0 += OB) {
-      float hv[OB][2];
+      float hv[OB][G];
 #pragma unroll
       for (int u = 0; u < OB; u++) {
-        hv[u][0] = hv[u][1] = 0.0f;
+#pragma unroll
+        for (int g = 0; g < G; g++)
+          hv[u][g] = 0.0f;
         if (k0 + u < oa.nops) {
           const TsOp o = oa.ops[k0 + u];
-          if (o.kind == TS_OP_LOAD || o.kind == TS_OP_ADD) {
-            hv[u][0] = hs[(long)o.hist * hstride + i];
-            hv[u][1] = hx[(long)o.hist * hstride + i];
-          }
+          if (o.kind == TS_OP_LOAD || o.kind == TS_OP_ADD)
+#pragma unroll
+            for (int g = 0; g < G; g++)
+              hv[u][g] = sa.h[g][(long)o.hist * hstride + i];
         }
       }
 #pragma unroll
@@ -67,26 +80,30 @@ __global__ void k_ts_source(TsOpsArgs oa, const float* __restrict__ hs, const fl
           break;
         const TsOp o = oa.ops[k0 + u];
 #pragma unroll
-        for (int g = 0; g < 2; g++) {
-          if (o.kind == TS_OP_CLEAR)
-            v[g] = 0.0f;
-          else if (o.kind == TS_OP_SCALE)
-            v[g] = (float)((double)v[g] * o.wd);
-          else if (o.kind == TS_OP_LOAD)
+        for (int g = 0; g < G; g++) {
+          if (o.kind == TS_OP_CLEAR) {
+            if (g < 2)
+              v[g] = 0.0f;
+          } else if (o.kind == TS_OP_SCALE) {
+            if (g < 2)
+              v[g] = (float)((double)v[g] * o.wd);
+          } else if (o.kind == TS_OP_LOAD) {
             v[g] = __fmul_rn(hv[u][g], o.wf);
-          else
+          } else {
             v[g] = __fadd_rn(v[g], __fmul_rn(hv[u][g], o.wf));
+          }
         }
       }
     }
-    ss[i] = v[0];
-    sx[i] = v[1];
+#pragma unroll
+    for (int g = 0; g < G; g++)
+      sa.s[g][i] = v[g];
   }
 }
 
 // ---- inverse z-pass storing one smoothed shell ------------------------------------------------
 struct TsStore {
-  float* out[2];  // sfr (clamped at 0), xray (as is); real layout
+  float* out[TS_MAXG];  // sfr and the AGN bands clamped at 0, xray as is; real layout
 };
 
 template <int N, int NT, int TR>
@@ -117,7 +134,7 @@ __global__ void __launch_bounds__(NT) k_row_c2r_ts(CGridPtrs in, long nrows, con
   row_c2r_grid<N, T, TR>(pick(in, g), valid, row, lds, tw, kc, c, t, res);
   if (!valid)
     return;
-  float* __restrict__ dst = (g == 0 ? st.out[0] : st.out[1]) + row * N;
+  float* __restrict__ dst = (g == 0 ? st.out[0] : g == 1 ? st.out[1] : g == 2 ? st.out[2] : st.out[3]) + row * N;
 #pragma unroll
   for (int q = 0; q < SL::QL; q++) {
     const int j = t + T * q;
@@ -126,8 +143,8 @@ __global__ void __launch_bounds__(NT) k_row_c2r_ts(CGridPtrs in, long nrows, con
       for (int r = 0; r < SL::RL; r++) {
         const int n = j + r * SL::BL;
         float2 y = res[q * SL::RL + r];
-        if (g == 0)
-          y = make_float2(fmaxf(y.x, 0.0f), fmaxf(y.y, 0.0f));  // ComputeTs.c:592, 703
+        if (g != 1)
+          y = make_float2(fmaxf(y.x, 0.0f), fmaxf(y.y, 0.0f));  // ComputeTs.c:592, 612-628, 703-735
         *reinterpret_cast<float2*>(dst + 2 * n) = y;
       }
   }
@@ -292,22 +309,24 @@ struct TsEvolve {
   float* xe;            // padded, in/out (x_e_box_prev)
   float* Tk;            // real, in/out
   float* TS;            // real, out
-  const float* sm;      // [2][F][N^3]: smoothed SFR then X-ray shells
+  const float* sm;      // [2 + NB][F][N^3]: smoothed SFR, X-ray, then the NB AGN bands' shells
   double* partials;
 };
 
 // The shell sums run in FP64 over the folded per-shell factors staged in LDS (6 x 14 x F for
-// the heat / ionisation / Lyman-alpha rates, F for the stellar Lyman-alpha flux): per cell
-// and shell two coalesced FP32 loads and seven FP64 FMAs.  The per-cell tail (Euler step,
-// floors, T_S) follows the reference's evaluation order.  Three waves per SIMD (168 VGPRs, a
-// few spilled) with 16 loads in flight each: 1.86 ms at 256^3 x 40 shells, against 2.18 ms at
-// two waves and 2.32 ms with 4 loads in flight.
-template <int NT>
+// the heat / ionisation / Lyman-alpha rates, F for the stellar Lyman-alpha flux, and with NB
+// AGN bands NB x 6 x 14 x F more): per cell and shell 2 + NB coalesced FP32 loads and 7 + 6 NB
+// FP64 FMAs.  The per-cell tail (Euler step, floors, T_S) follows the reference's evaluation
+// order.  Without AGN bands: three waves per SIMD (168 VGPRs, a few spilled) with 16 loads in
+// flight each, 256 threads: 1.86 ms at 256^3 x 40 shells, against 2.18 ms at two waves and
+// 2.32 ms with 4 loads in flight.  With AGN bands the tables take up to 130 KB of LDS at 64
+// shells, so one 768-thread workgroup per CU keeps the same three waves per SIMD.
+template <int NT, int NB>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3)))
 k_ts_evolve(const TsDev* __restrict__ T, TsEvolve a)
 {
 #pragma clang fp contract(off)
-  extern __shared__ double fl[];  // [6][XNH][F] then bs[F]
+  extern __shared__ double fl[];  // [6][XNH][F], bs[F], then [NB][6][XNH][F]
   const int N = a.N, H = N / 2 + 1, F = T->F;
   const long n = (long)N * N * N;
   for (int i = threadIdx.x; i < 6 * XNH * F; i += NT) {
@@ -316,12 +335,26 @@ k_ts_evolve(const TsDev* __restrict__ T, TsEvolve a)
   }
   for (int r = threadIdx.x; r < F; r += NT)
     fl[6 * XNH * F + r] = T->bs[r];
+  if constexpr (NB > 0) {
+    for (int i = threadIdx.x; i < NB * 6 * XNH * F; i += NT) {
+      const int b = i / (6 * XNH * F), k = (i / (XNH * F)) % 6, m = (i / F) % XNH, r = i % F;
+      fl[(6 * XNH + 1) * F + i] = T->fold_agn[b][k][m][r];
+    }
+  }
   __syncthreads();
   const double* __restrict__ bsl = fl + 6 * XNH * F;
+  const double* __restrict__ fag = fl + (6 * XNH + 1) * F;
   const float* __restrict__ sm_sfr = a.sm;
   const float* __restrict__ sm_xr = a.sm + (long)F * n;
+  const float* __restrict__ sm_agn = a.sm + 2L * F * n;  // band b, shell r at (b F + r) n
   const int no_light = T->no_light;
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // 0 J_alpha, 1 x_alpha, 2 Xheat, 3 Xion, 4 T_S, 5 T_K, 6 x_e, 7 HMXB emissivity, then per
+  // AGN band its heating term and its R_ct = 0 emissivity
+  constexpr int NV = 8 + 2 * NB;
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++)
+    acc[q] = 0.0;
   for (long ir = (long)blockIdx.x * NT + threadIdx.x; ir < n; ir += (long)gridDim.x * NT) {
     const long ip = (ir / N) * 2 * H + ir % N;
     const float dx = a.deltax[ip];
@@ -340,7 +373,17 @@ k_ts_evolve(const TsDev* __restrict__ T, TsEvolve a)
       m--;
     const double dxl = xcall - (double)T->xint[m];
     acc[7] += (double)sm_xr[ir] * T->cx;  // <L_X> of the unfiltered shell (:603, 677-682)
+    if constexpr (NB > 0) {
+#pragma unroll
+      for (int b = 0; b < NB; b++)  // <AGN emissivity> of the unfiltered shell (:612-628, 669-681)
+        acc[8 + NB + b] += (double)sm_agn[(long)b * F * n + ir] * T->cagn;
+    }
     double hs = 0, hb = 0, is = 0, ib = 0, ls = 0, lb = 0, st = 0;
+    // AGN: heat per band (slope, intercept), ionisation and Lyman-alpha summed over the bands
+    double ha[NB > 0 ? NB : 1][2], ia0 = 0, ia1 = 0, la0 = 0, la1 = 0;
+#pragma unroll
+    for (int b = 0; b < (NB > 0 ? NB : 1); b++)
+      ha[b][0] = ha[b][1] = 0.0;
     if (!no_light) {
       const double* __restrict__ f0 = fl + (0 * XNH + m) * F;
       const double* __restrict__ f1 = fl + (1 * XNH + m) * F;
@@ -348,9 +391,9 @@ k_ts_evolve(const TsDev* __restrict__ T, TsEvolve a)
       const double* __restrict__ f3 = fl + (3 * XNH + m) * F;
       const double* __restrict__ f4 = fl + (4 * XNH + m) * F;
       const double* __restrict__ f5 = fl + (5 * XNH + m) * F;
-      // shells in batches of SB: a batch's 2 SB loads are issued together (one round trip),
-      // then its FMAs in shell order, so the sums are those of the plain loop
-      constexpr int SB = 8;
+      // shells in batches of SB: a batch's loads are issued together (one round trip), then
+      // its FMAs in shell order, so the sums are those of the plain loop
+      constexpr int SB = NB == 0 ? 8 : NB == 1 ? 4 : 2;
       auto add = [&](int r, double xr, double sf) {
         hs = fma(xr, f0[r], hs);
         hb = fma(xr, f1[r], hb);
@@ -360,27 +403,57 @@ k_ts_evolve(const TsDev* __restrict__ T, TsEvolve a)
         lb = fma(xr, f5[r], lb);
         st = fma(sf, bsl[r], st);
       };
+      auto add_agn = [&](int b, int r, double xb) {
+        const double* __restrict__ fb = fag + ((long)(b * 6) * XNH + m) * F + r;
+        ha[b][0] = fma(xb, fb[0 * XNH * F], ha[b][0]);
+        ha[b][1] = fma(xb, fb[1 * XNH * F], ha[b][1]);
+        ia0 = fma(xb, fb[2 * XNH * F], ia0);
+        ia1 = fma(xb, fb[3 * XNH * F], ia1);
+        la0 = fma(xb, fb[4 * XNH * F], la0);
+        la1 = fma(xb, fb[5 * XNH * F], la1);
+      };
       int r = 0;
       for (; r + SB <= F; r += SB) {
-        float xv[SB], sv[SB];
+        float xv[SB], sv[SB], av[NB > 0 ? NB : 1][SB];
 #pragma unroll
         for (int u = 0; u < SB; u++) {
           xv[u] = sm_xr[(long)(r + u) * n + ir];
           sv[u] = sm_sfr[(long)(r + u) * n + ir];
+          if constexpr (NB > 0) {
+#pragma unroll
+            for (int b = 0; b < NB; b++)
+              av[b][u] = sm_agn[((long)b * F + r + u) * n + ir];
+          }
         }
 #pragma unroll
-        for (int u = 0; u < SB; u++)
+        for (int u = 0; u < SB; u++) {
           add(r + u, (double)xv[u], (double)sv[u]);
+          if constexpr (NB > 0) {
+#pragma unroll
+            for (int b = 0; b < NB; b++)
+              add_agn(b, r + u, (double)av[b][u]);
+          }
+        }
       }
-      for (; r < F; r++)
+      for (; r < F; r++) {
         add(r, (double)sm_xr[(long)r * n + ir], (double)sm_sfr[(long)r * n + ir]);
+        if constexpr (NB > 0) {
+#pragma unroll
+          for (int b = 0; b < NB; b++)
+            add_agn(b, r, (double)sm_agn[((long)b * F + r) * n + ir]);
+        }
+      }
     }
     const double x_e = xe0, Td = tk;
     const double zpd = (double)T->zpf;
     const double n_b = (T->N_b0 * T->nb3) * (double)__fadd_rn(1.0f, dx);
     const double dheat = (hs * dxl + hb) * T->prefac_gal;
-    const double dion = (is * dxl + ib) * T->prefac_gal;
-    const double dlya = (ls * dxl + lb) * (T->prefac_gal * n_b);
+    double dion = (is * dxl + ib) * T->prefac_gal;
+    double dlya = (ls * dxl + lb) * (T->prefac_gal * n_b);
+    if constexpr (NB > 0) {  // the bands' prefactors are in their folds (XRayHeatingFunctions.c:1466-1472)
+      dion = dion + (ia0 * dxl + ia1);
+      dlya = dlya + (la0 * dxl + la1) * n_b;
+    }
     const double dstar = st * T->conv;
     const double sink = alpha_A(Td) * CLUMPING_FACTOR * x_e * x_e * T->f_H * n_b;
     const double dxe_dzp = T->dt_dzp * (dion - sink);
@@ -391,7 +464,17 @@ k_ts_evolve(const TsDev* __restrict__ T, TsEvolve a)
     dadia *= (2.0 / 3.0) * Td;
     const double dspec = -dxe_dzp * Td * i1x;
     const double dcomp = T->c_comp * (x_e / ((1.0 + x_e) + T->f_He)) * (T->Trad - Td);
-    const double dxheat_dzp = dheat * T->c_heat * i1x;
+    double dxheat_dzp = dheat * T->c_heat * i1x;
+    if constexpr (NB > 0) {  // deriv[5], deriv[6] (:1517-1529)
+      double dsum = 0.0;
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        const double dA = (ha[b][0] * dxl + ha[b][1]) * T->c_heat * i1x;
+        acc[8 + b] += dA;
+        dsum = b == 0 ? dA : dsum + dA;
+      }
+      dxheat_dzp = dxheat_dzp + dsum;
+    }
     const double d1 = ((dxheat_dzp + dcomp) + dspec) + dadia;
     const double d2 = dlya + dstar;
     const double d4 = T->dt_dzp * dion;
@@ -418,7 +501,7 @@ k_ts_evolve(const TsDev* __restrict__ T, TsEvolve a)
     acc[5] += (double)tk;
     acc[6] += (double)xe1;
   }
-  block_sum<NT, 8>(acc, a.partials + (long)blockIdx.x * 8);
+  block_sum<NT, NV>(acc, a.partials + (long)blockIdx.x * NV);
 }
 
 // homogeneous branch (ComputeTs.c:252-287) with the final averages
@@ -447,14 +530,14 @@ __global__ void __launch_bounds__(NT) k_ts_homog(const TsDev* __restrict__ T, Ts
 }
 
 // ---- driver -----------------------------------------------------------------------------------
-static int ts_ensure(Ctx* c, int F)
+static int ts_ensure(Ctx* c, int F, int G)
 {
   const long npad = 2 * n_complex(c->N), nreal = n_real(c->N);
-  RC(ensure_buffers(c, F > 0 ? 2 : 0));
-  for (int g = 0; g < 2 && F > 0; g++)
+  RC(ensure_buffers(c, F > 0 ? G : 0));
+  for (int g = 0; g < G && F > 0; g++)
     if (!c->ts_src[g])
       MXR_HIP(hipMalloc(&c->ts_src[g], sizeof(float) * npad));
-  const size_t need = (size_t)2 * F * nreal;
+  const size_t need = (size_t)G * F * nreal;
   if (F > 0 && c->ts_store_cap < need) {
     (void)hipFree(c->ts_store);
     c->ts_store = nullptr;
@@ -482,9 +565,25 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
   TsPlan pl;
   RC(ts_plan(p, &pl));
   const int F = p->TsNumFilterSteps;
+  // source grids: 0 sfr, 1 xray_luminosity, then the AGN soft and hard bands the flag includes
+  const int NB = pl.homogeneous ? 0 : pl.agn_soft + pl.agn_hard;
+  const int G = 2 + NB;
+  const float* hist_in[TS_MAXG] = {g->sfr_histories, g->xray_luminosity_histories, nullptr, nullptr};
+  {
+    int b = 2;
+    if (pl.agn_soft && !pl.homogeneous)
+      hist_in[b++] = g->bh_xray_histories_soft;
+    if (pl.agn_hard && !pl.homogeneous)
+      hist_in[b++] = g->bh_xray_histories_hard;
+  }
   if (!pl.homogeneous) {
-    if (!g->sfr_histories || !g->xray_luminosity_histories || g->n_histories <= pl.hist_max ||
-        g->n_histories < 1) {
+    for (int k = 0; k < G; k++)
+      if (!hist_in[k]) {
+        set_error("ComputeTs: the %s histories are NULL",
+                  k == 0 ? "sfr" : k == 1 ? "xray_luminosity" : "AGN X-ray emissivity");
+        return MXR_ERR_INVALID_ARG;
+      }
+    if (g->n_histories <= pl.hist_max || g->n_histories < 1) {
       set_error("ComputeTs: the history plan reads history %d but %d histories were given", pl.hist_max,
                 g->n_histories);
       return MXR_ERR_INVALID_ARG;
@@ -494,7 +593,7 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
       return MXR_ERR_INVALID_ARG;
     }
   }
-  RC(ts_ensure(c, pl.homogeneous ? 0 : F));
+  RC(ts_ensure(c, pl.homogeneous ? 0 : F, G));
   TsDev* hd = reinterpret_cast<TsDev*>(c->ts_dev_host);
   const TsDev* dd = reinterpret_cast<const TsDev*>(c->ts_dev);
 
@@ -515,8 +614,7 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
   float* Tk = g->Tk_box;
   float* TS = g->TS_box;
   float* xe_out = g->x_e_box;
-  const float* hs = g->sfr_histories;
-  const float* hx = g->xray_luminosity_histories;
+  const float* hist[TS_MAXG] = {hist_in[0], hist_in[1], hist_in[2], hist_in[3]};
   int hist_head = pl.homogeneous ? 0 : g->history_head;  // ring slot of history 0 on the device
   const int nstage = pl.homogeneous ? 0 : pl.hist_max + 1 > 1 ? pl.hist_max + 1 : 1;
   if (g->mem == MXR_MEM_HOST) {
@@ -543,7 +641,7 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
       MXR_HIP(hipMalloc(&c->out_stage[0], sizeof(float) * npad));
     xe_out = c->out_stage[0];
     if (!pl.homogeneous) {
-      const size_t need = (size_t)2 * nstage * npad;
+      const size_t need = (size_t)G * nstage * npad;
       if (c->ts_hist_cap < need) {
         (void)hipFree(c->ts_hist);
         c->ts_hist = nullptr;
@@ -552,21 +650,25 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
         c->ts_hist_cap = need;
       }
       // staged in logical order (history s at s), whatever ring slot holds it
-      for (int s = 0; s < nstage; s++) {
-        const size_t src = (size_t)((g->history_head + s) % g->n_histories) * npad;
-        MXR_HIP(hipMemcpyAsync(c->ts_hist + (size_t)s * npad, hs + src, sizeof(float) * npad, hipMemcpyHostToDevice,
-                               c->stream));
-        MXR_HIP(hipMemcpyAsync(c->ts_hist + (size_t)(nstage + s) * npad, hx + src, sizeof(float) * npad,
-                               hipMemcpyHostToDevice, c->stream));
+      for (int k = 0; k < G; k++) {
+        for (int s = 0; s < nstage; s++) {
+          const size_t src = (size_t)((g->history_head + s) % g->n_histories) * npad;
+          MXR_HIP(hipMemcpyAsync(c->ts_hist + ((size_t)k * nstage + s) * npad, hist[k] + src, sizeof(float) * npad,
+                                 hipMemcpyHostToDevice, c->stream));
+        }
+        hist[k] = c->ts_hist + (size_t)k * nstage * npad;
       }
-      hs = c->ts_hist;
-      hx = c->ts_hist + (size_t)nstage * npad;
       hist_head = 0;
     }
   }
 
-  const float* hs0 = hs + (size_t)hist_head * npad;
-  const float* hx0 = hx + (size_t)hist_head * npad;
+  // the histories' stride: a staged copy holds nstage grids, a caller's array n_histories
+  TsSrcArgs sa{};
+  for (int k = 0; k < G && !pl.homogeneous; k++) {
+    sa.h[k] = hist[k];
+    sa.h0[k] = hist[k] + (size_t)hist_head * npad;
+    sa.s[k] = c->ts_src[k];
+  }
   const int nblk = (int)ew_blocks(nreal);
   double* sums = c->sums_host;
   MXR_HIP(hipEventRecord(c->ev0, c->stream));
@@ -638,7 +740,7 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
       CRealPtrs rin{};
       GridPtrs spec{}, work{};
       CGridPtrs cspec{}, cwork{};
-      for (int i = 0; i < 2; i++) {
+      for (int i = 0; i < G; i++) {
         spec.p[i] = c->spec[i];
         work.p[i] = c->work[i];
         cspec.p[i] = c->spec[i];
@@ -661,20 +763,24 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
                 oa.ops[k].hist = (hist_head + oa.ops[k].hist) % g->n_histories;
               nread += (pl.ops[r][k].kind >= TS_OP_LOAD) ? 1 : 0;
             }
-            RC(T(K_TS_SRC, 2.0 * 4.0 * npad * (nread + 1), [&] {
-              hipLaunchKernelGGL(k_ts_source, dim3(ew_blocks(npad)), dim3(256), 0, c->stream, oa, hs, hx, hs0, hx0,
-                                 npad, c->ts_src[0], c->ts_src[1], npad);
+            RC(T(K_TS_SRC, G * 4.0 * npad * (nread + 1), [&] {
+              if (G == 2)
+                hipLaunchKernelGGL(k_ts_source<2>, dim3(ew_blocks(npad)), dim3(256), 0, c->stream, oa, sa, npad, npad);
+              else if (G == 3)
+                hipLaunchKernelGGL(k_ts_source<3>, dim3(ew_blocks(npad)), dim3(256), 0, c->stream, oa, sa, npad, npad);
+              else
+                hipLaunchKernelGGL(k_ts_source<4>, dim3(ew_blocks(npad)), dim3(256), 0, c->stream, oa, sa, npad, npad);
               MXR_HIP(hipGetLastError());
               return MXR_OK;
             }));
             src_valid = true;
           }
-          rin.p[0] = src_valid ? c->ts_src[0] : hs0;
-          rin.p[1] = src_valid ? c->ts_src[1] : hx0;
-          RC(T(K_TS_FWD, 2.0 * 3 * 2 * cplx, [&] {
-            RC(K::r2c(c, rin, spec, 2));
-            RC((K::template col<-1, COL_PLAIN>(c, cspec, spec, 2, K::ygeo(), none, 1.0)));
-            return K::template col<-1, COL_FSCALE>(c, cspec, spec, 2, K::xgeo(), none, total_n_cells);
+          for (int k = 0; k < G; k++)
+            rin.p[k] = src_valid ? c->ts_src[k] : sa.h0[k];
+          RC(T(K_TS_FWD, G * 3 * 2 * cplx, [&] {
+            RC(K::r2c(c, rin, spec, G));
+            RC((K::template col<-1, COL_PLAIN>(c, cspec, spec, G, K::ygeo(), none, 1.0)));
+            return K::template col<-1, COL_FSCALE>(c, cspec, spec, G, K::xgeo(), none, total_n_cells);
           }));
         }
         fa.R = (float)pl.R[r];
@@ -682,14 +788,16 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
         const KeptModes km = (c->prof && fa.apply && fa.prune) ? kept_modes(N, fa.dk, fa.R) : KeptModes::all(N);
         const int kc = kept_kz(fa);
         fa.live = (kc < N / 2 + 1 && c->live_tiles) ? kc : 0;  // live-tile x / y grids
-        TsStore st{{c->ts_store + (size_t)r * nreal, c->ts_store + ((size_t)F + r) * nreal}};
-        RC(T(K_TS_INV_X, 2 * 8.0 * ((double)km.k3 + (double)N * km.c2),
-             [&] { return K::template col<+1, COL_FILTER>(c, cspec, work, 2, K::xgeo(), fa, 1.0, c->col_tc_x); }));
-        RC(T(K_TS_INV_Y, 2 * 8.0 * ((double)N * km.c2 + (double)N * N * km.k1),
-             [&] { return K::template col<+1, COL_YPRUNE>(c, cwork, work, 2, K::ygeo(), fa, 1.0, c->col_tc_y); }));
-        RC(T(K_TS_INV_Z, 2 * (8.0 * (double)N * N * km.k1 + 4.0 * nreal), [&] {
+        TsStore st{};
+        for (int k = 0; k < G; k++)
+          st.out[k] = c->ts_store + ((size_t)k * F + r) * nreal;
+        RC(T(K_TS_INV_X, G * 8.0 * ((double)km.k3 + (double)N * km.c2),
+             [&] { return K::template col<+1, COL_FILTER>(c, cspec, work, G, K::xgeo(), fa, 1.0, c->col_tc_x); }));
+        RC(T(K_TS_INV_Y, G * 8.0 * ((double)N * km.c2 + (double)N * N * km.k1),
+             [&] { return K::template col<+1, COL_YPRUNE>(c, cwork, work, G, K::ygeo(), fa, 1.0, c->col_tc_y); }));
+        RC(T(K_TS_INV_Z, G * (8.0 * (double)N * N * km.k1 + 4.0 * nreal), [&] {
           const long nrows = (long)N * N;
-          dim3 grid((unsigned)((nrows + Cfg<N>::TR - 1) / Cfg<N>::TR), 2u);
+          dim3 grid((unsigned)((nrows + Cfg<N>::TR - 1) / Cfg<N>::TR), (unsigned)G);
           hipLaunchKernelGGL((k_row_c2r_ts<N, Cfg<N>::ROW_NT, Cfg<N>::TR>), grid, dim3(Cfg<N>::ROW_NT), 0,
                              c->stream, cwork, nrows, c->tw, fa, st);
           MXR_HIP(hipGetLastError());
@@ -705,15 +813,45 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
       set_error("%s", terr.c_str());
       return trc;
     }
+    if (hd->nagn != NB) {
+      set_error("ComputeTs: %d AGN bands in the tables, %d in the plan", hd->nagn, NB);
+      return MXR_ERR_INVALID_ARG;
+    }
     MXR_HIP(hipMemcpyAsync(c->ts_dev, hd, sizeof(TsDev), hipMemcpyHostToDevice, c->stream));
     TsEvolve ea{N, dx, xe, Tk, TS, c->ts_store, c->partials};
-    const size_t lds = sizeof(double) * (6 * XNH + 1) * (size_t)F;
-    RC(T(K_TS_EVOLVE, (double)nreal * (4 + 8 + 8 + 4 + (hd->no_light ? 4 : 8.0 * F)), [&] {
-      hipLaunchKernelGGL((k_ts_evolve<NT>), dim3(nblk), dim3(NT), lds, c->stream, dd, ea);
-      MXR_HIP(hipGetLastError());
-      return launch_final_sum(c, c->partials, nblk, 8, c->sums);
-    }));
-    MXR_HIP(hipMemcpyAsync(sums, c->sums, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream));
+    const size_t lds = sizeof(double) * ((6 * XNH + 1) + (size_t)NB * 6 * XNH) * (size_t)F;
+    const int nv = 8 + 2 * NB;
+    // with AGN bands: 768-thread workgroups, one per CU (their tables fill most of the LDS)
+    constexpr int NTA = 768;
+    const int nblk_ev = NB == 0 ? nblk : (int)std::min<long>((nreal + NTA - 1) / NTA, 1024);
+    RC(T(K_TS_EVOLVE,
+         (double)nreal * (4 + 8 + 8 + 4 + (hd->no_light ? 4.0 * (1 + NB) : 4.0 * G * F)), [&] {
+           if (NB == 0) {
+             hipLaunchKernelGGL((k_ts_evolve<NT, 0>), dim3(nblk_ev), dim3(NT), lds, c->stream, dd, ea);
+           } else {
+             int lmax = 0;
+             MXR_HIP(hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+             if ((size_t)lmax < lds + 1024) {
+               set_error("ComputeTs: the AGN bands' tables need %zu bytes of LDS at %d shells, the device has %d",
+                         lds, F, lmax);
+               return MXR_ERR_UNSUPPORTED;
+             }
+             if (NB == 1) {
+               if (hipFuncSetAttribute((const void*)k_ts_evolve<NTA, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds) != hipSuccess)
+                 (void)hipGetLastError();  // the launch below reports a size the device refuses
+               hipLaunchKernelGGL((k_ts_evolve<NTA, 1>), dim3(nblk_ev), dim3(NTA), lds, c->stream, dd, ea);
+             } else {
+               if (hipFuncSetAttribute((const void*)k_ts_evolve<NTA, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds) != hipSuccess)
+                 (void)hipGetLastError();  // the launch below reports a size the device refuses
+               hipLaunchKernelGGL((k_ts_evolve<NTA, 2>), dim3(nblk_ev), dim3(NTA), lds, c->stream, dd, ea);
+             }
+           }
+           MXR_HIP(hipGetLastError());
+           return launch_final_sum(c, c->partials, nblk_ev, nv, c->sums);
+         }));
+    MXR_HIP(hipMemcpyAsync(sums, c->sums, sizeof(double) * nv, hipMemcpyDeviceToHost, c->stream));
   }
   // x_e_box = x_e_box_prev (ComputeTs.c:1459), the whole padded array
   MXR_HIP(hipMemcpyAsync(xe_out, xe, sizeof(float) * npad, hipMemcpyDeviceToDevice, c->stream));
@@ -741,6 +879,13 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
     out->volume_ave_TK = sums[5] / total_n_cells;
     out->volume_ave_xe = sums[6] / total_n_cells;
     out->XrayEmissivity_HMXB = sums[7] / total_n_cells;
+    out->XrayEmissivity_soft = out->XrayEmissivity_hard = 0.0;
+    out->volume_ave_Xheat_AGN_soft = out->volume_ave_Xheat_AGN_hard = 0.0;
+    for (int b = 0; b < NB; b++) {
+      const bool soft = hd->agn_kind[b] == 0;
+      (soft ? out->volume_ave_Xheat_AGN_soft : out->volume_ave_Xheat_AGN_hard) = sums[8 + b] / total_n_cells;
+      (soft ? out->XrayEmissivity_soft : out->XrayEmissivity_hard) = sums[8 + NB + b] / total_n_cells;
+    }
   }
   out->device_ms = ms;
   return MXR_OK;

@@ -17,6 +17,7 @@ constexpr int KAP_N = 27;                  // kappa_10 knots (XRayHeatingFunctio
 constexpr double SPEED_OF_LIGHT = 2.9979e10;
 constexpr double BOLTZMANN = 1.3806e-16;
 constexpr double XRAY_LUMINOSITY_UNIT = 1.0e40;
+constexpr double SOLAR_LUM = 3.826e33;  // erg/s (src/meraxes.h:23)
 constexpr double NU_over_EV = 1.60217646e-12 / PLANCK;
 constexpr double NUIONIZATION = 13.60 * NU_over_EV;
 constexpr double HeI_NUIONIZATION = 24.59 * NU_over_EV;
@@ -59,6 +60,7 @@ struct TsCosmo {
 // shell geometry, history plan and the scalars that precede the GPU work
 struct TsPlan {
   int F = 0;
+  int agn_soft = 0, agn_hard = 0;  // AGN bands the flag asks for (ComputeTs.c:159-162)
   int homogeneous = 0;
   double zp = 0, dzp = 0, dt_dzp = 0;
   double R[TS_MAXF], zpp_edge[TS_MAXF], zpp[TS_MAXF];
@@ -91,6 +93,15 @@ struct TsDev {
   double fold[6][XNH][TS_MAXF];
   double bs[TS_MAXF];
   double cx;                       // 1e40 UnitLength^-3 / V_pix: X-ray field -> emissivity density
+  // AGN X-ray bands (ComputeTs.c:835-899, 1157-1340): nagn of them, band b of kind agn_kind[b]
+  // (0 soft, 1 hard).  fold_agn[b] is laid out as fold with the band's (1 + z'')^-alpha, its
+  // unit constant cagn and its prefactor const_zp_prefactor_AGN_* folded in, so the heat,
+  // ionisation and Lyman-alpha rates of band b are sum_r B_b,r (fold_agn[b][2k][m][r] dx +
+  // fold_agn[b][2k+1][m][r]) with B_b,r the smoothed emissivity field (1e10 L_sun per cell)
+  int nagn;
+  int agn_kind[2];
+  double cagn;                     // 1e10 SOLAR_LUM / V_pix UnitLength^-3
+  double fold_agn[2][6][XNH][TS_MAXF];
   float xint[XNH];
   // kappa splines in log-log (x, y, c = S''/2): kappa_10 (27), e-H (20), p-H (17)
   int kn[3];

@@ -2,7 +2,7 @@
 // plan (src/core/ComputeTs.c:32-83, 339-478), the per-shell frequency tables (:761-1011), the
 // luminosity prefactors (:1013-1128) and the helper physics of
 // src/core/XRayHeatingFunctions.c.  Everything here is grid-independent; the kernels that
-// use it live in mxr_hii.hip.
+// use it live in mxr_ts.hip.
 //
 // Arithmetic follows the reference's C types: functions the reference declares with a float
 // argument (dtdz, hubble, drdz, zmax) take float here, `1 + z` with a float z is a float sum,
@@ -623,6 +623,12 @@ int ts_plan(const mxr_ts_params* p, TsPlan* pl)
   pl->F = F;
   pl->hist_max = -1;
   std::memset(pl->nops, 0, sizeof(pl->nops));
+  if (p->Flag_IncludeAGNXray < MXR_AGN_XRAY_NONE || p->Flag_IncludeAGNXray > MXR_AGN_XRAY_SOFT) {
+    set_error("ComputeTs: Flag_IncludeAGNXray %d is not one of 0..3", p->Flag_IncludeAGNXray);
+    return MXR_ERR_INVALID_ARG;
+  }
+  pl->agn_soft = p->Flag_IncludeAGNXray == MXR_AGN_XRAY_BOTH || p->Flag_IncludeAGNXray == MXR_AGN_XRAY_SOFT;
+  pl->agn_hard = p->Flag_IncludeAGNXray == MXR_AGN_XRAY_BOTH || p->Flag_IncludeAGNXray == MXR_AGN_XRAY_HARD;
   const double redshift = ZZ[snapshot];
   const double prev_redshift = snapshot == 0 ? ZZ[snapshot] : ZZ[snapshot - 1];
   const double zp = redshift;
@@ -979,6 +985,39 @@ int ts_tables(const mxr_ts_params* p, const TsPlan& pl, double collapse_fraction
       ion[(size_t)m * F + r] = ui[k];
       lya[(size_t)m * F + r] = ul[k];
     }
+  // AGN bands (ComputeTs.c:835-899): the limits scale with (1 + zp) / (1 + zpp), so every
+  // shell has its own; soft = NuXrayThreshold..NuXraySoftCut, hard = NuXraySoftCut..NuXrayMax,
+  // both floored at the shell's tau = 1 frequency
+  const int agn_on[2] = {pl.agn_soft, pl.agn_hard};
+  const double agn_alpha[2] = {p->SpecIndexXrayAGNSoft, p->SpecIndexXrayAGNHard};
+  std::vector<double> agn_tab[2][3];
+  for (int b = 0; b < 2; b++)
+    for (int k = 0; k < 3; k++)
+      agn_tab[b][k].assign((size_t)XNH * F, 0.0);
+  {
+    std::vector<int> tasks;  // (band, shell, bin) as band * F * XNH + r * XNH + m
+    for (int b = 0; b < 2; b++)
+      if (agn_on[b])
+        for (int i = 0; i < F * XNH; i++)
+          tasks.push_back(b * F * XNH + i);
+    parallel((int)tasks.size(), [&](int t) {
+      const int b = tasks[t] / (F * XNH), r = (tasks[t] / XNH) % F, m = tasks[t] % XNH;
+      const double prev_zpp = r == 0 ? zp : pl.zpp_edge[r - 1];
+      const double zpp = (pl.zpp_edge[r] + prev_zpp) * 0.5;
+      double lo, hi;
+      if (b == 0) {
+        lo = std::fmax(ntau[r], p->NuXrayThreshold * NU_over_EV * (1. + zp) / (1. + zpp));
+        hi = p->NuXraySoftCut * NU_over_EV * (1. + zp) / (1. + zpp);
+      } else {
+        lo = std::fmax(ntau[r], p->NuXraySoftCut * NU_over_EV * (1. + zp) / (1. + zpp));
+        hi = p->NuXrayMax * NU_over_EV * (1. + zp) / (1. + zpp);
+      }
+      const double xm = d->x_int_XHII[m];
+      for (int k = 0; k < 3; k++)
+        agn_tab[b][k][(size_t)m * F + r] =
+          integrate_over_nu(d, cz, zp, xm, lo, hi, p->NuXrayThreshold, agn_alpha[b], k);
+    });
+  }
   if (filling < 0)
     filling = 0;
 
@@ -1043,6 +1082,31 @@ int ts_tables(const mxr_ts_params* p, const TsPlan& pl, double collapse_fraction
   }
   L *= 1.0 / PLANCK;
   const double prefac = L / (p->NuXrayThreshold * NU_over_EV) * SPEED_OF_LIGHT * std::pow(1 + zp, a + 3);
+  // the AGN bands' conversions C = L0 / L_band, / PLANCK, and A_b = C / (h nu_th) c (1 + zp)^(alpha_b + 3),
+  // computed (and checked) whatever the flag (ComputeTs.c:1045-1128)
+  double prefac_agn[2];
+  for (int b = 0; b < 2; b++) {
+    const double ab = agn_alpha[b];
+    const double nlo = b == 0 ? p->NuXrayThreshold : p->NuXraySoftCut;
+    const double nhi = b == 0 ? p->NuXraySoftCut : p->NuXrayMax;
+    double C;
+    if (std::fabs(ab - 1.0) < REL_TOL) {
+      C = (p->NuXrayThreshold * NU_over_EV) * std::log(nhi / nlo);
+      C = 1.0 / C;
+    } else {
+      C = std::pow(nhi * NU_over_EV, 1.0 - ab) - std::pow(nlo * NU_over_EV, 1.0 - ab);
+      C = 1.0 / C;
+      C *= std::pow(p->NuXrayThreshold * NU_over_EV, -ab) * (1.0 - ab);
+    }
+    C /= (PLANCK);
+    prefac_agn[b] = C / (p->NuXrayThreshold * NU_over_EV) * SPEED_OF_LIGHT * std::pow(1.0 + zp, ab + 3.0);
+    if (!std::isfinite(prefac_agn[b])) {
+      set_error("const_zp_prefactor_AGN_%s is not finite: check NuXrayThreshold / NuXraySoftCut / NuXrayMax / "
+                "SpecIndexXrayAGN%s",
+                b == 0 ? "soft" : "hard", b == 0 ? "Soft" : "Hard");
+      return MXR_ERR_INVALID_ARG;
+    }
+  }
   dev->growth = growth;
   dev->dgrowth = dgrowth;
   dev->dt_dzp = dt_dzp;
@@ -1054,6 +1118,11 @@ int ts_tables(const mxr_ts_params* p, const TsPlan& pl, double collapse_fraction
   const double zpf = (double)dev->zpf;
   dev->cx = XRAY_LUMINOSITY_UNIT * dev->ULm3 / dev->pv;
   const double csfr = dev->UMUT * dev->ULm3 / SOLAR_MASS / dev->pv;
+  dev->cagn = 1e10 * SOLAR_LUM / dev->pv * dev->ULm3;  // ComputeTs.c:617-618
+  dev->nagn = 0;
+  for (int b = 0; b < 2; b++)
+    if (agn_on[b])
+      dev->agn_kind[dev->nagn++] = b;
   for (int r = 0; r < F; r++) {
     double zpp, dzpp;
     if (r == 0) {
@@ -1075,6 +1144,15 @@ int ts_tables(const mxr_ts_params* p, const TsPlan& pl, double collapse_fraction
         dev->fold[2 * k][m][r] = wx * (mh != m ? (hi - lo) / dx : 0.0);
         dev->fold[2 * k + 1][m][r] = wx * lo;
       }
+      for (int j = 0; j < dev->nagn; j++) {
+        const int b = dev->agn_kind[j];
+        const double wb = dt_dzpp * dzpp * std::pow(1 + zpp, -agn_alpha[b]) * dev->cagn * prefac_agn[b];
+        for (int k = 0; k < 3; k++) {
+          const double lo = agn_tab[b][k][(size_t)m * F + r], hi = agn_tab[b][k][(size_t)mh * F + r];
+          dev->fold_agn[j][2 * k][m][r] = wb * (mh != m ? (hi - lo) / dx : 0.0);
+          dev->fold_agn[j][2 * k + 1][m][r] = wb * lo;
+        }
+      }
     }
   }
 
@@ -1090,6 +1168,8 @@ int ts_tables(const mxr_ts_params* p, const TsPlan& pl, double collapse_fraction
     out->growth_factor_zp = growth;
     out->dgrowth_factor_dzp = dgrowth;
     out->const_zp_prefactor_GAL = prefac;
+    out->const_zp_prefactor_AGN_soft = prefac_agn[0];
+    out->const_zp_prefactor_AGN_hard = prefac_agn[1];
     out->collapse_fraction = collapse_fraction;
     out->x_e_ave = x_e_ave;
     out->filling_factor_of_HI_zp = filling_raw;
@@ -1103,6 +1183,12 @@ int ts_tables(const mxr_ts_params* p, const TsPlan& pl, double collapse_fraction
         out->freq_int_heat[m][r] = heat[(size_t)m * F + r];
         out->freq_int_ion[m][r] = ion[(size_t)m * F + r];
         out->freq_int_lya[m][r] = lya[(size_t)m * F + r];
+        out->freq_int_heat_AGN_soft[m][r] = agn_tab[0][0][(size_t)m * F + r];
+        out->freq_int_ion_AGN_soft[m][r] = agn_tab[0][1][(size_t)m * F + r];
+        out->freq_int_lya_AGN_soft[m][r] = agn_tab[0][2][(size_t)m * F + r];
+        out->freq_int_heat_AGN_hard[m][r] = agn_tab[1][0][(size_t)m * F + r];
+        out->freq_int_ion_AGN_hard[m][r] = agn_tab[1][1][(size_t)m * F + r];
+        out->freq_int_lya_AGN_hard[m][r] = agn_tab[1][2][(size_t)m * F + r];
       }
       out->n_ops[r] = pl.nops[r];
       for (int k = 0; k < pl.nops[r]; k++) {

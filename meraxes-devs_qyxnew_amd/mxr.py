@@ -275,6 +275,9 @@ class TsParams(ctypes.Structure):
         ("NuXraySoftCut", ctypes.c_double),
         ("NuXrayMax", ctypes.c_double),
         ("SpecIndexXrayGal", ctypes.c_double),
+        ("Flag_IncludeAGNXray", ctypes.c_int),
+        ("SpecIndexXrayAGNSoft", ctypes.c_double),
+        ("SpecIndexXrayAGNHard", ctypes.c_double),
         ("UnitLength_in_cm", ctypes.c_double),
         ("UnitMass_in_g", ctypes.c_double),
         ("UnitVelocity_in_cm_per_s", ctypes.c_double),
@@ -291,6 +294,8 @@ class TsGrids(ctypes.Structure):
         ("stars", ctypes.c_void_p),
         ("sfr_histories", ctypes.c_void_p),
         ("xray_luminosity_histories", ctypes.c_void_p),
+        ("bh_xray_histories_hard", ctypes.c_void_p),
+        ("bh_xray_histories_soft", ctypes.c_void_p),
         ("n_histories", ctypes.c_int),
         ("x_e_box_prev", ctypes.c_void_p),
         ("x_e_box", ctypes.c_void_p),
@@ -326,6 +331,14 @@ class TsTables(ctypes.Structure):
         ("freq_int_heat", (ctypes.c_double * _S) * _X),
         ("freq_int_ion", (ctypes.c_double * _S) * _X),
         ("freq_int_lya", (ctypes.c_double * _S) * _X),
+        ("const_zp_prefactor_AGN_soft", ctypes.c_double),
+        ("const_zp_prefactor_AGN_hard", ctypes.c_double),
+        ("freq_int_heat_AGN_soft", (ctypes.c_double * _S) * _X),
+        ("freq_int_ion_AGN_soft", (ctypes.c_double * _S) * _X),
+        ("freq_int_lya_AGN_soft", (ctypes.c_double * _S) * _X),
+        ("freq_int_heat_AGN_hard", (ctypes.c_double * _S) * _X),
+        ("freq_int_ion_AGN_hard", (ctypes.c_double * _S) * _X),
+        ("freq_int_lya_AGN_hard", (ctypes.c_double * _S) * _X),
         ("x_int_XHII", ctypes.c_float * _X),
         ("n_ops", ctypes.c_int * _S),
         ("op_kind", (ctypes.c_int * _K) * _S),
@@ -359,6 +372,10 @@ class TsGlobals(ctypes.Structure):
         ("volume_ave_TK", ctypes.c_double),
         ("volume_ave_xe", ctypes.c_double),
         ("XrayEmissivity_HMXB", ctypes.c_double),
+        ("XrayEmissivity_hard", ctypes.c_double),
+        ("XrayEmissivity_soft", ctypes.c_double),
+        ("volume_ave_Xheat_AGN_soft", ctypes.c_double),
+        ("volume_ave_Xheat_AGN_hard", ctypes.c_double),
         ("device_ms", ctypes.c_double),
     ]
 
@@ -468,7 +485,7 @@ def baryon_params(N: int, box: float, spin_temp: bool = False, bh: bool = False,
 TS_DEFAULTS = dict(OmegaR=0.0, wLambda=-1.0, Y_He=0.24, ReionRtoMFilterType=0, TsNumFilterSteps=40,
                    TsHeatingFilterType=1, ReionMaxHeatingRedshift=20.0, ReionEfficiency=1.0,
                    ReionNionPhotPerBary=4000.0, NuXrayThreshold=500.0, NuXraySoftCut=2000.0, NuXrayMax=10000.0,
-                   SpecIndexXrayGal=1.0)
+                   SpecIndexXrayGal=1.0, Flag_IncludeAGNXray=0, SpecIndexXrayAGNSoft=2.2, SpecIndexXrayAGNHard=1.7)
 
 
 def snapshot_redshifts(z_hi: float = 20.0, z_lo: float = 8.0, n: int = 41) -> np.ndarray:
@@ -863,16 +880,20 @@ class Context:
 
     def compute_ts(self, params: TsParams, deltax, stars, sfr_histories, xray_luminosity_histories, n_histories: int,
                    x_e_box_prev, x_e_box, Tk_box, TS_box, stored_fcoll: np.ndarray, want_tables: bool = True,
-                   history_head: int = 0):
+                   history_head: int = 0, bh_xray_histories_soft=None, bh_xray_histories_hard=None):
         """One ComputeTs step.  Returns (TsGlobals, TsTables or None); stored_fcoll (host float64, one
-        entry per snapshot) is updated in place.  history_head: ring slot of the newest history grid."""
+        entry per snapshot) is updated in place.  history_head: ring slot of the newest history grid.
+        The AGN histories (same layout as the others) are read as params.Flag_IncludeAGNXray asks."""
         g = TsGrids()
+        bh = [a for a in (bh_xray_histories_soft, bh_xray_histories_hard) if a is not None]
         g.mem = _mem_of([deltax, stars, sfr_histories, xray_luminosity_histories, x_e_box_prev, x_e_box, Tk_box,
-                         TS_box])
+                         TS_box] + bh)
         if g.mem == MXR_MEM_DEVICE:
             _sync_torch()
         g.deltax, g.stars = _ptr(deltax), _ptr(stars)
         g.sfr_histories, g.xray_luminosity_histories = _ptr(sfr_histories), _ptr(xray_luminosity_histories)
+        g.bh_xray_histories_soft = _ptr(bh_xray_histories_soft) if bh_xray_histories_soft is not None else None
+        g.bh_xray_histories_hard = _ptr(bh_xray_histories_hard) if bh_xray_histories_hard is not None else None
         g.n_histories = n_histories
         g.x_e_box_prev, g.x_e_box, g.Tk_box, g.TS_box = _ptr(x_e_box_prev), _ptr(x_e_box), _ptr(Tk_box), _ptr(TS_box)
         assert isinstance(stored_fcoll, np.ndarray) and stored_fcoll.dtype == np.float64

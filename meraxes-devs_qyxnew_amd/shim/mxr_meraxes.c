@@ -204,17 +204,13 @@ void ComputeBrightnessTemperatureBox_mxr(int snapshot)
             "mxr_brightness_temperature");
 }
 
-/* _ComputeTs (src/core/ComputeTs.c:85-1554) for Flag_IncludeAGNXray = 0, no mini-haloes,
- * timed as the reference's dispatcher ComputeTs (:1558-1579) times it */
+/* _ComputeTs (src/core/ComputeTs.c:85-1554) without mini-haloes, AGN X-ray bands as
+ * Flag_IncludeAGNXray asks, timed as the reference's dispatcher ComputeTs (:1558-1579) times it */
 void ComputeTs_mxr(int snapshot, timer_info* timer_total)
 {
   reion_grids_t* g = &run_globals.reion_grids;
   const run_params_t* rp = &run_globals.params;
   const physics_params_t* ph = &run_globals.params.physics;
-  if (ph->Flag_IncludeAGNXray != 0) {
-    mlog_error("the MI355X ComputeTs path has no AGN X-ray bands");
-    ABORT(EXIT_FAILURE);
-  }
   timer_info timer;
   timer_start(&timer);
   mxr_ts_params p;
@@ -238,6 +234,9 @@ void ComputeTs_mxr(int snapshot, timer_info* timer_total)
   p.NuXraySoftCut = ph->NuXraySoftCut;
   p.NuXrayMax = ph->NuXrayMax;
   p.SpecIndexXrayGal = ph->SpecIndexXrayGal;
+  p.Flag_IncludeAGNXray = ph->Flag_IncludeAGNXray;
+  p.SpecIndexXrayAGNSoft = ph->SpecIndexXrayAGNSoft;
+  p.SpecIndexXrayAGNHard = ph->SpecIndexXrayAGNHard;
   p.UnitLength_in_cm = run_globals.units.UnitLength_in_cm;
   p.UnitMass_in_g = run_globals.units.UnitMass_in_g;
   p.UnitVelocity_in_cm_per_s = run_globals.units.UnitVelocity_in_cm_per_s;
@@ -250,6 +249,8 @@ void ComputeTs_mxr(int snapshot, timer_info* timer_total)
   io.stars = g->stars;
   io.sfr_histories = g->sfr_histories;
   io.xray_luminosity_histories = g->xray_luminosity_histories;
+  io.bh_xray_histories_hard = g->bh_xray_histories_hard; /* NULL unless the flag allocated them */
+  io.bh_xray_histories_soft = g->bh_xray_histories_soft;
   io.n_histories = run_globals.NstoreSnapshots_Heating;
   io.x_e_box_prev = g->x_e_box_prev;
   io.x_e_box = g->x_e_box;
@@ -267,6 +268,10 @@ void ComputeTs_mxr(int snapshot, timer_info* timer_total)
   out.volume_ave_TK = g->volume_ave_TK;
   out.volume_ave_xe = g->volume_ave_xe;
   out.XrayEmissivity_HMXB = stored_XrayEmissivity_HMXB[snapshot];
+  out.XrayEmissivity_hard = stored_XrayEmissivity_hard[snapshot];
+  out.XrayEmissivity_soft = stored_XrayEmissivity_soft[snapshot];
+  out.volume_ave_Xheat_AGN_soft = g->volume_ave_Xheat_AGN_soft;
+  out.volume_ave_Xheat_AGN_hard = g->volume_ave_Xheat_AGN_hard;
   out.device_ms = 0.0;
   mxr_check(mxr_compute_ts(mxr_context(p.ReionGridDim), &p, &io, &out, NULL), "mxr_compute_ts");
   g->volume_ave_J_alpha = out.volume_ave_J_alpha;
@@ -277,6 +282,10 @@ void ComputeTs_mxr(int snapshot, timer_info* timer_total)
   g->volume_ave_TK = out.volume_ave_TK;
   g->volume_ave_xe = out.volume_ave_xe;
   stored_XrayEmissivity_HMXB[snapshot] = out.XrayEmissivity_HMXB;
+  stored_XrayEmissivity_hard[snapshot] = out.XrayEmissivity_hard;
+  stored_XrayEmissivity_soft[snapshot] = out.XrayEmissivity_soft;
+  g->volume_ave_Xheat_AGN_soft = out.volume_ave_Xheat_AGN_soft;
+  g->volume_ave_Xheat_AGN_hard = out.volume_ave_Xheat_AGN_hard;
   timer_stop(&timer);
   timer_stop(timer_total);
   timer_gpu += timer_delta(timer);

@@ -98,9 +98,11 @@ int oracle_power_spectrum(int N, double BoxSize, double Hubble_h, const float* d
                           float* PS_error, int PS_Length);
 
 /* One spin-temperature step on host grids, with the grid-independent tables `tb` the library
- * reported for the same call (mxr_ts_tables).  red0 (optional, 3 doubles) receives the R_ct = 0
- * averages: collapse fraction, x_e, X-ray emissivity; smoothed (optional, 2 F N^3 floats) the
- * smoothed SFR (clamped) then X-ray shells, real layout. */
+ * reported for the same call (mxr_ts_tables), AGN X-ray bands as p->Flag_IncludeAGNXray asks.
+ * red0 (optional, 5 doubles) receives the R_ct = 0 averages: collapse fraction, x_e, X-ray
+ * emissivity, AGN soft and hard emissivity; smoothed (optional, G F N^3 floats, G = 2 + the AGN
+ * bands) the smoothed SFR (clamped), X-ray, then AGN soft and hard (clamped) shells, real
+ * layout. */
 int oracle_ts_step_f32(const mxr_ts_params* p, const mxr_ts_tables* tb, const mxr_ts_grids* g,
                        mxr_ts_globals* out, double* red0, float* smoothed);
 /* The same with the shells smoothed in FP64 (double source grids, FFTs, filter, clamp): the

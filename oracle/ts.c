@@ -11,6 +11,9 @@
  *   src/core/ComputeTs.c:1171-1505  per-cell tables at x_e, evolveInt, Euler step, get_Ts,
  *                                   volume averages
  *   src/core/reionization.c:2750-2797  load_reion_sfr_grids (new load / accumulate, float)
+ *   src/core/reionization.c:2799-2870  load_reion_bh_grids (the AGN grids: loads and
+ *                                   accumulations only; ComputeTs.c clears and rescales sfr
+ *                                   and xray_luminosity alone)
  *   src/core/XRayHeatingFunctions.c:134-252 (T/xion_RECFAST), 426-453 (dtdz, hubble),
  *     1308-1561 (evolveInt), 1564-1595 (dT_comp, alpha_A), 1639-2008 (get_Ts, xcoll, kappa_10*,
  *     xalpha_tilde, taugp, Salpha_tilde, Tc_eff)
@@ -33,6 +36,7 @@
 #define BOLTZMANN 1.3806e-16
 #define TCMB 2.728
 #define XRAY_LUMINOSITY_UNIT 1.0e40
+#define SOLAR_LUM 3.826e33
 #define FRACT_FLOAT_ERR (double)(1e-7)
 #define MAX_TK (float)5e4
 #define MIN_TK (float)0.1
@@ -221,16 +225,23 @@ static double dT_comp(const ts_env* e, double z, double TK, double xe)
   return ans;
 }
 
-/* evolveInt without AGN, mini-haloes or Lyman-Werner (XRayHeatingFunctions.c:1308-1561) */
+/* evolveInt without mini-haloes or Lyman-Werner (XRayHeatingFunctions.c:1308-1561): the
+ * galaxy terms and the AGN soft / hard bands (XAGN_* zero and freq_int_*_AGN_* zero for a
+ * band the flag leaves out, as ComputeTs.c:1189-1191, 1260-1300 pass them) */
 static void evolve_int(const ts_env* e, const mxr_ts_tables* tb, const mxr_ts_params* p, float zp,
-                       float curr_delNL0, const double* SFR, const double* XR, const double* fheat,
-                       const double* fion, const double* flya, int NO_LIGHT, const double* y, double* deriv)
+                       float curr_delNL0, const double* SFR, const double* XR, const double* XAGN_soft,
+                       const double* XAGN_hard, const double* fheat, const double* fion, const double* flya,
+                       const double* fheat_s, const double* fion_s, const double* flya_s, const double* fheat_h,
+                       const double* fion_h, const double* flya_h, int NO_LIGHT, const double* y, double* deriv)
 {
   const int F = tb->n;
   const double Conversion_factor = (SPEED_OF_LIGHT / (4. * M_PI)) / (PROTONMASS / SOLAR_MASS);
   double x_e = y[0], T = y[1];
   double n_b = e->N_b0 * pow(1 + zp, 3) * (1 + curr_delNL0);
   double dxheat = 0, dxion = 0, dxlya = 0, dstarlya = 0;
+  double heat_s = 0, ion_s = 0, lya_s = 0, heat_h = 0, ion_h = 0, lya_h = 0;
+  deriv[5] = 0.0;
+  deriv[6] = 0.0;
   if (!NO_LIGHT) {
     for (int ct = 0; ct < F; ct++) {
       double zpp, dzpp;
@@ -247,19 +258,28 @@ static void evolve_int(const ts_env* e, const mxr_ts_tables* tb, const mxr_ts_pa
       dxion += dt_dzpp * dzpp * integrand * fion[ct];
       dxlya += dt_dzpp * dzpp * integrand * flya[ct];
       dstarlya += SFR[ct] * pow(1 + zp, 2) * (1 + zpp) * tb->sum_lyn[ct] * dt_dzpp * dzpp;
+      double integrand_s = XAGN_soft[ct] * pow(1 + zpp, -p->SpecIndexXrayAGNSoft);
+      heat_s += dt_dzpp * dzpp * integrand_s * fheat_s[ct];
+      ion_s += dt_dzpp * dzpp * integrand_s * fion_s[ct];
+      lya_s += dt_dzpp * dzpp * integrand_s * flya_s[ct];
+      double integrand_h = XAGN_hard[ct] * pow(1 + zpp, -p->SpecIndexXrayAGNHard);
+      heat_h += dt_dzpp * dzpp * integrand_h * fheat_h[ct];
+      ion_h += dt_dzpp * dzpp * integrand_h * fion_h[ct];
+      lya_h += dt_dzpp * dzpp * integrand_h * flya_h[ct];
     }
     dxheat *= tb->const_zp_prefactor_GAL;
     dxion *= tb->const_zp_prefactor_GAL;
     dxlya *= tb->const_zp_prefactor_GAL * n_b;
     dstarlya *= Conversion_factor;
-  } else {
-    dxheat *= tb->const_zp_prefactor_GAL;
-    dxion *= tb->const_zp_prefactor_GAL;
-    dxlya *= tb->const_zp_prefactor_GAL * n_b;
-    dstarlya *= Conversion_factor;
+    heat_s *= tb->const_zp_prefactor_AGN_soft;
+    ion_s *= tb->const_zp_prefactor_AGN_soft;
+    lya_s *= tb->const_zp_prefactor_AGN_soft * n_b;
+    heat_h *= tb->const_zp_prefactor_AGN_hard;
+    ion_h *= tb->const_zp_prefactor_AGN_hard;
+    lya_h *= tb->const_zp_prefactor_AGN_hard * n_b;
   }
   double sink = alpha_A(T) * CLUMPING_FACTOR * x_e * x_e * e->f_H * n_b;
-  double dxe_dzp = tb->dt_dzp * (dxion - sink);
+  double dxe_dzp = tb->dt_dzp * (dxion + ion_s + ion_h - sink);
   deriv[0] = dxe_dzp;
   double dadia = 3 / (1.0 + zp);
   if (fabs(curr_delNL0) > FRACT_FLOAT_ERR)
@@ -267,27 +287,38 @@ static void evolve_int(const ts_env* e, const mxr_ts_tables* tb, const mxr_ts_pa
   dadia *= (2.0 / 3.0) * T;
   double dspec = -dxe_dzp * T / (1 + x_e);
   double dcomp = dT_comp(e, zp, T, x_e);
+  deriv[5] = heat_s * tb->dt_dzp * 2.0 / 3.0 / BOLTZMANN / (1.0 + x_e);
+  deriv[6] = heat_h * tb->dt_dzp * 2.0 / 3.0 / BOLTZMANN / (1.0 + x_e);
   double dxheat_dzp = dxheat * tb->dt_dzp * 2.0 / 3.0 / BOLTZMANN / (1.0 + x_e);
+  dxheat_dzp += deriv[5] + deriv[6];
   deriv[1] = dxheat_dzp + dcomp + dspec + dadia;
-  deriv[2] = dxlya + dstarlya;
+  deriv[2] = dxlya + lya_s + lya_h + dstarlya;
   deriv[3] = dxheat_dzp;
-  deriv[4] = tb->dt_dzp * dxion;
+  deriv[4] = tb->dt_dzp * (dxion + ion_s + ion_h);
 }
 
 static float cT_approx(float z) { return 0.58 - 0.006 * (z - 10.0); }
 
 /* The shells of one call (ComputeTs.c:366-746) in FP32 (fftwf, float filter and clamp, as
  * the reference) or FP64 (double source grids, FFTs, filter and clamp).  sm receives the
- * smoothed SFR then X-ray shells [2][F][N^3]; red0 the R_ct = 0 averages. */
+ * smoothed shells [G][F][N^3] of the G source grids in hist: SFR, X-ray, then the AGN bands
+ * (soft before hard) the call includes.  Every grid but the X-ray one is clamped at 0; the
+ * AGN grids take the plan's loads and accumulations only. */
 #define SMOOTH_BODY(REAL, R2C, C2R, FILTER, RCAST, FMAX)                                                              \
   do {                                                                                                                 \
-    REAL* src[2] = {malloc(sizeof(REAL) * npad), malloc(sizeof(REAL) * npad)};                                        \
-    REAL* spec[2] = {malloc(sizeof(REAL) * 2 * nc), malloc(sizeof(REAL) * 2 * nc)};                                   \
+    REAL* src[4] = {NULL, NULL, NULL, NULL};                                                                           \
+    REAL* spec[4] = {NULL, NULL, NULL, NULL};                                                                          \
+    int ok = 1;                                                                                                        \
+    for (int k = 0; k < G; k++) {                                                                                      \
+      src[k] = malloc(sizeof(REAL) * npad);                                                                            \
+      spec[k] = malloc(sizeof(REAL) * 2 * nc);                                                                         \
+      ok = ok && src[k] && spec[k];                                                                                    \
+    }                                                                                                                  \
     REAL* work = malloc(sizeof(REAL) * 2 * nc);                                                                        \
-    if (!src[0] || !src[1] || !spec[0] || !spec[1] || !work) {                                                         \
+    if (!ok || !work) {                                                                                                \
       rc = -3;                                                                                                         \
     } else {                                                                                                           \
-      for (int k = 0; k < 2; k++)                                                                                      \
+      for (int k = 0; k < G; k++)                                                                                      \
         for (long i = 0; i < npad; i++)                                                                                \
           src[k][i] = hist[k][i];                                                                                      \
       for (int r = 0; r < F; r++) {                                                                                    \
@@ -295,8 +326,10 @@ static float cT_approx(float z) { return 0.58 - 0.006 * (z - 10.0); }
         for (int q = 0; q < tb->n_ops[r]; q++) {                                                                       \
           const int kind = tb->op_kind[r][q], hi = tb->op_hist[r][q];                                                  \
           const double w = tb->op_weight[r][q];                                                                        \
-          for (int k = 0; k < 2; k++) {                                                                                \
+          for (int k = 0; k < G; k++) {                                                                                \
             const float* h = hist[k] + (long)hi * npad;                                                                \
+            if (k >= 2 && (kind == MXR_TS_OP_CLEAR || kind == MXR_TS_OP_SCALE))                                        \
+              continue;                                                                                                \
             for (long i = 0; i < npad; i++) {                                                                          \
               if (kind == MXR_TS_OP_CLEAR)                                                                             \
                 src[k][i] = 0;                                                                                         \
@@ -309,7 +342,7 @@ static float cT_approx(float z) { return 0.58 - 0.006 * (z - 10.0); }
             }                                                                                                          \
           }                                                                                                            \
         }                                                                                                              \
-        for (int k = 0; k < 2; k++) {                                                                                  \
+        for (int k = 0; k < G; k++) {                                                                                  \
           if (r == 0 || tb->n_ops[r] > 0) {                                                                            \
             R2C(N, src[k], spec[k]);                                                                                   \
             for (long i = 0; i < 2 * nc; i++)                                                                          \
@@ -322,24 +355,24 @@ static float cT_approx(float z) { return 0.58 - 0.006 * (z - 10.0); }
           double* dst = sm + ((size_t)k * F + r) * nreal;                                                              \
           for (long ir = 0; ir < nreal; ir++) {                                                                        \
             long ip = (ir / N) * 2 * H + ir % N;                                                                       \
-            dst[ir] = k == 0 ? FMAX(work[ip], 0.0) : work[ip];                                                         \
+            dst[ir] = k != 1 ? FMAX(work[ip], 0.0) : work[ip];                                                         \
           }                                                                                                            \
         }                                                                                                              \
       }                                                                                                                \
     }                                                                                                                  \
-    free(src[0]);                                                                                                      \
-    free(src[1]);                                                                                                      \
-    free(spec[0]);                                                                                                     \
-    free(spec[1]);                                                                                                     \
+    for (int k = 0; k < 4; k++) {                                                                                      \
+      free(src[k]);                                                                                                    \
+      free(spec[k]);                                                                                                   \
+    }                                                                                                                  \
     free(work);                                                                                                        \
   } while (0)
 
-static int smooth(const mxr_ts_params* p, const mxr_ts_tables* tb, const mxr_ts_grids* g, double* sm, int f64)
+static int smooth(const mxr_ts_params* p, const mxr_ts_tables* tb, const float* const* hist, int G, double* sm,
+                  int f64)
 {
   const int N = p->ReionGridDim, H = N / 2 + 1, F = tb->n;
   const long nreal = (long)N * N * N, nc = (long)N * N * H, npad = 2 * nc;
   const double total_n_cells = pow((double)N, 3);
-  const float* hist[2] = {g->sfr_histories, g->xray_luminosity_histories};
   int rc = 0;
   if (f64)
     SMOOTH_BODY(double, oracle_r2c3d_f64, oracle_c2r3d_f64, oracle_filter_f64, double, fmax);
@@ -384,14 +417,35 @@ static int ts_step(const mxr_ts_params* p, const mxr_ts_tables* tb, const mxr_ts
     }
   } else {
     const int F = tb->n;
-    double* sm = malloc(sizeof(double) * 2 * (size_t)F * nreal);
-    if (!sm || smooth(p, tb, g, sm, f64)) {
+    /* ComputeTs.c:159-162: which AGN bands the flag includes */
+    const int agn_soft = p->Flag_IncludeAGNXray == 1 || p->Flag_IncludeAGNXray == 3;
+    const int agn_hard = p->Flag_IncludeAGNXray == 1 || p->Flag_IncludeAGNXray == 2;
+    const float* hist[4] = {g->sfr_histories, g->xray_luminosity_histories, NULL, NULL};
+    int G = 2, ks = -1, kh = -1;
+    if (agn_soft) {
+      ks = G;
+      hist[G++] = g->bh_xray_histories_soft;
+    }
+    if (agn_hard) {
+      kh = G;
+      hist[G++] = g->bh_xray_histories_hard;
+    }
+    for (int k = 0; k < G; k++)
+      if (!hist[k]) {
+        free(e);
+        return -1;
+      }
+    double* sm = malloc(sizeof(double) * G * (size_t)F * nreal);
+    if (!sm || smooth(p, tb, hist, G, sm, f64)) {
       free(sm);
       free(e);
       return -3;
     }
+    /* SMOOTHED_AGN_* of a shell and cell (ComputeTs.c:617-618, 626-627, 722-731) */
+#define AGN_FIELD(k, r, ir) \
+  ((k) < 0 ? 0.0 : sm[((size_t)(k) * F + (r)) * nreal + (ir)] * 1e10 * SOLAR_LUM / pixel_volume * pow(p->UnitLength_in_cm, -3.0))
     {
-      double collapse_fraction = 0, x_e_ave = 0, hmxb = 0;
+      double collapse_fraction = 0, x_e_ave = 0, hmxb = 0, agn_soft_ave = 0, agn_hard_ave = 0;
       mxr_hii_params hp;
       memset(&hp, 0, sizeof(hp));
       hp.OmegaM = p->OmegaM;
@@ -412,23 +466,32 @@ static int ts_step(const mxr_ts_params* p, const mxr_ts_tables* tb, const mxr_ts
         x_e_ave += xe[ip];
         hmxb += (sm[((size_t)F + 0) * nreal + ir] / pixel_volume) * XRAY_LUMINOSITY_UNIT *
                 pow(p->UnitLength_in_cm, -3.);
+        agn_soft_ave += AGN_FIELD(ks, 0, ir);
+        agn_hard_ave += AGN_FIELD(kh, 0, ir);
       }
       if (red0) {
         red0[0] = collapse_fraction / total_n_cells;
         red0[1] = x_e_ave / total_n_cells;
         red0[2] = hmxb / total_n_cells;
+        red0[3] = agn_soft_ave / total_n_cells;
+        red0[4] = agn_hard_ave / total_n_cells;
       }
+      out->XrayEmissivity_HMXB = hmxb / total_n_cells;
+      out->XrayEmissivity_soft = agn_soft_ave / total_n_cells;
+      out->XrayEmissivity_hard = agn_hard_ave / total_n_cells;
     }
     if (smoothed)
-      for (size_t i = 0; i < 2 * (size_t)F * nreal; i++)
+      for (size_t i = 0; i < (size_t)G * F * nreal; i++)
         smoothed[i] = (float)sm[i];
-    double J_alpha_ave = 0, xalpha_ave = 0, Xheat_ave = 0, Xion_ave = 0;
+    double J_alpha_ave = 0, xalpha_ave = 0, Xheat_ave = 0, Xion_ave = 0, Xheat_s = 0, Xheat_h = 0;
     double SFR[MXR_TS_MAX_STEPS], XR[MXR_TS_MAX_STEPS], fh[MXR_TS_MAX_STEPS], fi[MXR_TS_MAX_STEPS],
       fl[MXR_TS_MAX_STEPS];
+    double XS[MXR_TS_MAX_STEPS], XH[MXR_TS_MAX_STEPS], fhs[MXR_TS_MAX_STEPS], fis[MXR_TS_MAX_STEPS],
+      fls[MXR_TS_MAX_STEPS], fhh[MXR_TS_MAX_STEPS], fih[MXR_TS_MAX_STEPS], flh[MXR_TS_MAX_STEPS];
     const float* X = tb->x_int_XHII;
     for (long ir = 0; ir < nreal; ir++) {
       long ip = (ir / N) * 2 * H + ir % N;
-      double ans[2] = {xe[ip], Tk[ir]}, d[5];
+      double ans[2] = {xe[ip], Tk[ir]}, d[7];
       for (int r = 0; r < F; r++) {
         SFR[r] = (sm[(size_t)r * nreal + ir] / pixel_volume) * (p->UnitMass_in_g / UT) *
                  pow(p->UnitLength_in_cm, -3.) / SOLAR_MASS;
@@ -452,8 +515,22 @@ static int ts_step(const mxr_ts_params* p, const mxr_ts_tables* tb, const mxr_ts
         fl[r] = (tb->freq_int_lya[hi][r] - tb->freq_int_lya[lo][r]) / (X[hi] - X[lo]);
         fl[r] *= (xc - X[lo]);
         fl[r] += tb->freq_int_lya[lo][r];
+        /* the AGN bands' tables at x_e (ComputeTs.c:1260-1300), zero for a band left out */
+        XS[r] = AGN_FIELD(ks, r, ir);
+        XH[r] = AGN_FIELD(kh, r, ir);
+#define INTERP(T) (((T)[hi][r] - (T)[lo][r]) / (X[hi] - X[lo]) * (xc - X[lo]) + (T)[lo][r])
+        fhs[r] = agn_soft ? INTERP(tb->freq_int_heat_AGN_soft) : 0.0;
+        fis[r] = agn_soft ? INTERP(tb->freq_int_ion_AGN_soft) : 0.0;
+        fls[r] = agn_soft ? INTERP(tb->freq_int_lya_AGN_soft) : 0.0;
+        fhh[r] = agn_hard ? INTERP(tb->freq_int_heat_AGN_hard) : 0.0;
+        fih[r] = agn_hard ? INTERP(tb->freq_int_ion_AGN_hard) : 0.0;
+        flh[r] = agn_hard ? INTERP(tb->freq_int_lya_AGN_hard) : 0.0;
+#undef INTERP
       }
-      evolve_int(e, tb, p, (float)zp, g->deltax[ip], SFR, XR, fh, fi, fl, tb->NO_LIGHT, ans, d);
+      evolve_int(e, tb, p, (float)zp, g->deltax[ip], SFR, XR, XS, XH, fh, fi, fl, fhs, fis, fls, fhh, fih, flh,
+                 tb->NO_LIGHT, ans, d);
+      Xheat_s += d[5];
+      Xheat_h += d[6];
       xe[ip] += d[0] * tb->dzp;
       if (xe[ip] > 1)
         xe[ip] = (float)(1 - FRACT_FLOAT_ERR);
@@ -474,6 +551,9 @@ static int ts_step(const mxr_ts_params* p, const mxr_ts_tables* tb, const mxr_ts
     out->volume_ave_xalpha = xalpha_ave / total_n_cells;
     out->volume_ave_Xheat = Xheat_ave / total_n_cells;
     out->volume_ave_Xion = Xion_ave / total_n_cells;
+    out->volume_ave_Xheat_AGN_soft = Xheat_s / total_n_cells;
+    out->volume_ave_Xheat_AGN_hard = Xheat_h / total_n_cells;
+#undef AGN_FIELD
     free(sm);
   }
   memcpy(g->x_e_box, xe, sizeof(float) * npad);

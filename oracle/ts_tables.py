@@ -7,6 +7,7 @@ Follows:
   src/core/ComputeTs.c:32-83       set_sfr_history (history depth)
   src/core/ComputeTs.c:339-478     shell radii, zpp_edge, history weighting plan
   src/core/ComputeTs.c:761-1011    nu_tau_one limits, sum_lyn and the kink correction
+  src/core/ComputeTs.c:835-899     the AGN bands' frequency-integral limits
   src/core/ComputeTs.c:1013-1128   luminosity conversion, const_zp_prefactor_GAL
   src/core/XRayHeatingFunctions.c:262-424 (nu_tau_one, tauX), 426-453 (dtdz, hubble),
     458-516 (cross sections), 545-728 (zmax, nu_n, frecycle, spectral_emissivity),
@@ -363,8 +364,11 @@ def _gl_pieces(bounds, npts):
     return ((a + b) / 2 + (b - a) / 2 * xg[None, :]).ravel(), ((b - a) / 2 * wg[None, :]).ravel()
 
 
-def integrate_over_nu(xi: XInt, cz: Cosmo, p, zp, x_e, lo, hi, flag, npts=16):
-    """The frequency integral of integrate_over_nu to ~1e-10 (kinks at the table energies)."""
+def integrate_over_nu(xi: XInt, cz: Cosmo, p, zp, x_e, lo, hi, flag, npts=16, spec_index=None):
+    """The frequency integral of integrate_over_nu to ~1e-10 (kinks at the table energies), for
+    the spectral index `spec_index` (default: the galaxies' SpecIndexXrayGal; the AGN bands pass
+    theirs, ComputeTs.c:842-896)."""
+    alpha = p["SpecIndexXrayGal"] if spec_index is None else spec_index
     if lo >= hi:
         return 0.0
     cuts = [lo, hi]
@@ -392,7 +396,7 @@ def integrate_over_nu(xi: XInt, cz: Cosmo, p, zp, x_e, lo, hi, flag, npts=16):
             s += Fi.astype(float) * frac * cs(nu)
         else:
             s += xi.interp("nLya", En, xf, 0.0).astype(float) * frac * cs(nu)
-    s *= (nu / (p["NuXrayThreshold"] * NU_over_EV)) ** (-p["SpecIndexXrayGal"] - 1)
+    s *= (nu / (p["NuXrayThreshold"] * NU_over_EV)) ** (-alpha - 1)
     res = float(np.sum(s * w))
     if flag == 2:
         res *= SPEED_OF_LIGHT / (4 * math.pi) / NU_LA / cz.hubble(zp)

@@ -331,10 +331,11 @@ def power_spectrum(delta_T: np.ndarray, N: int, box_size: float, hubble_h: float
 
 
 def ts_step(params, tables, deltax, stars, sfr_histories, xray_histories, n_hist, x_e_box_prev, Tk_box,
-            want_smoothed=False, f64=False):
+            want_smoothed=False, f64=False, bh_soft_histories=None, bh_hard_histories=None):
     """One oracle spin-temperature step on copies of the state.  Returns a dict with the new
-    x_e_box_prev, x_e_box, Tk_box, TS_box, the globals, red0 (collapse fraction, <x_e>, <L_X>)
-    and (optionally) the smoothed shells [2, F, N^3]."""
+    x_e_box_prev, x_e_box, Tk_box, TS_box, the globals, red0 (collapse fraction, <x_e>, <L_X>,
+    <AGN soft>, <AGN hard>) and (optionally) the smoothed shells [G, F, N^3] (SFR, X-ray, then
+    the AGN bands params.Flag_IncludeAGNXray includes, soft before hard)."""
     N = params.ReionGridDim
     npad, nreal = mxr.padded_len(N), N ** 3
     xe = np.array(x_e_box_prev, np.float32, copy=True)
@@ -342,16 +343,21 @@ def ts_step(params, tables, deltax, stars, sfr_histories, xray_histories, n_hist
     xe_out = np.zeros(npad, np.float32)
     TS = np.zeros(nreal, np.float32)
     arrs = [np.ascontiguousarray(a, np.float32) for a in (deltax, stars, sfr_histories, xray_histories)]
+    bh = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (bh_soft_histories, bh_hard_histories)]
     g = mxr.TsGrids()
     g.mem = mxr.MXR_MEM_HOST
     g.deltax, g.stars, g.sfr_histories, g.xray_luminosity_histories = (_p(a) for a in arrs)
+    g.bh_xray_histories_soft = _p(bh[0]) if bh[0] is not None else None
+    g.bh_xray_histories_hard = _p(bh[1]) if bh[1] is not None else None
+    flag = params.Flag_IncludeAGNXray
+    G = 2 + (flag in (1, 3)) + (flag in (1, 2))
     g.n_histories = n_hist
     g.x_e_box_prev, g.x_e_box, g.Tk_box, g.TS_box = _p(xe), _p(xe_out), _p(Tk), _p(TS)
     sf = np.zeros(1, np.float64)
     g.stored_fcoll = _p(sf)
     out = mxr.TsGlobals()
-    red0 = np.zeros(3, np.float64)
-    sm = np.zeros(2 * max(tables.n, 1) * nreal, np.float32) if want_smoothed else None
+    red0 = np.zeros(5, np.float64)
+    sm = np.zeros(G * max(tables.n, 1) * nreal, np.float32) if want_smoothed else None
     fn = lib().oracle_ts_step_f64 if f64 else lib().oracle_ts_step_f32
     rc = fn(ctypes.byref(params), ctypes.byref(tables), ctypes.byref(g), ctypes.byref(out),
                                   _p(red0), _p(sm) if sm is not None else None)
@@ -359,5 +365,5 @@ def ts_step(params, tables, deltax, stars, sfr_histories, xray_histories, n_hist
         raise RuntimeError(f"oracle ts step failed: {rc}")
     res = dict(x_e_box_prev=xe, x_e_box=xe_out, Tk_box=Tk, TS_box=TS, globals=out.as_dict(), red0=red0)
     if sm is not None:
-        res["smoothed"] = sm.reshape(2, tables.n, nreal)
+        res["smoothed"] = sm.reshape(G, tables.n, nreal)
     return res

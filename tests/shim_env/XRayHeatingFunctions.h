@@ -3,4 +3,6 @@
 #define MXR_SHIM_ENV_XRAY_H
 extern double* stored_fcoll;
 extern double* stored_XrayEmissivity_HMXB;
+extern double* stored_XrayEmissivity_hard;
+extern double* stored_XrayEmissivity_soft;
 #endif

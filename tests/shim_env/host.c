@@ -13,6 +13,8 @@ run_globals_t run_globals;
 float timer_gpu = 0.f;
 double* stored_fcoll;
 double* stored_XrayEmissivity_HMXB;
+double* stored_XrayEmissivity_hard;
+double* stored_XrayEmissivity_soft;
 
 void myexit(int signum) { exit(signum); }
 void mlog_error(const char* fmt, ...)

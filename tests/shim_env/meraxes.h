@@ -21,6 +21,8 @@ typedef struct physics_params_t {
   double NuXraySoftCut;
   double NuXrayMax;
   int Flag_IncludeAGNXray;
+  double SpecIndexXrayAGNSoft;
+  double SpecIndexXrayAGNHard;
   double ReionMaxHeatingRedshift;
   double ReionGammaHaloBias;
   double ReionAlphaUV;
@@ -72,6 +74,8 @@ typedef struct reion_grids_t {
   float* sfr_histories;
   float* weighted_sfr;
   float* xray_luminosity_histories;
+  float* bh_xray_histories_hard;
+  float* bh_xray_histories_soft;
   float* xH;
   float* z_at_ionization;
   float* J_21_at_ionization;
@@ -107,6 +111,8 @@ typedef struct reion_grids_t {
   double volume_ave_J_alpha;
   double volume_ave_xalpha;
   double volume_ave_Xheat;
+  double volume_ave_Xheat_AGN_soft;
+  double volume_ave_Xheat_AGN_hard;
   double volume_ave_Xion;
   double volume_ave_TS;
   double volume_ave_TK;

@@ -50,3 +50,35 @@ def rel_err(a, b, floor=0.0):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return np.abs(a - b) / np.maximum(np.abs(b), floor if floor else 1e-300)
+
+
+SOLAR_LUM = 3.826e33
+
+
+def agn_histories(N, d, n_hist, level=1.0, seed=11):
+    """AGN soft / hard X-ray emissivity histories for the inputs d (padded float32 grids, 1e10 L_sun
+    per cell, [s] = s snapshots back): each snapshot's cells at or above the 99th percentile of its
+    positive SFR host an AGN whose hard emissivity follows that SFR with a lognormal scatter, and
+    whose soft emissivity is half the hard one with its own scatter.  `level` is the ratio of the
+    AGN's hard-band emissivity to the galaxies' X-ray emissivity, volume-averaged per snapshot."""
+    rng = np.random.default_rng(seed)
+    H = N // 2 + 1
+    npad = N * N * 2 * H
+    sfr = d["sfr_histories"].reshape(-1, npad)[:n_hist]
+    xr = d["xray_luminosity_histories"].reshape(-1, npad)[:n_hist]
+    hard = np.zeros((n_hist, npad), np.float32)
+    soft = np.zeros((n_hist, npad), np.float32)
+    real = np.zeros((N, N, 2 * H), bool)
+    real[:, :, :N] = True
+    real = real.reshape(-1)
+    for s in range(n_hist):
+        v = np.where(real, sfr[s].astype(np.float64), 0.0)
+        if not np.any(v > 0):
+            continue
+        host = v >= np.percentile(v[v > 0], 99)
+        e = np.where(host, v * rng.lognormal(0.0, 0.5, npad), 0.0)
+        want = level * float(np.sum(np.where(real, xr[s], 0.0), dtype=np.float64)) * 1e40 / (1e10 * SOLAR_LUM)
+        e *= want / e.sum()
+        hard[s] = e.astype(np.float32)
+        soft[s] = (0.5 * e * rng.lognormal(0.0, 0.3, npad)).astype(np.float32)
+    return soft.reshape(-1), hard.reshape(-1)
