@@ -152,7 +152,7 @@ typedef struct mxr_ctx mxr_ctx;
  * grid/plan set-up of malloc_reionization_grids (src/core/reionization.c:718-1000).      */
 int mxr_create(mxr_ctx** ctx, int device, int grid_dim);
 /* The grid dimensions this build compiles kernels for, in increasing order:
- *   16, 32, 64, 96, 128, 160, 256, 512
+ *   16, 32, 64, 96, 128, 160, 256, 512, 1024
  * (the reference plans FFTW for any ReionGridDim, src/core/reionization.c:907-997; here each
  * transform length is a compile-time radix plan).  Writes up to `capacity` of them into dims
  * (may be NULL) and returns how many there are.  mxr_create rejects any other dimension with

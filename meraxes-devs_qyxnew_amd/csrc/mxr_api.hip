@@ -260,7 +260,7 @@ int mxr_create(mxr_ctx** out, int device, int grid_dim)
   }
   *out = nullptr;
   if (!supported_dim(grid_dim)) {
-    set_error("grid dimension %d is not supported (need one of 16,32,64,96,128,160,256,512)",
+    set_error("grid dimension %d is not supported (need one of 16,32,64,96,128,160,256,512,1024)",
               grid_dim);
     return MXR_ERR_INVALID_ARG;
   }
@@ -1151,7 +1151,7 @@ int mxr_prepare_grid(mxr_ctx* h, const float* hi, int n_hi, int mem_in, int prop
     return MXR_ERR_INVALID_ARG;
   }
   if (n_hi != h->c.N && !supported_dim(n_hi)) {
-    set_error("smoothing a %d^3 grid is not supported (need one of 16,32,64,96,128,160,256,512)", n_hi);
+    set_error("smoothing a %d^3 grid is not supported (need one of 16,32,64,96,128,160,256,512,1024)", n_hi);
     return MXR_ERR_UNSUPPORTED;
   }
   if (hipSetDevice(h->c.device) != hipSuccess) {

@@ -178,7 +178,7 @@ int check_hii_params(const mxr_hii_params* p, int N);
 
 // grid-dimension dispatch (mxr_hii.hip)
 // grid dimensions the library instantiates (include/mxr.h lists them at mxr_supported_dims)
-#define MXR_DIMS(X) X(16) X(32) X(64) X(96) X(128) X(160) X(256) X(512)
+#define MXR_DIMS(X) X(16) X(32) X(64) X(96) X(128) X(160) X(256) X(512) X(1024)
 bool supported_dim(int N);
 int run_brightness_temperature_noalloc(const mxr_tb_params* p, const mxr_tb_grids* g, double* ave_dev,
                                        hipStream_t stream);

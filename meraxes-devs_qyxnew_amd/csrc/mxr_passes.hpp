@@ -549,10 +549,14 @@ struct Cfg {
   static constexpr int M = N / 2;
   static constexpr int H = N / 2 + 1;
   static constexpr int TC = 16;
-  static constexpr int COL_T = N == 512 ? 32 : (N / 8 >= 16) ? 16 : (N / 8 < 1 ? 1 : N / 8);
+  // threads per column: 16 values each at N = 512 and 1024 (a 1024-point column over 16
+  // threads would hold 64 per thread)
+  static constexpr int COL_T = N >= 1024 ? 64 : N == 512 ? 32 : (N / 8 >= 16) ? 16 : (N / 8 < 1 ? 1 : N / 8);
   static constexpr int COL_NT = TC * COL_T;
   static constexpr int COL_TPB = 1;  // column tiles per block (2 overlaps loads with transforms but needs ~280 VGPRs)
-  static constexpr int ROW_T = pow2_floor((M / 8 >= 16) ? 16 : (M / 8 < 1 ? 1 : M / 8));
+  // threads per row (one wavefront at most): 32 at N = 1024 keeps 16 values per thread and
+  // the row tile's LDS at 45 KB
+  static constexpr int ROW_T = N >= 1024 ? 32 : pow2_floor((M / 8 >= 16) ? 16 : (M / 8 < 1 ? 1 : M / 8));
   static constexpr int ROW_NT = 256;
   static constexpr int TR = ROW_NT / ROW_T;
 };
