@@ -10,7 +10,9 @@
 // occupied cells' values and zero elsewhere, which is what the reference's clear and
 // finalisation give a cell without galaxies.  The histories move up by device copies on a second
 // stream meanwhile.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_run_length_encode.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -382,15 +384,15 @@ int run_construct_baryon_grids(Ctx* c, const mxr_baryon_params* p, const mxr_gal
     end_bit++;
   if (n > 0) {
     size_t b = 0;
-    MXR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b, (const unsigned*)nullptr, (unsigned*)nullptr,
-                                               (const unsigned*)nullptr, (unsigned*)nullptr, (int)n, 0, end_bit,
-                                               c->stream));
+    MXR_HIP(rocprim::radix_sort_pairs(nullptr, b, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                      (const unsigned*)nullptr, (unsigned*)nullptr, (unsigned)n, 0u,
+                                      (unsigned)end_bit, c->stream));
     tmp_bytes = std::max(tmp_bytes, b);
-    MXR_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, b, (const unsigned*)nullptr, (unsigned*)nullptr,
-                                                  (unsigned*)nullptr, (unsigned*)nullptr, (int)n, c->stream));
+    MXR_HIP(rocprim::run_length_encode(nullptr, b, (const unsigned*)nullptr, (unsigned)n, (unsigned*)nullptr,
+                                       (unsigned*)nullptr, (unsigned*)nullptr, c->stream));
     tmp_bytes = std::max(tmp_bytes, b);
-    MXR_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, (const unsigned*)nullptr, (unsigned*)nullptr, (int)n,
-                                             c->stream));
+    MXR_HIP(rocprim::exclusive_scan(nullptr, b, (const unsigned*)nullptr, (unsigned*)nullptr, 0u, (size_t)n,
+                                    rocprim::plus<unsigned>(), c->stream));
     tmp_bytes = std::max(tmp_bytes, b);
   }
   const size_t kb = align256(sizeof(unsigned) * (size_t)n);
@@ -547,20 +549,19 @@ int run_construct_baryon_grids(Ctx* c, const mxr_baryon_params* p, const mxr_gal
   }
   if (rc == MXR_OK && n > 0) {
     size_t b = tmp_bytes;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(lib_tmp, b, keys_in, keys_out, vals_in, vals_out, (int)n, 0,
-                                                      end_bit, c->stream);
+    hipError_t e = rocprim::radix_sort_pairs(lib_tmp, b, keys_in, keys_out, vals_in, vals_out, (unsigned)n, 0u,
+                                             (unsigned)end_bit, c->stream);
     if (e != hipSuccess)
-      rc = hip_fail(e, "hipcub::DeviceRadixSort::SortPairs", __FILE__, __LINE__);
+      rc = hip_fail(e, "rocprim::radix_sort_pairs", __FILE__, __LINE__);
     // the occupied cells: their keys, run lengths and count, and the runs' offsets
     b = tmp_bytes;
-    if (rc == MXR_OK &&
-        (e = hipcub::DeviceRunLengthEncode::Encode(lib_tmp, b, keys_out, run_keys, run_len, nrun, (int)n,
-                                                   c->stream)) != hipSuccess)
-      rc = hip_fail(e, "hipcub::DeviceRunLengthEncode::Encode", __FILE__, __LINE__);
+    if (rc == MXR_OK && (e = rocprim::run_length_encode(lib_tmp, b, keys_out, (unsigned)n, run_keys, run_len, nrun,
+                                                        c->stream)) != hipSuccess)
+      rc = hip_fail(e, "rocprim::run_length_encode", __FILE__, __LINE__);
     b = tmp_bytes;
-    if (rc == MXR_OK &&
-        (e = hipcub::DeviceScan::ExclusiveSum(lib_tmp, b, run_len, run_off, (int)n, c->stream)) != hipSuccess)
-      rc = hip_fail(e, "hipcub::DeviceScan::ExclusiveSum", __FILE__, __LINE__);
+    if (rc == MXR_OK && (e = rocprim::exclusive_scan(lib_tmp, b, run_len, run_off, 0u, (size_t)n,
+                                                     rocprim::plus<unsigned>(), c->stream)) != hipSuccess)
+      rc = hip_fail(e, "rocprim::exclusive_scan", __FILE__, __LINE__);
   }
   if (rc == MXR_OK) {
     hipError_t e = hipSuccess;

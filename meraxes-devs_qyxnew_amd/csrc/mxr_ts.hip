@@ -24,6 +24,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "mxr_passes.hpp"
@@ -317,10 +318,11 @@ struct TsEvolve {
 // the heat / ionisation / Lyman-alpha rates, F for the stellar Lyman-alpha flux, and with NB
 // AGN bands NB x 6 x 14 x F more): per cell and shell 2 + NB coalesced FP32 loads and 7 + 6 NB
 // FP64 FMAs.  The per-cell tail (Euler step, floors, T_S) follows the reference's evaluation
-// order.  Without AGN bands: three waves per SIMD (168 VGPRs, a few spilled) with 16 loads in
-// flight each, 256 threads: 1.86 ms at 256^3 x 40 shells, against 2.18 ms at two waves and
-// 2.32 ms with 4 loads in flight.  With AGN bands the tables take up to 130 KB of LDS at 64
-// shells, so one 768-thread workgroup per CU keeps the same three waves per SIMD.
+// order.  Launched with NB = 1 or 2 AGN bands (k_ts_evolve_g below serves the galaxy-only step,
+// cell for cell the same arithmetic as NB = 0 here): the bands' tables take up to 130 KB of LDS
+// at 64 shells, so one 768-thread workgroup per CU keeps three waves per SIMD.  For the
+// galaxy-only form three waves per SIMD with 16 loads in flight measured 1.86 ms at 256^3 x 40
+// shells, against 2.18 ms at two waves and 2.32 ms with 4 loads in flight.
 template <int NT, int NB>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3)))
 k_ts_evolve(const TsDev* __restrict__ T, TsEvolve a)
@@ -502,6 +504,149 @@ k_ts_evolve(const TsDev* __restrict__ T, TsEvolve a)
     acc[6] += (double)xe1;
   }
   block_sum<NT, NV>(acc, a.partials + (long)blockIdx.x * NV);
+}
+
+// Galaxy sources only (no AGN bands), CPT adjacent cells per thread (launched with CPT = 1):
+// k_ts_evolve<NT, 0>'s arithmetic in the same order (shell sums as FMAs in shell order, then the
+// same tail), so every cell's result is the same bit for bit, with the cell's prologue, shell loop
+// and tail as separate phases: 1.74 ms at 256^3 x 40 shells against 1.82 ms.  Measured and not
+// kept: two adjacent cells per thread with float2 shell loads (1.89 ms with 4 shells per batch,
+// 1.97 ms with 8: the accumulators spill), a bucket lookup of the kappa splines' interval instead
+// of the binary search (no change), and the kappa helpers inlined (1.79 ms).  Alone, the FP64
+// tail (rates, Euler step, T_S) takes 1.08 ms and the shell loop with a trivial tail 1.26 ms.
+template <int NT, int CPT>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3)))
+k_ts_evolve_g(const TsDev* __restrict__ T, TsEvolve a)
+{
+#pragma clang fp contract(off)
+  using VecF = typename std::conditional<CPT == 2, float2, float>::type;
+  extern __shared__ double fl[];  // [6][XNH][F], bs[F]
+  const int N = a.N, H = N / 2 + 1, F = T->F;
+  const long n = (long)N * N * N;
+  for (int i = threadIdx.x; i < 6 * XNH * F; i += NT) {
+    const int k = i / (XNH * F), m = (i / F) % XNH, r = i % F;
+    fl[i] = T->fold[k][m][r];
+  }
+  for (int r = threadIdx.x; r < F; r += NT)
+    fl[6 * XNH * F + r] = T->bs[r];
+  __syncthreads();
+  const double* __restrict__ bsl = fl + 6 * XNH * F;
+  const float* __restrict__ sm_sfr = a.sm;
+  const float* __restrict__ sm_xr = a.sm + (long)F * n;
+  const int no_light = T->no_light;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // CPT cells share a row (N is even), so their padded indices are adjacent too
+  for (long ir0 = ((long)blockIdx.x * NT + threadIdx.x) * CPT; ir0 < n; ir0 += (long)gridDim.x * NT * CPT) {
+    const long ip0 = (ir0 / N) * 2 * H + ir0 % N;
+    float dx[CPT], xe0[CPT], tk[CPT];
+    int m[CPT];
+    double dxl[CPT];
+    double hs[CPT], hb[CPT], is[CPT], ib[CPT], ls[CPT], lb[CPT], st[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+      dx[c] = a.deltax[ip0 + c];
+      xe0[c] = a.xe[ip0 + c];
+      tk[c] = a.Tk[ir0 + c];
+      // table interpolation at the cell's x_e (:1190-1287)
+      double xcall = xe0[c];
+      const double xtop = (double)T->xint[XNH - 1] * 0.999;
+      if (xcall > xtop)
+        xcall = xtop;
+      else if (xcall < (double)T->xint[0])
+        xcall = 1.001 * (double)T->xint[0];
+      const float xcf = (float)xcall;
+      int mm = XNH - 1;
+      while (mm > 0 && xcf < T->xint[mm])
+        mm--;
+      m[c] = mm;
+      dxl[c] = xcall - (double)T->xint[mm];
+      acc[7] += (double)sm_xr[ir0 + c] * T->cx;  // <L_X> of the unfiltered shell (:603, 677-682)
+      hs[c] = hb[c] = is[c] = ib[c] = ls[c] = lb[c] = st[c] = 0.0;
+    }
+    if (!no_light) {
+      constexpr int SB = 8;
+      auto add = [&](int c, int r, double xr, double sf) {
+        const double* __restrict__ f = fl + m[c] * F + r;
+        hs[c] = fma(xr, f[0 * XNH * F], hs[c]);
+        hb[c] = fma(xr, f[1 * XNH * F], hb[c]);
+        is[c] = fma(xr, f[2 * XNH * F], is[c]);
+        ib[c] = fma(xr, f[3 * XNH * F], ib[c]);
+        ls[c] = fma(xr, f[4 * XNH * F], ls[c]);
+        lb[c] = fma(xr, f[5 * XNH * F], lb[c]);
+        st[c] = fma(sf, bsl[r], st[c]);
+      };
+      auto add_v = [&](int r, VecF xv, VecF sv) {
+        if constexpr (CPT == 2) {
+          add(0, r, (double)xv.x, (double)sv.x);
+          add(1, r, (double)xv.y, (double)sv.y);
+        } else {
+          add(0, r, (double)xv, (double)sv);
+        }
+      };
+      int r = 0;
+      for (; r + SB <= F; r += SB) {
+        VecF xv[SB], sv[SB];
+#pragma unroll
+        for (int u = 0; u < SB; u++) {
+          xv[u] = *reinterpret_cast<const VecF*>(sm_xr + (long)(r + u) * n + ir0);
+          sv[u] = *reinterpret_cast<const VecF*>(sm_sfr + (long)(r + u) * n + ir0);
+        }
+#pragma unroll
+        for (int u = 0; u < SB; u++)
+          add_v(r + u, xv[u], sv[u]);
+      }
+      for (; r < F; r++)
+        add_v(r, *reinterpret_cast<const VecF*>(sm_xr + (long)r * n + ir0),
+              *reinterpret_cast<const VecF*>(sm_sfr + (long)r * n + ir0));
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+      const long ir = ir0 + c, ip = ip0 + c;
+      const double x_e = xe0[c], Td = tk[c];
+      const double n_b = (T->N_b0 * T->nb3) * (double)__fadd_rn(1.0f, dx[c]);
+      const double dheat = (hs[c] * dxl[c] + hb[c]) * T->prefac_gal;
+      const double dion = (is[c] * dxl[c] + ib[c]) * T->prefac_gal;
+      const double dlya = (ls[c] * dxl[c] + lb[c]) * (T->prefac_gal * n_b);
+      const double dstar = st[c] * T->conv;
+      const double sink = alpha_A(Td) * CLUMPING_FACTOR * x_e * x_e * T->f_H * n_b;
+      const double dxe_dzp = T->dt_dzp * (dion - sink);
+      const double i1x = 1.0 / (1.0 + x_e);
+      double dadia = T->c_adia;
+      if (fabs((double)dx[c]) > FRACT_FLOAT_ERR)
+        dadia += T->dgrowth * (double)dx[c] / (1.0 + T->growth * (double)dx[c]);
+      dadia *= (2.0 / 3.0) * Td;
+      const double dspec = -dxe_dzp * Td * i1x;
+      const double dcomp = T->c_comp * (x_e / ((1.0 + x_e) + T->f_He)) * (T->Trad - Td);
+      const double dxheat_dzp = dheat * T->c_heat * i1x;
+      const double d1 = ((dxheat_dzp + dcomp) + dspec) + dadia;
+      const double d2 = dlya + dstar;
+      const double d4 = T->dt_dzp * dion;
+      // Euler step and floors (:1345-1377)
+      float xe1 = (float)((double)xe0[c] + dxe_dzp * T->dzp);
+      if (xe1 > 1)
+        xe1 = (float)(1 - FRACT_FLOAT_ERR);
+      else if (xe1 < 0)
+        xe1 = 0;
+      float t1 = tk[c];
+      if (t1 < MAX_TK)
+        t1 = (float)((double)t1 + d1 * T->dzp);
+      if (t1 < MIN_TK)
+        t1 = (float)(TCMB * (1 + T->zp));
+      float xa;
+      const float ts = get_Ts(T, dx[c], t1, xe1, (float)d2, &xa);
+      a.xe[ip] = xe1;
+      a.Tk[ir] = t1;
+      a.TS[ir] = ts;
+      acc[0] += d2;
+      acc[1] += (double)xa;
+      acc[2] += dxheat_dzp;
+      acc[3] += d4;
+      acc[4] += (double)ts;
+      acc[5] += (double)t1;
+      acc[6] += (double)xe1;
+    }
+  }
+  block_sum<NT, 8>(acc, a.partials + (long)blockIdx.x * 8);
 }
 
 // homogeneous branch (ComputeTs.c:252-287) with the final averages
@@ -827,7 +972,7 @@ static int ts_run(Ctx* c, const mxr_ts_params* p, const mxr_ts_grids* g, mxr_ts_
     RC(T(K_TS_EVOLVE,
          (double)nreal * (4 + 8 + 8 + 4 + (hd->no_light ? 4.0 * (1 + NB) : 4.0 * G * F)), [&] {
            if (NB == 0) {
-             hipLaunchKernelGGL((k_ts_evolve<NT, 0>), dim3(nblk_ev), dim3(NT), lds, c->stream, dd, ea);
+             hipLaunchKernelGGL((k_ts_evolve_g<NT, 1>), dim3(nblk_ev), dim3(NT), lds, c->stream, dd, ea);
            } else {
              int lmax = 0;
              MXR_HIP(hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
