@@ -243,8 +243,10 @@ __device__ __noinline__ double kappa_tab(const TsDev* __restrict__ T, int s, dou
 // (:1973-2008).  z is the (float) redshift of the call, so Trad and (1+z)^3 come from the block.
 // The fixed point is iterated on 1/T_S: with u = 1/T_S the reference's update
 //   T_S' = (1 + xa + xc) / (1/Trad + xa / Tc_eff + xc / T_K),  1/Tc_eff = 1/T_K + 0.405535/T_K (u - 1/T_K)
-// and S_alpha = (A + B u) / D need no division but the one per step, and its stopping test
-// |T_S' - T_S| / T_S' > 1e-3 reads |u - u'| > 1e-3 u.
+// and S_alpha = (A + B u) / D need no division but the one per step.  Its loop condition
+// |T_S' - T_S| / T_S' > 1e-3 is sign(u') |u - u'| / u > 1e-3 (u > 0 throughout): it goes on while
+// u' > 0 and |u - u'| > 1e-3 u, and stops at the first T_S' <= 0 (whose absolute value, or an
+// infinity at u' = 0, the reference returns) -- in very cold cells the update can cross zero.
 __device__ __noinline__ float get_Ts(const TsDev* __restrict__ T, float delta, float TKf, float xef, float Jalpha,
                                      float* curr_xalpha)
 {
@@ -273,7 +275,7 @@ __device__ __noinline__ float get_Ts(const TsDev* __restrict__ T, float delta, f
       xa = ka * (A + B * u);
       const double iTce = iTK + 0.405535 * iTK * (u - iTK);
       const double un = (base + xa * iTce) / ((1.0 + xa) + xc);
-      const bool more = fabs(u - un) > 1.0e-3 * u;
+      const bool more = un > 0.0 && fabs(u - un) > 1.0e-3 * u;
       u = un;
       if (!more)
         break;
