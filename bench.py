@@ -25,7 +25,6 @@ import argparse
 import json
 import os
 import sys
-import tempfile
 import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
@@ -540,7 +539,9 @@ def main():
                     help="run just this leg and print its JSON (for profiling one component)")
     ap.add_argument("--autotune", choices=("auto", "always", "never"), default="auto",
                     help="launch shapes: wisdom entry if present else autotune (auto), always tune, or defaults")
-    ap.add_argument("--wisdom", default=os.path.join(tempfile.gettempdir(), "mxr-wisdom.json"))
+    # the launch shapes measured on one MI355X (mxr_autotune, stored by --autotune always); the
+    # committed profiles were taken with them, so a default run uses the same kernel forms
+    ap.add_argument("--wisdom", default=os.path.join(REPO, "meraxes-devs_qyxnew_amd", "wisdom", "mi355x.json"))
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
