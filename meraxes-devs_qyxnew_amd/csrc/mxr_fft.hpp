@@ -20,6 +20,27 @@ namespace mxr {
 // product with a neighbouring sum differently in different kernels (with contraction left to
 // the compiler, the same butterfly compiled into two kernels with different epilogues took
 // different FMA choices and rounded differently).
+// Single-rounding float multiply, add and subtract that FP contraction never merges with a
+// neighbouring operation.  HIP's __fmul_rn and __fadd_rn are the plain operators, which the
+// compiler fuses into an FMA when it can (x*y + z rounded once), unlike the unfused float
+// arithmetic they restate; an operation written under contract(off) keeps its own rounding
+// wherever it is inlined.
+__device__ __forceinline__ float mul_rn(float a, float b)
+{
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ float add_rn(float a, float b)
+{
+#pragma clang fp contract(off)
+  return a + b;
+}
+__device__ __forceinline__ float sub_rn(float a, float b)
+{
+#pragma clang fp contract(off)
+  return a - b;
+}
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b)
 {
 #pragma clang fp contract(off)

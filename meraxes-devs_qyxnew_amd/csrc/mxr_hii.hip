@@ -106,7 +106,7 @@ __device__ __noinline__ float fully_ionized_T(float z_re, float z, float delta, 
 // T_HI * (1 + cT delta) with the reference's promotions: float product, double sum
 __device__ __forceinline__ float thermal_T(float TK, float cT, float dx)
 {
-  return (float)((double)TK * (1. + (double)__fmul_rn(cT, dx)));
+  return (float)((double)TK * (1. + (double)mul_rn(cT, dx)));
 }
 
 
@@ -152,9 +152,12 @@ __device__ __forceinline__ bool hii_criterion(const HiiEpi& ea, float df, float&
 // J_21 at a first crossing with UVBFlag 1 (:531-542): the stars term, plus the BH term
 __device__ __forceinline__ float j21_at_crossing(const HiiEpi& ea, float sf, float bf)
 {
-  float j = __fmul_rn(sf, ea.J21c);
+  // two rounded products and a rounded sum: with contraction on, the compiler may fuse either
+  // product into the add, differently in each kernel, and the paths' values would differ
+#pragma clang fp contract(off)
+  float j = mul_rn(sf, ea.J21c);
   if (ea.bh)
-    j = __fadd_rn(j, __fmul_rn(bf, ea.J21c_bh));
+    j = add_rn(j, mul_rn(bf, ea.J21c_bh));
   return j;
 }
 
@@ -338,6 +341,9 @@ __device__ __forceinline__ void row_c2r_body(CGridPtrs in, int ngrid, long nrows
   } else {
     if (!valid)
       return;
+    HiiEpi ek = ea;  // the grid set fixes x_e and the BH terms
+    ek.spin = SPIN;
+    ek.bh = BH;
 #pragma unroll
     for (int q = 0; q < SL::QL; q++) {
       const int j = t + T * q;
@@ -351,7 +357,7 @@ __device__ __forceinline__ void row_c2r_body(CGridPtrs in, int ngrid, long nrows
           const int ir = (int)row * N + 2 * n;  // N^3 < 2^31 for N <= 1024
           const int ip = (int)row * 2 * H + 2 * n;
           const unsigned f = flags[s / 2] >> (16 * (s % 2));
-          hii_pair(ea, ir, ip, res[0][s], res[1][s], xe, bf, make_uchar2(f & 0xffu, (f >> 8) & 0xffu));
+          hii_pair(ek, ir, ip, res[0][s], res[1][s], xe, bf, make_uchar2(f & 0xffu, (f >> 8) & 0xffu));
         }
     }
   }
@@ -437,12 +443,18 @@ __device__ __forceinline__ void direct_sum(const float2* cf, const float* C, con
   o[1] = lane0 ? dA : sA + sB;
   o[2] = lane0 ? a02 - b13 : dA - dB;
   o[3] = lane0 ? a02 + b13 : dA + dB;
+  // the sums are complete here, in program order.  Without this the compiler sinks a grid's
+  // multiply-adds past the next row's LDS commit while its coefficient reads must stay ahead of
+  // that store, so every read of the row stayed live across it (KMAX 24 needed 183 VGPRs and
+  // spilled; with the sums pinned it takes 111, with nothing spilled up to KMAX 64)
+  asm volatile("" ::"v"(o[0]), "v"(o[1]), "v"(o[2]), "v"(o[3]) : "memory");
 }
 
 template <int N, int KMAX, int EPI>
-// four waves per SIMD (128 VGPRs) for the find_HII epilogue with up to 16 modes; the x_e / BH
-// epilogues keep their registers (they would spill)
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KMAX <= 16 && EPI == C2R_HII ? 4 : 1)))
+// waves per SIMD by the registers each form needs with its sums pinned (at most 128 VGPRs for
+// four waves, 168 for three): the find_HII epilogue fits four up to 32 modes and three up to 48;
+// the x_e / BH epilogues, a few registers larger, four up to 24 and three up to 40
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KMAX <= (EPI == C2R_HII ? 32 : 24) ? 4 : KMAX <= (EPI == C2R_HII ? 48 : 40) ? 3 : 2)))
 k_row_direct(CGridPtrs in, long nrows, int rpb, const float2* __restrict__ twg, int kc, HiiEpi ea)
 {
   constexpr int H = N / 2 + 1, Q = N / 4, SEGS = Q / 64;
@@ -508,6 +520,12 @@ k_row_direct(CGridPtrs in, long nrows, int rpb, const float2* __restrict__ twg, 
     }
     SyncWave::sync();
   };
+  // the epilogue's flags as compile-time facts: this form serves filtered radii only, never the
+  // last (unfiltered) one, and its grid set fixes x_e and the BH terms
+  HiiEpi ek = ea;
+  ek.last = 0;
+  ek.spin = SPIN;
+  ek.bh = BH;
   float2 pre[NL];
   if (row0 < row1) {
     fetch(row0, pre);
@@ -560,7 +578,7 @@ k_row_direct(CGridPtrs in, long nrows, int rpb, const float2* __restrict__ twg, 
     }
 #pragma unroll
     for (int e = 0; e < 4; e++)
-      hii_cell(ea, (int)row * N + n[e], (int)row * 2 * H + n[e], dx[e], st[e], xe[e], bf[e], fl[e]);
+      hii_cell(ek, (int)row * N + n[e], (int)row * 2 * H + n[e], dx[e], st[e], xe[e], bf[e], fl[e]);
 #pragma unroll
     for (int e = 0; e < 4; e++)
       fl[e] = nfl[e];
@@ -601,7 +619,7 @@ __global__ void k_filter(float2* __restrict__ box, int local_ix_start, int slab_
     } else {
       const float w = window_mul(nx, ny, nz, fa);
       const float2 x = box[idx];
-      box[idx] = make_float2(__fmul_rn(x.x, w), __fmul_rn(x.y, w));
+      box[idx] = make_float2(mul_rn(x.x, w), mul_rn(x.y, w));
     }
   }
 }
@@ -662,9 +680,9 @@ __global__ void k_criterion(PlainArgs pa, HiiEpi ea)
       // first crossing in this call (largest R) with recombinations: the ionising background,
       // the radius and, for UVBFlag 2, J_21 (:488-506)
       if (ea.rec && xh > REL_TOL) {
-        float g12 = __fmul_rn(pa.wfl[iw], ea.gpref);
+        float g12 = mul_rn(pa.wfl[iw], ea.gpref);
         if (ea.bh)
-          g12 = __fadd_rn(g12, __fmul_rn(pa.bafl[iw], ea.gpref_bh));
+          g12 = add_rn(g12, mul_rn(pa.bafl[iw], ea.gpref_bh));
         ea.Gamma12[ir] = g12;
         ea.r_bubble[ir] = ea.R;
         if (ea.uvb == 2)
@@ -680,7 +698,7 @@ __global__ void k_criterion(PlainArgs pa, HiiEpi ea)
       else if (xh >= 1.f)
         tp = T_HI;
       else
-        tp = (float)((double)__fmul_rn(T_HI, xh) + T_RE * (1. - (double)xh));
+        tp = (float)((double)mul_rn(T_HI, xh) + T_RE * (1. - (double)xh));
       ea.temp[ir] = tp;
       xh = (float)(neutral_fraction - lhs);
       if (xh < 0.0f)
@@ -739,7 +757,7 @@ __global__ void __launch_bounds__(NT) k_postloop(int N, HiiEpi ea)
       const float g12 = ea.Gamma12[ir];
       double rate, rnh, cf;
       splined_recombination(ea.rt, z_eff, (double)g12 * h * h, T, &rate, &rnh, &cf);
-      const float step = __fmul_rn(__fmul_rn((float)rate, ea.fabs_dtdz), ea.zstep);
+      const float step = mul_rn(mul_rn((float)rate, ea.fabs_dtdz), ea.zstep);
       const float nrec = (float)((double)ea.N_rec[ip] + (double)step * (1. - (double)(float)xh));
       ea.N_rec[ip] = nrec;
       const float res = (float)rnh, clump = (float)cf;
@@ -1146,6 +1164,10 @@ struct Hii : Passes<N> {
     if constexpr (N % 256 != 0 || N / 2 + 1 <= 64) {
       return MXR_ERR_UNSUPPORTED;
     } else {
+      if (ea.last) {  // the kernel compiles the last radius's epilogue out
+        set_error("direct z pass called for the last (unfiltered) radius");
+        return MXR_ERR_UNSUPPORTED;
+      }
       const long nrows = (long)N * N;
       constexpr int SEGS = N / 4 / 64;
       const dim3 grid((unsigned)(SEGS * ((nrows + rpb - 1) / rpb)));

@@ -109,16 +109,16 @@ __device__ __forceinline__ float sqrt_rn(float x)
 __device__ __forceinline__ float kmag_ref(int nx, int ny, int nz, int N, float dk)
 {
   const int mid = N / 2;
-  const float kx = __fmul_rn((float)(nx > mid ? nx - N : nx), dk);
-  const float ky = __fmul_rn((float)(ny > mid ? ny - N : ny), dk);
-  const float kz = __fmul_rn((float)nz, dk);
-  return sqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz, kz)));
+  const float kx = mul_rn((float)(nx > mid ? nx - N : nx), dk);
+  const float ky = mul_rn((float)(ny > mid ? ny - N : ny), dk);
+  const float kz = mul_rn((float)nz, dk);
+  return sqrt_rn(add_rn(add_rn(mul_rn(kx, kx), mul_rn(ky, ky)), mul_rn(kz, kz)));
 }
 
 // sharp-k (type 1): kR *= 0.413566994 (double product, float result); zero if kR > 1
 __device__ __forceinline__ bool sharpk_keep(int nx, int ny, int nz, const FiltArgs& fa)
 {
-  const float kR = __fmul_rn(kmag_ref(nx, ny, nz, fa.N, fa.dk), fa.R);
+  const float kR = mul_rn(kmag_ref(nx, ny, nz, fa.N, fa.dk), fa.R);
   const float s = (float)((double)kR * 0.413566994);
   return !(s > 1.0f);
 }
@@ -146,17 +146,17 @@ __device__ __forceinline__ int kept_extent(Keep keep, int N)
 // serves every kernel, so the fused and plain paths evaluate identical code
 static __device__ __noinline__ float window_mul(int nx, int ny, int nz, const FiltArgs& fa)
 {
-  float kR = __fmul_rn(kmag_ref(nx, ny, nz, fa.N, fa.dk), fa.R);
+  float kR = mul_rn(kmag_ref(nx, ny, nz, fa.N, fa.dk), fa.R);
   if (fa.ftype == 0) {
     if ((double)kR > 1e-4) {
       const float a = __fdiv_rn(sinf(kR), powf(kR, 3.0f));
       const float b = __fdiv_rn(cosf(kR), powf(kR, 2.0f));
-      return (float)(3.0 * (double)__fsub_rn(a, b));
+      return (float)(3.0 * (double)sub_rn(a, b));
     }
     return 1.0f;
   }
   kR = (float)((double)kR * 0.643);
-  return powf(2.71828182845904523536f, (float)((double)__fmul_rn(-kR, kR) / 2.0));
+  return powf(2.71828182845904523536f, (float)((double)mul_rn(-kR, kR) / 2.0));
 }
 
 // ---- column pass (x: columns = (j,k) flattened, elements i; y: columns = (i,k), elements j)
@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(NT) k_col(CGridPtrs in, GridPtrs out, ColGeom 
           if (fa.apply && fa.ftype != 1 && !fa.wtab && in_col) {
             const float w = window_mul(e, fj[i], fk[i], fa);
             // rounded on its own (no fusion into the first butterfly), as filter() does
-            x = make_float2(__fmul_rn(x.x, w), __fmul_rn(x.y, w));
+            x = make_float2(mul_rn(x.x, w), mul_rn(x.y, w));
           }
         }
         v[i][q * SL::R0 + r] = keep ? x : make_float2(0.f, 0.f);
@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(NT) k_col(CGridPtrs in, GridPtrs out, ColGeom 
             float2& x = v[i][q * SL::R0 + r];
             const float wv = w[q * SL::R0 + r];
             if (in_col)
-              x = make_float2(__fmul_rn(x.x, wv), __fmul_rn(x.y, wv));
+              x = make_float2(mul_rn(x.x, wv), mul_rn(x.y, wv));
           }
         }
       }

@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(PS_NT) k_ps_bin(PsArgs a)
   auto row_k = [&] {
     const float kx = (float)(nx > N / 2 ? nx - N : nx) * a.delta_k;
     const float ky = (float)(ny > N / 2 ? ny - N : ny) * a.delta_k;
-    kxy2 = __fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky));
+    kxy2 = add_rn(mul_rn(kx, kx), mul_rn(ky, ky));
   };
   auto step = [&] {
     nz += 64;
@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(PS_NT) k_ps_bin(PsArgs a)
         // the correctly rounded float square root: rounding sqrt to double (53 bits) and then to
         // float (24 bits) equals rounding it to float once, as 53 >= 2 x 24 + 2
         const float kz = (float)nz * a.delta_k;
-        const float k2 = __fadd_rn(kxy2, __fmul_rn(kz, kz));
+        const float k2 = add_rn(kxy2, mul_rn(kz, kz));
         const float k_mag = sqrt_rn(k2);
         // bin i: edge[i] <= k < edge[i + 1].  The guess 1 + floor(log_1.35(k / delta_k)) is within
         // one bin of i (the float ceilings drift from delta_k 1.35^j by under 3e-6 relative), so

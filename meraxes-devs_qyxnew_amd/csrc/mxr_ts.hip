@@ -89,9 +89,9 @@ __global__ void k_ts_source(TsOpsArgs oa, TsSrcArgs sa, long hstride, long n)
             if (g < 2)
               v[g] = (float)((double)v[g] * o.wd);
           } else if (o.kind == TS_OP_LOAD) {
-            v[g] = __fmul_rn(hv[u][g], o.wf);
+            v[g] = mul_rn(hv[u][g], o.wf);
           } else {
-            v[g] = __fadd_rn(v[g], __fmul_rn(hv[u][g], o.wf));
+            v[g] = add_rn(v[g], mul_rn(hv[u][g], o.wf));
           }
         }
       }
@@ -450,7 +450,7 @@ k_ts_evolve(const TsDev* __restrict__ T, TsEvolve a)
     }
     const double x_e = xe0, Td = tk;
     const double zpd = (double)T->zpf;
-    const double n_b = (T->N_b0 * T->nb3) * (double)__fadd_rn(1.0f, dx);
+    const double n_b = (T->N_b0 * T->nb3) * (double)add_rn(1.0f, dx);
     const double dheat = (hs * dxl + hb) * T->prefac_gal;
     double dion = (is * dxl + ib) * T->prefac_gal;
     double dlya = (ls * dxl + lb) * (T->prefac_gal * n_b);
@@ -605,7 +605,7 @@ k_ts_evolve_g(const TsDev* __restrict__ T, TsEvolve a)
     for (int c = 0; c < CPT; c++) {
       const long ir = ir0 + c, ip = ip0 + c;
       const double x_e = xe0[c], Td = tk[c];
-      const double n_b = (T->N_b0 * T->nb3) * (double)__fadd_rn(1.0f, dx[c]);
+      const double n_b = (T->N_b0 * T->nb3) * (double)add_rn(1.0f, dx[c]);
       const double dheat = (hs[c] * dxl[c] + hb[c]) * T->prefac_gal;
       const double dion = (is[c] * dxl[c] + ib[c]) * T->prefac_gal;
       const double dlya = (ls[c] * dxl[c] + lb[c]) * (T->prefac_gal * n_b);
@@ -663,7 +663,7 @@ __global__ void __launch_bounds__(NT) k_ts_homog(const TsDev* __restrict__ T, Ts
     const long ip = (ir / N) * 2 * H + ir % N;
     const float dx = a.deltax[ip];
     const float xe = T->xe_h;
-    const float tk = (float)((double)T->TK_h * (1.0 + (double)__fmul_rn(T->cT_ad, dx)));
+    const float tk = (float)((double)T->TK_h * (1.0 + (double)mul_rn(T->cT_ad, dx)));
     float xa;
     const float ts = get_Ts(T, dx, tk, xe, 0.0f, &xa);
     a.xe[ip] = xe;

@@ -283,6 +283,24 @@ def test_filter_entry_point(ctxs, ftype):
         np.testing.assert_allclose(got, ref, rtol=2e-6, atol=2e-7)
 
 
+def test_sharp_k_edge_matches_oracle(ctxs):
+    """A radius on the sharp-k edge of mode (0, 1, 10) at 64^3, 67.8 cMpc/h: the separately rounded
+    |k| (the reference's and the oracle's) keeps it and an FMA-contracted one drops it.  The
+    filter entry point matches the oracle bit for bit (its sharp-k test is the one the fused
+    passes' window and kept extents evaluate)."""
+    from test_oracle_filter import EDGE_BOX, EDGE_MODE, EDGE_N, EDGE_R
+
+    N, H = EDGE_N, EDGE_N // 2 + 1
+    r = np.random.default_rng(21)
+    spec = (r.standard_normal(N * N * H) + 1j * r.standard_normal(N * N * H)).astype(np.complex64)
+    ref = O.filter_box(spec, N, EDGE_BOX, EDGE_R, 1)
+    i, j, k = EDGE_MODE
+    assert ref.reshape(N, N, H)[i, j, k] != 0
+    got = spec.copy()
+    ctxs(N).filter(got, 0, N, N, EDGE_BOX, EDGE_R, 1)
+    np.testing.assert_array_equal(got, ref)
+
+
 def test_brightness_temperature(ctxs):
     N = 64
     r = np.random.default_rng(5)

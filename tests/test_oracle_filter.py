@@ -71,3 +71,37 @@ def test_undefined_filter_type():
     with pytest.raises(mxr.MxrError) as e:
         window(3, 1.0)
     assert e.value.code == mxr.MXR_ERR_FILTER
+
+
+# a radius on the sharp-k edge of one mode at N = 64, 67.8 cMpc/h: found by a search over the
+# modes whose float |k| differs between the reference's separately rounded sum
+# (kx^2 + ky^2) + kz^2 and the same sum contracted into FMAs, fma(kz, kz, fma(kx, kx, ky^2))
+EDGE_N, EDGE_BOX, EDGE_R, EDGE_MODE = 64, 67.8, 2.596230983734131, (0, 1, 10)
+
+
+def _f32_kmag(mode, N, box, fused):
+    f32 = np.float32
+    dk = f32(2 * np.pi / f32(box))
+    kx, ky, kz = (f32(f32(n - N if n > N // 2 else n) * dk) for n in mode)
+    if fused:
+        s = f32(np.float64(kz) * kz + np.float64(f32(np.float64(kx) * kx + np.float64(f32(ky * ky)))))
+    else:
+        s = f32(f32(f32(kx * kx) + f32(ky * ky)) + f32(kz * kz))
+    return f32(np.sqrt(np.float64(s)))
+
+
+def _keep(kmag, R):
+    kR = np.float32(kmag * np.float32(R))
+    return not (np.float32(np.float64(kR) * 0.413566994) > np.float32(1.0))
+
+
+def test_sharp_k_edge_rounds_unfused():
+    """The oracle keeps the edge mode, as the separately rounded |k| does; the FMA-contracted
+    |k| would drop it, so this radius tells the two roundings apart."""
+    assert _keep(_f32_kmag(EDGE_MODE, EDGE_N, EDGE_BOX, False), EDGE_R)
+    assert not _keep(_f32_kmag(EDGE_MODE, EDGE_N, EDGE_BOX, True), EDGE_R)
+    H = EDGE_N // 2 + 1
+    ones = np.ones(EDGE_N * EDGE_N * H, np.complex64)
+    w = O.filter_box(ones, EDGE_N, EDGE_BOX, EDGE_R, 1).real.reshape(EDGE_N, EDGE_N, H)
+    i, j, k = EDGE_MODE
+    assert w[i, j, k] == 1.0 and w[i, EDGE_N - j, k] == 1.0
