@@ -236,7 +236,9 @@ __device__ __forceinline__ void hii_pair(const HiiEpi& ea, int ir, int ip, float
 // C2R_HII keeps the filtered deltax and stars (and x_e) in registers for the update;
 // weighted_sfr is transformed and clamped like the reference does (find_HII_bubbles.c:382-385)
 // but, with recombinations off, nothing reads it afterwards.
-template <int N, int NT, int TR, int EPI>
+// LAST: this launch is the last (unfiltered) radius; the update's last-radius work (partial xH,
+// T_HI, r_bubble) is compiled only into that instance.
+template <int N, int NT, int TR, int EPI, int LAST>
 __device__ __forceinline__ void row_c2r_body(CGridPtrs in, int ngrid, long nrows, const float2* __restrict__ twg,
                                              FiltArgs fa, const HiiEpi& ea)
 {
@@ -341,9 +343,10 @@ __device__ __forceinline__ void row_c2r_body(CGridPtrs in, int ngrid, long nrows
   } else {
     if (!valid)
       return;
-    HiiEpi ek = ea;  // the grid set fixes x_e and the BH terms
+    HiiEpi ek = ea;  // the grid set fixes x_e and the BH terms, the launch the last radius
     ek.spin = SPIN;
     ek.bh = BH;
+    ek.last = LAST;
 #pragma unroll
     for (int q = 0; q < SL::QL; q++) {
       const int j = t + T * q;
@@ -365,17 +368,17 @@ __device__ __forceinline__ void row_c2r_body(CGridPtrs in, int ngrid, long nrows
 
 // Three waves per SIMD for the store and find_HII epilogues (their registers fit without
 // spilling); the epilogues with x_e or the BH grids keep their registers.
-template <int N, int NT, int TR, int EPI>
+template <int N, int NT, int TR, int EPI, int LAST>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3)))
 k_row_c2r(CGridPtrs in, int ngrid, long nrows, const float2* __restrict__ twg, FiltArgs fa, HiiEpi ea)
 {
-  row_c2r_body<N, NT, TR, EPI>(in, ngrid, nrows, twg, fa, ea);
+  row_c2r_body<N, NT, TR, EPI, LAST>(in, ngrid, nrows, twg, fa, ea);
 }
-template <int N, int NT, int TR, int EPI>
+template <int N, int NT, int TR, int EPI, int LAST>
 __global__ void __launch_bounds__(NT) k_row_c2r_spin(CGridPtrs in, int ngrid, long nrows,
                                                      const float2* __restrict__ twg, FiltArgs fa, HiiEpi ea)
 {
-  row_c2r_body<N, NT, TR, EPI>(in, ngrid, nrows, twg, fa, ea);
+  row_c2r_body<N, NT, TR, EPI, LAST>(in, ngrid, nrows, twg, fa, ea);
 }
 
 // One cell of the fused update (the per-cell form of hii_pair, same order of operations).
@@ -1147,12 +1150,24 @@ struct Hii : Passes<N> {
   {
     const long nrows = (long)N * N;
     dim3 grid((unsigned)((nrows + C::TR - 1) / C::TR), (unsigned)(EPI == C2R_STORE ? ngrid : 1));
-    if constexpr (epi_spin(EPI) || epi_bh(EPI))
-      hipLaunchKernelGGL((k_row_c2r_spin<N, C::ROW_NT, C::TR, EPI>), grid, dim3(C::ROW_NT), 0, c->stream, in, ngrid,
+    if constexpr (epi_spin(EPI) || epi_bh(EPI)) {
+      if (ea.last)
+        hipLaunchKernelGGL((k_row_c2r_spin<N, C::ROW_NT, C::TR, EPI, 1>), grid, dim3(C::ROW_NT), 0, c->stream, in,
+                           ngrid, nrows, c->tw, fa, ea);
+      else
+        hipLaunchKernelGGL((k_row_c2r_spin<N, C::ROW_NT, C::TR, EPI, 0>), grid, dim3(C::ROW_NT), 0, c->stream, in,
+                           ngrid, nrows, c->tw, fa, ea);
+    } else if constexpr (EPI == C2R_STORE) {
+      hipLaunchKernelGGL((k_row_c2r<N, C::ROW_NT, C::TR, EPI, 0>), grid, dim3(C::ROW_NT), 0, c->stream, in, ngrid,
                          nrows, c->tw, fa, ea);
-    else
-      hipLaunchKernelGGL((k_row_c2r<N, C::ROW_NT, C::TR, EPI>), grid, dim3(C::ROW_NT), 0, c->stream, in, ngrid, nrows,
-                         c->tw, fa, ea);
+    } else {
+      if (ea.last)
+        hipLaunchKernelGGL((k_row_c2r<N, C::ROW_NT, C::TR, EPI, 1>), grid, dim3(C::ROW_NT), 0, c->stream, in, ngrid,
+                           nrows, c->tw, fa, ea);
+      else
+        hipLaunchKernelGGL((k_row_c2r<N, C::ROW_NT, C::TR, EPI, 0>), grid, dim3(C::ROW_NT), 0, c->stream, in, ngrid,
+                           nrows, c->tw, fa, ea);
+    }
     MXR_HIP(hipGetLastError());
     return MXR_OK;
   }
