@@ -490,7 +490,7 @@ int autotune(Ctx* c, const mxr_hii_params* p, const mxr_hii_grids* g, mxr_tuning
   std::vector<Knob> knobs;
   const bool sharpk = p->ReionFilterType == 1;
   if (sharpk && N % 256 == 0) {
-    knobs.push_back({0, {0, 8, 16, 24, 32, 48, 64}});
+    knobs.push_back({0, {0, 8, 16, 24, 32, 40, 48, 56, 64}});
     knobs.push_back({1, {1, 2, 4, 8, 16, 32}});
   }
   knobs.push_back({4, {4, 8, 16, 32}});
