@@ -73,3 +73,18 @@ def test_invalid_tuning_is_not_stored(tmp_path):
             mxr.wisdom_store(p, "gfx950", 256, 512, 1, 0, _t(**bad))
         assert e.value.code == mxr.MXR_ERR_INVALID_ARG
     assert mxr.wisdom_lookup(p, "gfx950", 256, 512, 1) is None
+
+
+def test_shipped_mi355x_wisdom():
+    """The package's MI355X entries (bench.py's default wisdom file) parse through the C ABI and
+    hold valid launch shapes for the benchmark's 512^3 sharp-k call and the batch's sizes."""
+    import os
+
+    path = os.path.join(os.path.dirname(mxr.__file__), "wisdom", "mi355x.json")
+    for n in (512, 96, 128, 160):
+        t = mxr.wisdom_lookup(path, "gfx950", 256, n, 1)
+        assert t is not None, n
+        d = t.as_dict()
+        assert 0 <= d["zdirect_kmax"] <= 64 and 1 <= d["zdirect_rpb"] <= 4096, d
+        assert d["col_tc_x"] in (4, 8, 16, 32) and d["col_tc_y"] in (4, 8, 16, 32), d
+    assert mxr.wisdom_lookup(path, "gfx950", 256, 512, 1).zdirect_kmax == 32
