@@ -348,10 +348,12 @@ void mxr_destroy(mxr_ctx* h)
   (void)hipFree(c.bg_ws);
   if (c.bg_ev_in)
     (void)hipEventDestroy(c.bg_ev_in);
-  if (c.bg_ev_shift)
-    (void)hipEventDestroy(c.bg_ev_shift);
-  if (c.bg_stream)
-    (void)hipStreamDestroy(c.bg_stream);
+  for (int k = 0; k < BG_STREAMS; k++) {
+    if (c.bg_ev_shift[k])
+      (void)hipEventDestroy(c.bg_ev_shift[k]);
+    if (c.bg_stream[k])
+      (void)hipStreamDestroy(c.bg_stream[k]);
+  }
   (void)hipFree(c.wtab);
   if (c.hires)
     mxr_destroy(c.hires);

@@ -8,8 +8,8 @@
 // sums its run of galaxies in order with the reference's arithmetic and finalises the cell's
 // values.  One wavefront per grid row then writes the row of every grid once, coalesced: the
 // occupied cells' values and zero elsewhere, which is what the reference's clear and
-// finalisation give a cell without galaxies.  The histories move up by device copies on a second
-// stream meanwhile.
+// finalisation give a cell without galaxies.  The histories move up by device copies meanwhile,
+// each history array on a stream of its own.
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_run_length_encode.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -517,33 +517,38 @@ int run_construct_baryon_grids(Ctx* c, const mxr_baryon_params* p, const mxr_gal
     }
   }
   // reionization.c:2093-2141, the history part: each history moves up by one slot, whole
-  // slots (padding included) from the top down, one device copy per slot (two streams each,
-  // which the memory system serves faster than all slots moved together).  The copies run on a
-  // second stream, beside the keys, sort and cell sums, which do not touch the grids; the row
-  // writer waits for them.  They start once the positions are known to be valid, so a rejected
+  // slots (padding included) from the top down, one device copy per slot (which the memory
+  // system serves faster than all slots moved together by one kernel).  Each history array's
+  // copies run on a stream of its own, beside each other and beside the keys, sort and cell sums,
+  // which do not touch the grids; the row writer waits for all of them.  They start once the positions are known to be valid, so a rejected
   // call leaves the grids as they were.  Slot 0's real cells are then written by the row writer, so its
   // padding keeps the old slot 0's (the reference's finalisation writes real cells only).
-  bool shifting = false;
+  int shifting = 0;  // streams the shift was started on
   if (rc == MXR_OK && a.spin && a.nstore > 1) {
     hipError_t e = hipSuccess;
-    if (!c->bg_stream)
-      e = hipStreamCreateWithFlags(&c->bg_stream, hipStreamNonBlocking);
-    if (e == hipSuccess && !c->bg_ev_in)
+    float* hs[4] = {a.h_sfr, a.h_x, a.hard ? a.h_h : nullptr, a.soft ? a.h_s : nullptr};
+    static_assert(BG_STREAMS >= 4, "one stream per history array");
+    if (!c->bg_ev_in)
       e = hipEventCreateWithFlags(&c->bg_ev_in, hipEventDisableTiming);
-    if (e == hipSuccess && !c->bg_ev_shift)
-      e = hipEventCreateWithFlags(&c->bg_ev_shift, hipEventDisableTiming);
     if (e == hipSuccess)
       e = hipEventRecord(c->bg_ev_in, c->stream);  // staged grids in place
-    if (e == hipSuccess)
-      e = hipStreamWaitEvent(c->bg_stream, c->bg_ev_in, 0);
-    shifting = e == hipSuccess;
-    float* hs[4] = {a.h_sfr, a.h_x, a.hard ? a.h_h : nullptr, a.soft ? a.h_s : nullptr};
-    for (int k = 0; k < 4 && e == hipSuccess; k++)
-      for (int sl = a.nstore - 2; hs[k] && sl >= 0 && e == hipSuccess; sl--)
+    for (int k = 0; k < 4 && e == hipSuccess; k++) {
+      if (!hs[k])
+        continue;
+      if (!c->bg_stream[k])
+        e = hipStreamCreateWithFlags(&c->bg_stream[k], hipStreamNonBlocking);
+      if (e == hipSuccess && !c->bg_ev_shift[k])
+        e = hipEventCreateWithFlags(&c->bg_ev_shift[k], hipEventDisableTiming);
+      if (e == hipSuccess)
+        e = hipStreamWaitEvent(c->bg_stream[k], c->bg_ev_in, 0);
+      if (e == hipSuccess)
+        shifting |= 1 << k;
+      for (int sl = a.nstore - 2; sl >= 0 && e == hipSuccess; sl--)
         e = hipMemcpyAsync(hs[k] + (long)(sl + 1) * npad, hs[k] + (long)sl * npad, sizeof(float) * npad,
-                           hipMemcpyDeviceToDevice, c->bg_stream);
-    if (e == hipSuccess)
-      e = hipEventRecord(c->bg_ev_shift, c->bg_stream);
+                           hipMemcpyDeviceToDevice, c->bg_stream[k]);
+      if (e == hipSuccess)
+        e = hipEventRecord(c->bg_ev_shift[k], c->bg_stream[k]);
+    }
     if (e != hipSuccess)
       rc = hip_fail(e, "history shift", __FILE__, __LINE__);
   }
@@ -580,8 +585,9 @@ int run_construct_baryon_grids(Ctx* c, const mxr_baryon_params* p, const mxr_gal
                          nrun, N, row_run);
       e = hipGetLastError();
     }
-    if (e == hipSuccess && shifting)
-      e = hipStreamWaitEvent(c->stream, c->bg_ev_shift, 0);
+    for (int k = 0; k < 4 && e == hipSuccess; k++)
+      if (shifting >> k & 1)
+        e = hipStreamWaitEvent(c->stream, c->bg_ev_shift[k], 0);
     if (e == hipSuccess) {
       hipLaunchKernelGGL(k_bg_dense, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0, c->stream, a, run_keys,
                          cell_vals, row_run);
@@ -604,8 +610,9 @@ int run_construct_baryon_grids(Ctx* c, const mxr_baryon_params* p, const mxr_gal
         *grids->history_head = ring_slot;
     }
   }
-  if (shifting)  // the copies may still run when an earlier step failed
-    (void)hipStreamSynchronize(c->bg_stream);
+  for (int k = 0; k < 4; k++)  // the copies may still run when an earlier step failed
+    if (shifting >> k & 1)
+      (void)hipStreamSynchronize(c->bg_stream[k]);
   for (int k = 0; k < 12; k++)
     (void)hipFree(tmp[k]);
   return rc;

@@ -81,6 +81,8 @@ struct KStat {
 // One grid dimension's worth of persistent device state.
 constexpr int MXR_PS_MAXB = 40;  // power-spectrum bins: 1 + log(N)/log(1.35), 21 at N = 512
 
+constexpr int BG_STREAMS = 4;  // history arrays of galaxies to grids, one copy stream each
+
 struct Ctx {
   // profiling: an event pair per timed launch, resolved after the call's final sync
   bool prof = false;
@@ -147,12 +149,13 @@ struct Ctx {
   float* wtab = nullptr;
   float wtab_dk = 0, wtab_R = 0;
   int wtab_ftype = -1;
-  // galaxies to grids: sort keys and values, the sort's storage, staged galaxy arrays; a second
-  // stream (default priority) for the history shift's copies and the events ordering it
+  // galaxies to grids: sort keys and values, the sort's storage, staged galaxy arrays; streams
+  // (default priority) for the history shift's copies, one per history array, and the events
+  // ordering them
   void* bg_ws = nullptr;
   size_t bg_ws_cap = 0;            // bytes
-  hipStream_t bg_stream = nullptr;
-  hipEvent_t bg_ev_in = nullptr, bg_ev_shift = nullptr;
+  hipStream_t bg_stream[BG_STREAMS] = {};
+  hipEvent_t bg_ev_in = nullptr, bg_ev_shift[BG_STREAMS] = {};
   double fin_total_n_cells = 0, fin_sfr_conv = 0;
   int fin_n_radii = 0;
   int fin_bh = 0;
