@@ -483,8 +483,12 @@ k_row_direct(CGridPtrs in, long nrows, int rpb, const float2* __restrict__ twg, 
     gp[s++] = in.p[1];
     gp[s++] = in.p[0];
   }
-  const int seg = (int)(blockIdx.x % SEGS);
-  const long row0 = (long)(blockIdx.x / SEGS) * rpb;
+  // XCD-aware order: the GPU deals blocks to its 8 XCDs round-robin by index, so the SEGS blocks
+  // of a row group take indices 8 apart and run on one XCD, whose L2 serves the group's
+  // coefficient rows to all of them (row groups past the last are empty)
+  const unsigned xcd = blockIdx.x % 8, kq = blockIdx.x / 8;
+  const int seg = (int)(kq % SEGS);
+  const long row0 = ((long)(kq / SEGS) * 8 + xcd) * rpb;
   const int p = seg * 64 + (int)threadIdx.x;
   const bool lane0 = p == 0;
   float C[KMAX], S[KMAX];
@@ -1185,7 +1189,8 @@ struct Hii : Passes<N> {
       }
       const long nrows = (long)N * N;
       constexpr int SEGS = N / 4 / 64;
-      const dim3 grid((unsigned)(SEGS * ((nrows + rpb - 1) / rpb)));
+      const long groups = (nrows + rpb - 1) / rpb;  // rounded up to whole sets of 8 below
+      const dim3 grid((unsigned)(SEGS * 8 * ((groups + 7) / 8)));
       switch ((kc + 7) / 8) {
 #define L(m)                                                                                                         \
   case m:                                                                                                            \
