@@ -77,11 +77,11 @@ def test_invalid_tuning_is_not_stored(tmp_path):
 
 def test_shipped_mi355x_wisdom():
     """The package's MI355X entries (bench.py's default wisdom file) parse through the C ABI and
-    hold valid launch shapes for the benchmark's 512^3 sharp-k call and the batch's sizes."""
+    hold valid launch shapes for the benchmark's 512^3 sharp-k call, 1024^3 and the batch's sizes."""
     import os
 
     path = os.path.join(os.path.dirname(mxr.__file__), "wisdom", "mi355x.json")
-    for n in (512, 96, 128, 160):
+    for n in (512, 1024, 96, 128, 160):
         t = mxr.wisdom_lookup(path, "gfx950", 256, n, 1)
         assert t is not None, n
         d = t.as_dict()
